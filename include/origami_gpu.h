@@ -1,0 +1,154 @@
+/*
+ * origami MI355X engine — C ABI.
+ *
+ * A drop-in for origami's energy-and-gradient entry point and its BAOAB /
+ * SHAKE / REMD step loop.  All inputs are resolved per-atom and per-term
+ * arrays (the content of origami's ForceScratch plus the resolved bonded
+ * lists), so the caller's structure/topology/force-field code (origami's
+ * crates/geom, chem) stays as it is.  Plain pointers and sizes only; every
+ * function returns 0 on success or a negative og_status, with a message in
+ * og_last_error().  Units: Å, kJ/mol, kJ/mol/Å, fs, K, Da, e.
+ *
+ * One og_engine holds a batch of 1..N independent systems (chains or
+ * replicas) resident on one GPU; single-system use is a batch of one.
+ * Handles are not thread-safe; distinct handles may be used concurrently.
+ */
+#ifndef ORIGAMI_GPU_H
+#define ORIGAMI_GPU_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct og_engine og_engine;
+
+enum og_status {
+    OG_OK = 0,
+    OG_ERR_ARG = -1,        /* invalid argument / size */
+    OG_ERR_HIP = -2,        /* HIP runtime failure */
+    OG_ERR_STATE = -3,      /* call sequence (e.g. run before setup) */
+    OG_ERR_NONFINITE = -4,  /* a system diverged (non-finite velocities/forces) */
+};
+
+/* Force flags. */
+#define OG_MODE_A 1u        /* origami's integrator cutoffs: LJ + reaction-field Coulomb
+                               and GB pair force at 10 Å, descreening at 20 Å.  Without
+                               it: all pairs, no cutoff (bare Coulomb). */
+#define OG_SASA 2u          /* add analytical (PowerSasa) hydrophobic SASA forces */
+#define OG_ENERGY 4u        /* also accumulate per-system energy terms */
+#define OG_SURROGATE 8u     /* Born radii from the learned surrogate (og_surrogate_*) */
+
+/* Per-system energy terms written by og_forces / og_energies
+ * (n_systems x OG_N_ENERGY doubles, kJ/mol). */
+enum og_energy_term {
+    OG_EN_BOND = 0, OG_EN_ANGLE, OG_EN_DIHEDRAL, OG_EN_IMPROPER, OG_EN_LJ, OG_EN_COULOMB,
+    OG_EN_GB_SELF, OG_EN_GB_PAIR, OG_EN_SASA, OG_EN_KINETIC, OG_N_ENERGY
+};
+
+typedef struct og_system_desc {
+    int32_t n_atoms;
+    const double* pos;              /* [n_atoms*3] Å, atom-major x,y,z */
+    const double* charge;           /* [n_atoms] e */
+    const double* rmin_half;        /* [n_atoms] CHARMM Rmin/2, Å */
+    const double* epsilon;          /* [n_atoms] LJ well depth, kcal/mol (positive) */
+    const double* rmin_half_14;     /* [n_atoms] 1-4 Rmin/2 (= rmin_half if absent) */
+    const double* epsilon_14;       /* [n_atoms] 1-4 epsilon (= epsilon if absent) */
+    const double* gb_rho;           /* [n_atoms] intrinsic Born radius rho, Å */
+    const double* gb_scale;         /* [n_atoms] HCT descreening scale */
+    const double* sasa_radius;      /* [n_atoms] vdW + 1.4 Å probe */
+    const double* sasa_gamma;       /* [n_atoms] surface tension, kJ/mol/Å^2 (unscaled) */
+    const double* mass;             /* [n_atoms] Da */
+    int32_t n_excl;  const int32_t* excl;    /* [n_excl*2] local i<j 1-2 / 1-3 pairs */
+    int32_t n_14;    const int32_t* pair14;  /* [n_14*2]  local i<j 1-4 pairs */
+    int32_t n_bonds; const int32_t* bonds; const double* bond_params;           /* (K kcal/mol/Å^2, r0 Å) */
+    int32_t n_angles; const int32_t* angles; const double* angle_params;        /* (K kcal/mol/rad^2, theta0 rad) */
+    int32_t n_dihedrals; const int32_t* dihedrals; const double* dihedral_params; /* one row per periodic term:
+                                                                                  (k kcal/mol, n, delta rad) */
+    int32_t n_impropers; const int32_t* impropers; const double* improper_params; /* (K, psi0 rad); atom 0 central */
+    int32_t n_constraints; const int32_t* constraints; const double* constraint_len; /* X-H bonds, (i,j), r0 Å */
+} og_system_desc;
+
+/* Replaces ForceScratch::new (energy/src/scratch.rs:211-262): uploads
+ * parameters, builds the tile-pair work list, exclusion masks and SHAKE
+ * clusters once.  All index arrays are local to their system. */
+int og_engine_create(const og_system_desc* systems, int32_t n_systems, og_engine** out);
+void og_engine_destroy(og_engine* e);
+const char* og_last_error(void);
+
+/* info[0]=total atoms, [1]=systems, [2]=tiles, [3]=tile pairs, [4]=masked
+ * tile pairs, [5]=clusters, [6]=sum over systems of N(N-1)/2 pairs,
+ * [7]=SASA work atoms (gamma != 0). */
+int og_engine_info(const og_engine* e, int64_t* info);
+
+/* Host <-> device state, all systems concatenated in atom order. */
+int og_set_positions(og_engine* e, const double* xyz);
+int og_get_positions(og_engine* e, double* xyz);
+int og_set_velocities(og_engine* e, const double* v);
+int og_get_velocities(og_engine* e, double* v);
+int og_get_forces(og_engine* e, double* f);
+
+/* Replaces total_force_with_scratch (energy/src/forces.rs:78-128):
+ * overwrites f_out (if not NULL) with bonded + LJ + Coulomb + GB (+ SASA)
+ * forces at xyz (if NULL, at the device-resident positions).  energies
+ * (if not NULL, needs OG_ENERGY) receives n_systems x OG_N_ENERGY terms.
+ * sasa_gamma_scale replaces the ORIGAMI_SASA_GAMMA_SCALE environment
+ * variable (energy/src/powersasa/analytical.rs:477-480). */
+int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+              double sasa_gamma_scale, double* energies);
+
+/* Replaces total_energy_with_options (dynamics/src/energy_eval.rs:20-34) and
+ * remd::potential_energy (dynamics/src/remd.rs:529-539): per-system energy
+ * terms; GB pair energy is uncut in both modes, as in gb_energy. */
+int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies);
+
+/* Effective Born radii and descreening integrals after the last force call
+ * (gb.rs:203-342); either pointer may be NULL. */
+int og_born_radii(og_engine* e, double* radii, double* integral);
+
+typedef struct og_dynamics_params {
+    double dt_fs;                   /* timestep, fs */
+    double friction_ps_inv;         /* gamma, 1/ps */
+    uint64_t seed;                  /* Philox key for the O-step noise */
+    int32_t constrain_h_bonds;      /* SHAKE X-H (tol 1e-6 Å^2, 64 iterations) */
+    uint32_t flags;                 /* force flags (OG_MODE_A, OG_SASA, OG_SURROGATE) */
+    double sasa_gamma_scale;
+} og_dynamics_params;
+
+/* Replaces run_langevin's set-up (dynamics/src/langevin.rs:138-199):
+ * per-system bath temperatures, DOF correction for SHAKE, and the initial
+ * force evaluation.  Velocities are whatever og_set_velocities left (zero
+ * after create). */
+int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, const double* temperatures_k);
+
+/* n_steps BAOAB steps for every system (langevin.rs:223-342 /
+ * remd.rs:286-405).  ke_per_step (optional, n_steps x n_systems) receives
+ * each step's kinetic energy in kJ/mol.  Returns OG_ERR_NONFINITE if any
+ * system diverged; diverged systems stop moving. */
+int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
+
+/* Set the bath temperature of one system (REMD ladder assignment). */
+int og_set_temperature(og_engine* e, int32_t system, double temperature_k);
+
+/* REMD exchange (remd.rs:447-466): swap positions, velocities and forces of
+ * systems a and b (same topology) and rescale velocities by sqrt(T_a/T_b)
+ * and sqrt(T_b/T_a). */
+int og_swap_systems(og_engine* e, int32_t a, int32_t b);
+
+/* Per-system diverged flags, SHAKE failure counts, Born-radius clamp counts. */
+int og_get_status(og_engine* e, int32_t* diverged, int32_t* shake_failures, int32_t* clamped);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on
+ * the engine's stream.  og_profile_read returns, per kernel class
+ * (OG_K_*), total milliseconds and launch counts since the last reset. */
+enum og_kernel_class {
+    OG_K_CLUSTER = 0, OG_K_BONDED, OG_K_DESCREEN, OG_K_RADII, OG_K_FORCE, OG_K_SASA, OG_K_BACK,
+    OG_K_SURROGATE, OG_K_OTHER, OG_N_KCLASS
+};
+int og_profile_enable(og_engine* e, int32_t on);
+int og_profile_read(og_engine* e, double* ms, int64_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,738 @@
+// origami MI355X engine: host side of the C ABI (include/origami_gpu.h).
+//
+// Owns the device-resident batch, builds the tile-pair work list, the
+// exclusion-mask tiles and the SHAKE clusters once at create time
+// (origami's ForceScratch::new, energy/src/scratch.rs:211-262), and drives
+// the per-step kernel sequence on one HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/origami_gpu.h"
+#include "og_device.h"
+#include "og_kernels.h"
+#include "og_sasa.h"
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t _e = (x);                                                                   \
+        if (_e != hipSuccess)                                                                  \
+            return fail(OG_ERR_HIP, "%s failed: %s (%s:%d)", #x, hipGetErrorString(_e),        \
+                        __FILE__, __LINE__);                                                   \
+    } while (0)
+
+template <class T>
+static hipError_t upload(T** dst, const std::vector<T>& src) {
+    size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)dst, bytes);
+    if (e != hipSuccess) return e;
+    if (!src.empty()) e = hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+}
+
+struct og_engine {
+    int n_sys = 0, n_atoms = 0, n_tiles = 0, n_work = 0, n_masked = 0, n_clusters = 0;
+    long long n_pairs = 0;
+    int n_sasa_work = 0;
+    std::vector<int> atom_off, sys_n;
+    std::vector<double> temps, dofcorr;
+    std::vector<int> n_cons_sys;
+    hipStream_t stream = nullptr;
+    // device buffers
+    double *pos = nullptr, *vel = nullptr, *force = nullptr, *ref = nullptr;
+    long long *facc = nullptr, *iacc = nullptr, *eacc = nullptr, *ke = nullptr;
+    double *mass = nullptr, *rho = nullptr, *charge = nullptr, *radii = nullptr;
+    double *sasa_r = nullptr, *sasa_g = nullptr;
+    double *eps = nullptr, *rmin = nullptr, *eps14 = nullptr, *rmin14 = nullptr;
+    int *atom_sys = nullptr, *atom_local = nullptr, *atom_slot = nullptr;
+    float4 *par = nullptr, *gb = nullptr;
+    float* qs = nullptr;
+    int *tile_atom0 = nullptr, *tile_cnt = nullptr;
+    og_work* work = nullptr;
+    unsigned long long* masks = nullptr;
+    int nb = 0, na = 0, nd = 0, ni = 0;
+    int *bonds = nullptr, *angles = nullptr, *dih = nullptr, *imp = nullptr;
+    double *bond_p = nullptr, *angle_p = nullptr, *dih_p = nullptr, *imp_p = nullptr;
+    int *cl_start = nullptr, *cl_atoms = nullptr, *cn_start = nullptr, *cn_pair = nullptr;
+    double* cn_len = nullptr;
+    double *sys_temp = nullptr, *sys_dofc = nullptr;
+    int *shake_fail = nullptr, *diverged = nullptr, *clamped = nullptr;
+    og_sasa_data sasa{};
+    // dynamics
+    bool dyn_ready = false;
+    og_dynamics_params dyn{};
+    double alpha = 1.0;
+    uint32_t step = 0;
+    // profiling
+    bool prof = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double prof_ms[OG_N_KCLASS] = {0};
+    long long prof_cnt[OG_N_KCLASS] = {0};
+};
+
+// ---------------------------------------------------------------------------
+// profiling helpers
+// ---------------------------------------------------------------------------
+static hipEvent_t take_event(og_engine* e) {
+    if (!e->ev_pool.empty()) {
+        hipEvent_t ev = e->ev_pool.back();
+        e->ev_pool.pop_back();
+        return ev;
+    }
+    hipEvent_t ev;
+    hipEventCreate(&ev);
+    return ev;
+}
+
+struct ProfScope {
+    og_engine* e;
+    int cls;
+    hipEvent_t a = nullptr, b = nullptr;
+    ProfScope(og_engine* e_, int c) : e(e_), cls(c) {
+        if (e->prof) {
+            a = take_event(e);
+            b = take_event(e);
+            hipEventRecord(a, e->stream);
+        }
+    }
+    ~ProfScope() {
+        if (e->prof) {
+            hipEventRecord(b, e->stream);
+            e->ev_pending.push_back({cls, {a, b}});
+        }
+    }
+};
+
+static void prof_collect(og_engine* e) {
+    for (auto& p : e->ev_pending) {
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, p.second.first, p.second.second);
+        e->prof_ms[p.first] += ms;
+        e->prof_cnt[p.first] += 1;
+        e->ev_pool.push_back(p.second.first);
+        e->ev_pool.push_back(p.second.second);
+    }
+    e->ev_pending.clear();
+}
+
+// ---------------------------------------------------------------------------
+// create / destroy
+// ---------------------------------------------------------------------------
+static void free_all(og_engine* e) {
+    void* ptrs[] = {e->pos, e->vel, e->force, e->ref, e->facc, e->iacc, e->eacc, e->ke, e->mass,
+                    e->rho, e->charge, e->radii, e->sasa_r, e->sasa_g, e->eps, e->rmin, e->eps14,
+                    e->rmin14, e->atom_sys, e->atom_local,
+                    e->atom_slot, e->par, e->gb, e->qs, e->tile_atom0, e->tile_cnt, e->work,
+                    e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
+                    e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start, e->cn_pair,
+                    e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
+    for (void* p : ptrs)
+        if (p) hipFree(p);
+    og_sasa_free(&e->sasa);
+    for (auto ev : e->ev_pool) hipEventDestroy(ev);
+    for (auto& p : e->ev_pending) {
+        hipEventDestroy(p.second.first);
+        hipEventDestroy(p.second.second);
+    }
+    if (e->stream) hipStreamDestroy(e->stream);
+}
+
+static int check_pairs(const int32_t* p, int n, int width, int n_atoms, const char* what, int s) {
+    if (n < 0) return fail(OG_ERR_ARG, "system %d: negative %s count", s, what);
+    if (n > 0 && !p) return fail(OG_ERR_ARG, "system %d: %s array is NULL", s, what);
+    for (int k = 0; k < n * width; ++k)
+        if (p[k] < 0 || p[k] >= n_atoms)
+            return fail(OG_ERR_ARG, "system %d: %s index %d out of range [0,%d)", s, what, p[k], n_atoms);
+    return OG_OK;
+}
+
+extern "C" const char* og_last_error(void) { return g_err.c_str(); }
+
+static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys);
+
+extern "C" int og_engine_create(const og_system_desc* systems, int32_t n_systems, og_engine** out) {
+    if (!out) return fail(OG_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (!systems || n_systems <= 0) return fail(OG_ERR_ARG, "need at least one system");
+    for (int s = 0; s < n_systems; ++s) {
+        const og_system_desc& d = systems[s];
+        if (d.n_atoms <= 0) return fail(OG_ERR_ARG, "system %d has no atoms", s);
+        const double* req[] = {d.pos, d.charge, d.rmin_half, d.epsilon, d.rmin_half_14, d.epsilon_14,
+                               d.gb_rho, d.gb_scale, d.sasa_radius, d.sasa_gamma, d.mass};
+        for (const double* p : req)
+            if (!p) return fail(OG_ERR_ARG, "system %d: a per-atom array is NULL", s);
+        int rc;
+        if ((rc = check_pairs(d.excl, d.n_excl, 2, d.n_atoms, "exclusion", s))) return rc;
+        if ((rc = check_pairs(d.pair14, d.n_14, 2, d.n_atoms, "1-4", s))) return rc;
+        if ((rc = check_pairs(d.bonds, d.n_bonds, 2, d.n_atoms, "bond", s))) return rc;
+        if ((rc = check_pairs(d.angles, d.n_angles, 3, d.n_atoms, "angle", s))) return rc;
+        if ((rc = check_pairs(d.dihedrals, d.n_dihedrals, 4, d.n_atoms, "dihedral", s))) return rc;
+        if ((rc = check_pairs(d.impropers, d.n_impropers, 4, d.n_atoms, "improper", s))) return rc;
+        if ((rc = check_pairs(d.constraints, d.n_constraints, 2, d.n_atoms, "constraint", s))) return rc;
+        for (int k = 0; k < d.n_atoms; ++k) {
+            if (!(d.mass[k] > 0.0)) return fail(OG_ERR_ARG, "system %d atom %d: mass must be > 0", s, k);
+            if (!(d.gb_rho[k] - OG_OBC_OFFSET > 0.0))
+                return fail(OG_ERR_ARG, "system %d atom %d: gb_rho must exceed 0.09 Å", s, k);
+            for (int c = 0; c < 3; ++c)
+                if (!std::isfinite(d.pos[3 * k + c]))
+                    return fail(OG_ERR_ARG, "system %d atom %d: non-finite position", s, k);
+        }
+        for (int k = 0; k < d.n_excl; ++k)
+            if (d.excl[2 * k] >= d.excl[2 * k + 1])
+                return fail(OG_ERR_ARG, "system %d: exclusion pairs must have i < j", s);
+        for (int k = 0; k < d.n_14; ++k)
+            if (d.pair14[2 * k] >= d.pair14[2 * k + 1])
+                return fail(OG_ERR_ARG, "system %d: 1-4 pairs must have i < j", s);
+    }
+    og_engine* e = new og_engine();
+    int rc = engine_build(e, systems, n_systems);
+    if (rc != OG_OK) {
+        std::string msg = g_err;
+        free_all(e);
+        delete e;
+        g_err = msg;
+        return rc;
+    }
+    *out = e;
+    return OG_OK;
+}
+
+extern "C" void og_engine_destroy(og_engine* e) {
+    if (!e) return;
+    if (e->stream) hipStreamSynchronize(e->stream);
+    free_all(e);
+    delete e;
+}
+
+static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
+    HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    e->n_sys = n_sys;
+    e->atom_off.resize(n_sys + 1, 0);
+    e->sys_n.resize(n_sys);
+    for (int s = 0; s < n_sys; ++s) {
+        e->sys_n[s] = sys[s].n_atoms;
+        e->atom_off[s + 1] = e->atom_off[s] + sys[s].n_atoms;
+    }
+    const int N = e->atom_off[n_sys];
+    e->n_atoms = N;
+
+    std::vector<double> pos(3 * (size_t)N), mass(N), rho(N), charge(N), sr(N), sg(N);
+    std::vector<double> epsv(N), rminv(N), eps14v(N), rmin14v(N);
+    std::vector<int> a_sys(N), a_loc(N), a_slot(N);
+    std::vector<int> tile_atom0, tile_cnt;
+    std::vector<og_work> work;
+    std::vector<unsigned long long> masks;
+    std::vector<float4> par, gbv;
+    std::vector<float> qs;
+    std::vector<int> bonds, angles, dih, imp;
+    std::vector<double> bond_p, angle_p, dih_p, imp_p;
+    std::vector<int> cl_start{0}, cl_atoms, cn_start{0}, cn_pair;
+    std::vector<double> cn_len;
+    e->n_cons_sys.resize(n_sys);
+
+    const double sqrt_lj = std::sqrt(12.0 * OG_KCAL_TO_KJ);
+    const double sqrt_kc = std::sqrt(OG_COULOMB_KCAL * OG_KCAL_TO_KJ);
+    for (int s = 0; s < n_sys; ++s) {
+        const og_system_desc& d = sys[s];
+        const int n = d.n_atoms, off = e->atom_off[s];
+        const int t0 = (int)tile_atom0.size();
+        const int nt = (n + OG_TILE - 1) / OG_TILE;
+        e->n_pairs += (long long)n * (n - 1) / 2;
+        for (int t = 0; t < nt; ++t) {
+            tile_atom0.push_back(off + t * OG_TILE);
+            tile_cnt.push_back(std::min(OG_TILE, n - t * OG_TILE));
+        }
+        for (int k = 0; k < n; ++k) {
+            const int g = off + k;
+            for (int c = 0; c < 3; ++c) pos[3 * (size_t)g + c] = d.pos[3 * k + c];
+            mass[g] = d.mass[k];
+            rho[g] = d.gb_rho[k];
+            charge[g] = d.charge[k];
+            sr[g] = d.sasa_radius[k];
+            sg[g] = d.sasa_gamma[k];
+            epsv[g] = d.epsilon[k];
+            rminv[g] = d.rmin_half[k];
+            eps14v[g] = d.epsilon_14[k];
+            rmin14v[g] = d.rmin_half_14[k];
+            a_sys[g] = s;
+            a_loc[g] = k;
+            a_slot[g] = (t0 + k / OG_TILE) * OG_TILE + (k % OG_TILE);
+            if (d.sasa_gamma[k] != 0.0) e->n_sasa_work++;
+        }
+        par.resize((size_t)(t0 + nt) * OG_TILE, make_float4(0.f, 0.f, 0.f, 0.f));
+        gbv.resize((size_t)(t0 + nt) * OG_TILE, make_float4(1.f, 0.f, 1.f, 1.f));
+        qs.resize((size_t)(t0 + nt) * OG_TILE, 0.f);
+        for (int k = 0; k < n; ++k) {
+            const size_t slot = (size_t)a_slot[off + k];
+            par[slot] = make_float4((float)(std::sqrt(d.epsilon[k]) * sqrt_lj), (float)d.rmin_half[k],
+                                    (float)(std::sqrt(d.epsilon_14[k]) * sqrt_lj), (float)d.rmin_half_14[k]);
+            const double rt = d.gb_rho[k] - OG_OBC_OFFSET;
+            gbv[slot] = make_float4((float)rt, (float)(d.gb_scale[k] * rt), (float)rt,
+                                    (float)(0.60056120439322491 / rt));
+            qs[slot] = (float)(d.charge[k] * sqrt_kc);
+        }
+        // Work list: every tile pair I <= J of this system.
+        std::map<std::pair<int, int>, int> mask_of;
+        const int w0 = (int)work.size();
+        for (int I = 0; I < nt; ++I)
+            for (int J = I; J < nt; ++J) work.push_back({t0 + I, t0 + J, -1, s});
+        auto tile_pair_index = [&](int I, int J) {  // I <= J local
+            return w0 + I * nt - I * (I - 1) / 2 + (J - I);
+        };
+        auto set_bit = [&](int a, int b, int which) {
+            const int I = a / OG_TILE, J = b / OG_TILE;
+            const int wi = tile_pair_index(I, J);
+            if (work[wi].mask < 0) {
+                work[wi].mask = (int)(masks.size() / 128);
+                masks.resize(masks.size() + 128, 0ull);
+            }
+            masks[(size_t)work[wi].mask * 128 + which * 64 + (a % OG_TILE)] |= 1ull << (b % OG_TILE);
+        };
+        for (int k = 0; k < d.n_excl; ++k) set_bit(d.excl[2 * k], d.excl[2 * k + 1], 0);
+        for (int k = 0; k < d.n_14; ++k) set_bit(d.pair14[2 * k], d.pair14[2 * k + 1], 1);
+        // Bonded terms (global atom indices).
+        for (int k = 0; k < d.n_bonds; ++k) {
+            bonds.push_back(off + d.bonds[2 * k]); bonds.push_back(off + d.bonds[2 * k + 1]);
+            bond_p.push_back(d.bond_params[2 * k]); bond_p.push_back(d.bond_params[2 * k + 1]);
+        }
+        for (int k = 0; k < d.n_angles; ++k) {
+            for (int c = 0; c < 3; ++c) angles.push_back(off + d.angles[3 * k + c]);
+            angle_p.push_back(d.angle_params[2 * k]); angle_p.push_back(d.angle_params[2 * k + 1]);
+        }
+        for (int k = 0; k < d.n_dihedrals; ++k) {
+            for (int c = 0; c < 4; ++c) dih.push_back(off + d.dihedrals[4 * k + c]);
+            for (int c = 0; c < 3; ++c) dih_p.push_back(d.dihedral_params[3 * k + c]);
+        }
+        for (int k = 0; k < d.n_impropers; ++k) {
+            for (int c = 0; c < 4; ++c) imp.push_back(off + d.impropers[4 * k + c]);
+            imp_p.push_back(d.improper_params[2 * k]); imp_p.push_back(d.improper_params[2 * k + 1]);
+        }
+        // SHAKE clusters: each constraint joins its heavy atom's cluster; every
+        // other atom is a singleton.  Constraint order within a cluster follows
+        // the caller's (origami's sorted bond) order.
+        std::vector<int> owner(n, -1);  // heavy atom of an H, or itself
+        std::vector<std::vector<int>> cons_of(n);
+        for (int k = 0; k < d.n_constraints; ++k) {
+            const int a = d.constraints[2 * k], b = d.constraints[2 * k + 1];
+            // the hydrogen is the lighter atom
+            const int h = d.mass[a] < d.mass[b] ? a : b;
+            const int x = h == a ? b : a;
+            if (owner[h] >= 0 && owner[h] != x)
+                return fail(OG_ERR_ARG, "system %d: atom %d is constrained to two heavy atoms", s, h);
+            if (owner[x] >= 0 && owner[x] != x)
+                return fail(OG_ERR_ARG, "system %d: constraint graph is not X-H star shaped", s);
+            owner[h] = x;
+            owner[x] = x;
+            cons_of[x].push_back(k);
+        }
+        e->n_cons_sys[s] = d.n_constraints;
+        std::vector<std::vector<int>> members(n);
+        for (int k = 0; k < n; ++k) {
+            const int o = owner[k] < 0 ? k : owner[k];
+            members[o].push_back(k);
+        }
+        for (int k = 0; k < n; ++k) {
+            if (members[k].empty()) continue;
+            if ((int)members[k].size() > 8)
+                return fail(OG_ERR_ARG, "system %d: atom %d has more than 7 constrained partners", s, k);
+            for (int m : members[k]) cl_atoms.push_back(off + m);
+            cl_start.push_back((int)cl_atoms.size());
+            for (int c : cons_of[k]) {
+                cn_pair.push_back(off + d.constraints[2 * c]);
+                cn_pair.push_back(off + d.constraints[2 * c + 1]);
+                cn_len.push_back(d.constraint_len[c]);
+            }
+            cn_start.push_back((int)cn_len.size());
+        }
+    }
+    e->n_tiles = (int)tile_atom0.size();
+    e->n_work = (int)work.size();
+    e->n_masked = (int)(masks.size() / 128);
+    e->n_clusters = (int)cl_start.size() - 1;
+    e->nb = (int)bond_p.size() / 2;
+    e->na = (int)angle_p.size() / 2;
+    e->nd = (int)dih_p.size() / 3;
+    e->ni = (int)imp_p.size() / 2;
+    e->temps.assign(n_sys, 0.0);
+    e->dofcorr.assign(n_sys, 1.0);
+
+    HIPCHK(upload(&e->pos, pos));
+    std::vector<double> zeros3(3 * (size_t)N, 0.0);
+    HIPCHK(upload(&e->vel, zeros3));
+    HIPCHK(upload(&e->force, zeros3));
+    HIPCHK(upload(&e->ref, zeros3));
+    std::vector<long long> zl3(3 * (size_t)N, 0), zl(N, 0), ze((size_t)n_sys * OG_NE, 0), zs(n_sys, 0);
+    HIPCHK(upload(&e->facc, zl3));
+    HIPCHK(upload(&e->iacc, zl));
+    HIPCHK(upload(&e->eacc, ze));
+    HIPCHK(upload(&e->ke, zs));
+    HIPCHK(upload(&e->mass, mass));
+    HIPCHK(upload(&e->rho, rho));
+    HIPCHK(upload(&e->charge, charge));
+    std::vector<double> r0(N);
+    for (int i = 0; i < N; ++i) r0[i] = rho[i] - OG_OBC_OFFSET;
+    HIPCHK(upload(&e->radii, r0));
+    HIPCHK(upload(&e->sasa_r, sr));
+    HIPCHK(upload(&e->sasa_g, sg));
+    HIPCHK(upload(&e->eps, epsv));
+    HIPCHK(upload(&e->rmin, rminv));
+    HIPCHK(upload(&e->eps14, eps14v));
+    HIPCHK(upload(&e->rmin14, rmin14v));
+    HIPCHK(upload(&e->atom_sys, a_sys));
+    HIPCHK(upload(&e->atom_local, a_loc));
+    HIPCHK(upload(&e->atom_slot, a_slot));
+    HIPCHK(upload(&e->par, par));
+    HIPCHK(upload(&e->gb, gbv));
+    HIPCHK(upload(&e->qs, qs));
+    HIPCHK(upload(&e->tile_atom0, tile_atom0));
+    HIPCHK(upload(&e->tile_cnt, tile_cnt));
+    HIPCHK(upload(&e->work, work));
+    HIPCHK(upload(&e->masks, masks));
+    HIPCHK(upload(&e->bonds, bonds));
+    HIPCHK(upload(&e->bond_p, bond_p));
+    HIPCHK(upload(&e->angles, angles));
+    HIPCHK(upload(&e->angle_p, angle_p));
+    HIPCHK(upload(&e->dih, dih));
+    HIPCHK(upload(&e->dih_p, dih_p));
+    HIPCHK(upload(&e->imp, imp));
+    HIPCHK(upload(&e->imp_p, imp_p));
+    HIPCHK(upload(&e->cl_start, cl_start));
+    HIPCHK(upload(&e->cl_atoms, cl_atoms));
+    HIPCHK(upload(&e->cn_start, cn_start));
+    HIPCHK(upload(&e->cn_pair, cn_pair));
+    HIPCHK(upload(&e->cn_len, cn_len));
+    std::vector<double> z_sys(n_sys, 0.0), o_sys(n_sys, 1.0);
+    HIPCHK(upload(&e->sys_temp, z_sys));
+    HIPCHK(upload(&e->sys_dofc, o_sys));
+    std::vector<int> zi(n_sys, 0);
+    HIPCHK(upload(&e->shake_fail, zi));
+    HIPCHK(upload(&e->diverged, zi));
+    HIPCHK(upload(&e->clamped, zi));
+    int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg);
+    if (rc != OG_OK) return fail(rc, "SASA set-up failed: %s", og_sasa_error());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    HIPCHK(hipDeviceSynchronize());
+    return OG_OK;
+}
+
+extern "C" int og_engine_info(const og_engine* e, int64_t* info) {
+    if (!e || !info) return fail(OG_ERR_ARG, "NULL argument");
+    info[0] = e->n_atoms;
+    info[1] = e->n_sys;
+    info[2] = e->n_tiles;
+    info[3] = e->n_work;
+    info[4] = e->n_masked;
+    info[5] = e->n_clusters;
+    info[6] = e->n_pairs;
+    info[7] = e->n_sasa_work;
+    return OG_OK;
+}
+
+static int copy_state(og_engine* e, double* dev, const double* host, bool to_dev) {
+    if (!e || !host) return fail(OG_ERR_ARG, "NULL argument");
+    const size_t bytes = 3 * (size_t)e->n_atoms * sizeof(double);
+    if (to_dev) {
+        for (size_t k = 0; k < 3 * (size_t)e->n_atoms; ++k)
+            if (!std::isfinite(host[k])) return fail(OG_ERR_ARG, "non-finite input at %zu", k);
+        HIPCHK(hipMemcpyAsync(dev, host, bytes, hipMemcpyHostToDevice, e->stream));
+    } else {
+        HIPCHK(hipMemcpyAsync((void*)host, dev, bytes, hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_set_positions(og_engine* e, const double* x) { return copy_state(e, e ? e->pos : nullptr, x, true); }
+extern "C" int og_get_positions(og_engine* e, double* x) { return copy_state(e, e ? e->pos : nullptr, x, false); }
+extern "C" int og_set_velocities(og_engine* e, const double* v) { return copy_state(e, e ? e->vel : nullptr, v, true); }
+extern "C" int og_get_velocities(og_engine* e, double* v) { return copy_state(e, e ? e->vel : nullptr, v, false); }
+extern "C" int og_get_forces(og_engine* e, double* f) { return copy_state(e, e ? e->force : nullptr, f, false); }
+
+// ---------------------------------------------------------------------------
+// Force pipeline (accumulators must be zero on entry).
+// ---------------------------------------------------------------------------
+static og_pair_args pair_args(og_engine* e) {
+    og_pair_args a{};
+    a.pos = e->pos; a.par = e->par; a.gb = e->gb; a.qs = e->qs;
+    a.tile_atom0 = e->tile_atom0; a.tile_cnt = e->tile_cnt;
+    a.work = e->work; a.n_work = e->n_work; a.masks = e->masks;
+    a.facc = e->facc; a.iacc = e->iacc; a.eacc = e->eacc;
+    const double rc = 10.0;
+    a.rc2_nb = (float)(rc * rc);
+    a.rc2_gb = (float)(rc * rc);
+    a.rc2_ds = (float)(20.0 * 20.0);
+    a.inv_rc3 = (float)(1.0 / (rc * rc * rc));
+    a.kappa = (float)(1.0 / (2.0 * rc * rc * rc));
+    a.crf = (float)(3.0 / (2.0 * rc));
+    return a;
+}
+
+static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy) {
+    const bool mode_a = flags & OG_MODE_A;
+    {
+        ProfScope p(e, OG_K_BONDED);
+        og_bonded_args b{};
+        b.pos = e->pos; b.facc = e->facc; b.eacc = e->eacc; b.atom_sys = e->atom_sys;
+        b.nb = e->nb; b.na = e->na; b.nd = e->nd; b.ni = e->ni;
+        b.bonds = e->bonds; b.bond_p = e->bond_p; b.angles = e->angles; b.angle_p = e->angle_p;
+        b.dih = e->dih; b.dih_p = e->dih_p; b.imp = e->imp; b.imp_p = e->imp_p;
+        og_launch_bonded(b, energy, e->stream);
+    }
+    og_pair_args a = pair_args(e);
+    {
+        ProfScope p(e, OG_K_DESCREEN);
+        og_launch_pair_descreen(a, mode_a, e->stream);
+    }
+    {
+        ProfScope p(e, OG_K_RADII);
+        og_radii_args r{};
+        r.iacc = e->iacc; r.atom_sys = e->atom_sys; r.atom_slot = e->atom_slot; r.rho = e->rho;
+        r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
+        r.eacc = e->eacc; r.n_atoms = e->n_atoms;
+        og_launch_radii(r, energy, e->stream);
+    }
+    {
+        ProfScope p(e, OG_K_FORCE);
+        og_launch_pair_force(a, mode_a, e->stream);
+    }
+    if (energy) {
+        ProfScope p(e, OG_K_OTHER);
+        og_energy64_args en{};
+        en.charge = e->charge; en.eps = e->eps; en.rmin = e->rmin; en.eps14 = e->eps14;
+        en.rmin14 = e->rmin14; en.radii = e->radii;
+        en.rc2_nb = 100.0; en.kappa = 1.0 / 2000.0; en.crf = 3.0 / 20.0;
+        og_launch_pair_energy64(a, en, mode_a, e->stream);
+    }
+    if (flags & OG_SASA) {
+        ProfScope p(e, OG_K_SASA);
+        int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, e->eacc, sasa_scale, energy, e->stream);
+        if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
+    }
+    HIPCHK(hipGetLastError());
+    return OG_OK;
+}
+
+static int read_energies(og_engine* e, double* out) {
+    std::vector<long long> acc((size_t)e->n_sys * OG_NE);
+    HIPCHK(hipMemcpyAsync(acc.data(), e->eacc, acc.size() * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (size_t k = 0; k < acc.size(); ++k) out[k] = (double)acc[k] * OG_FIX_INV;
+    return OG_OK;
+}
+
+extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+                         double sasa_gamma_scale, double* energies) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (flags & OG_SURROGATE) return fail(OG_ERR_ARG, "surrogate radii are not enabled in this build");
+    int rc;
+    if (xyz && (rc = og_set_positions(e, xyz))) return rc;
+    const bool energy = (flags & OG_ENERGY) || energies;
+    if (energy) HIPCHK(hipMemsetAsync(e->eacc, 0, (size_t)e->n_sys * OG_NE * sizeof(long long), e->stream));
+    HIPCHK(hipMemsetAsync(e->clamped, 0, (size_t)e->n_sys * sizeof(int), e->stream));
+    og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
+    if ((rc = launch_forces(e, flags, sasa_gamma_scale, energy))) return rc;
+    {
+        ProfScope p(e, OG_K_BACK);
+        og_back_args b{};
+        b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+        b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = 0.0; b.do_kick = 0;
+        b.ke_acc = nullptr; b.diverged = e->diverged;
+        og_launch_back(b, e->stream);
+    }
+    HIPCHK(hipGetLastError());
+    if (f_out) {
+        HIPCHK(hipMemcpyAsync(f_out, e->force, 3 * (size_t)e->n_atoms * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->prof) prof_collect(e);
+    if (energies) {
+        if ((rc = read_energies(e, energies))) return rc;
+    }
+    if (f_out) {
+        for (size_t k = 0; k < 3 * (size_t)e->n_atoms; ++k)
+            if (!std::isfinite(f_out[k])) return fail(OG_ERR_NONFINITE, "non-finite force at component %zu", k);
+    }
+    return OG_OK;
+}
+
+extern "C" int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies) {
+    if (!energies) return fail(OG_ERR_ARG, "energies is NULL");
+    return og_forces(e, xyz, nullptr, flags | OG_ENERGY, 1.0, energies);
+}
+
+extern "C" int og_born_radii(og_engine* e, double* radii, double* integral) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (radii) HIPCHK(hipMemcpyAsync(radii, e->radii, (size_t)e->n_atoms * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    std::vector<long long> acc;
+    if (integral) {
+        acc.resize(e->n_atoms);
+        HIPCHK(hipMemcpyAsync(acc.data(), e->iacc, (size_t)e->n_atoms * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (integral)
+        for (int i = 0; i < e->n_atoms; ++i) integral[i] = (double)acc[i] * OG_FIX_INV;
+    return OG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Dynamics
+// ---------------------------------------------------------------------------
+extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, const double* temps) {
+    if (!e || !p || !temps) return fail(OG_ERR_ARG, "NULL argument");
+    if (!(p->dt_fs > 0.0) || !(p->friction_ps_inv >= 0.0))
+        return fail(OG_ERR_ARG, "dt must be > 0 and friction >= 0");
+    if (p->flags & OG_SURROGATE) return fail(OG_ERR_ARG, "surrogate radii are not enabled in this build");
+    for (int s = 0; s < e->n_sys; ++s)
+        if (!(temps[s] >= 0.0)) return fail(OG_ERR_ARG, "temperature of system %d must be >= 0", s);
+    e->dyn = *p;
+    e->alpha = std::exp(-p->friction_ps_inv * p->dt_fs * 1.0e-3);
+    for (int s = 0; s < e->n_sys; ++s) {
+        e->temps[s] = temps[s];
+        const double total = 3.0 * e->sys_n[s];
+        const int nc = p->constrain_h_bonds ? e->n_cons_sys[s] : 0;
+        e->dofcorr[s] = nc > 0 ? total / std::max(total - nc, 1.0) : 1.0;
+    }
+    HIPCHK(hipMemcpyAsync(e->sys_temp, e->temps.data(), e->n_sys * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemcpyAsync(e->sys_dofc, e->dofcorr.data(), e->n_sys * sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipMemsetAsync(e->shake_fail, 0, e->n_sys * sizeof(int), e->stream));
+    HIPCHK(hipMemsetAsync(e->diverged, 0, e->n_sys * sizeof(int), e->stream));
+    e->step = 0;
+    e->dyn_ready = true;
+    // initial forces at the current positions
+    return og_forces(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->sasa_gamma_scale, nullptr);
+}
+
+extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
+    if (!e || s < 0 || s >= e->n_sys || !(t >= 0.0)) return fail(OG_ERR_ARG, "bad system or temperature");
+    e->temps[s] = t;
+    HIPCHK(hipMemcpyAsync(e->sys_temp + s, &e->temps[s], sizeof(double), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (!e->dyn_ready) return fail(OG_ERR_STATE, "og_dynamics_setup has not been called");
+    if (n_steps < 0) return fail(OG_ERR_ARG, "negative step count");
+    const og_dynamics_params& p = e->dyn;
+    long long* ke_dev = nullptr;
+    if (ke_per_step && n_steps > 0) {
+        HIPCHK(hipMallocAsync((void**)&ke_dev, (size_t)n_steps * e->n_sys * sizeof(long long), e->stream));
+        HIPCHK(hipMemsetAsync(ke_dev, 0, (size_t)n_steps * e->n_sys * sizeof(long long), e->stream));
+    }
+    og_cluster_args c{};
+    c.pos = e->pos; c.vel = e->vel; c.ref = e->ref; c.force = e->force; c.facc = e->facc;
+    c.iacc = e->iacc; c.mass = e->mass; c.atom_sys = e->atom_sys; c.atom_local = e->atom_local;
+    c.cl_start = e->cl_start; c.cl_atoms = e->cl_atoms; c.cn_start = e->cn_start;
+    c.cn_pair = e->cn_pair; c.cn_len = e->cn_len; c.n_clusters = e->n_clusters;
+    c.sys_temp = e->sys_temp; c.sys_dofcorr = e->sys_dofc; c.shake_fail = e->shake_fail;
+    c.diverged = e->diverged; c.dt = p.dt_fs; c.alpha = e->alpha;
+    c.use_shake = p.constrain_h_bonds ? 1 : 0;
+    c.key0 = (uint32_t)(p.seed & 0xffffffffu);
+    c.key1 = (uint32_t)(p.seed >> 32);
+    og_back_args b{};
+    b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+    b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = p.dt_fs; b.do_kick = 1;
+    b.diverged = e->diverged;
+    for (int k = 0; k < n_steps; ++k) {
+        c.step = e->step;
+        {
+            ProfScope ps(e, OG_K_CLUSTER);
+            og_launch_cluster_step(c, e->stream);
+        }
+        int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false);
+        if (rc) {
+            if (ke_dev) hipFreeAsync(ke_dev, e->stream);
+            return rc;
+        }
+        b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
+        {
+            ProfScope ps(e, OG_K_BACK);
+            og_launch_back(b, e->stream);
+        }
+        e->step++;
+        if (e->prof && (k & 63) == 63) {
+            HIPCHK(hipStreamSynchronize(e->stream));
+            prof_collect(e);
+        }
+    }
+    HIPCHK(hipGetLastError());
+    if (ke_dev) {
+        std::vector<long long> h((size_t)n_steps * e->n_sys);
+        HIPCHK(hipMemcpyAsync(h.data(), ke_dev, h.size() * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipFreeAsync(ke_dev, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        for (size_t k = 0; k < h.size(); ++k) ke_per_step[k] = (double)h[k] * OG_FIX_INV;
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->prof) prof_collect(e);
+    std::vector<int> div(e->n_sys);
+    HIPCHK(hipMemcpy(div.data(), e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
+    for (int s = 0; s < e->n_sys; ++s)
+        if (div[s]) return fail(OG_ERR_NONFINITE, "system %d diverged (non-finite velocities)", s);
+    return OG_OK;
+}
+
+extern "C" int og_swap_systems(og_engine* e, int32_t a, int32_t b) {
+    if (!e || a < 0 || b < 0 || a >= e->n_sys || b >= e->n_sys || a == b)
+        return fail(OG_ERR_ARG, "bad system indices");
+    if (e->sys_n[a] != e->sys_n[b]) return fail(OG_ERR_ARG, "systems %d and %d differ in size", a, b);
+    if (!(e->temps[a] > 0.0) || !(e->temps[b] > 0.0)) return fail(OG_ERR_STATE, "temperatures not set");
+    const double sa = std::sqrt(e->temps[a] / e->temps[b]);
+    const double sb = std::sqrt(e->temps[b] / e->temps[a]);
+    og_launch_swap(e->pos, e->vel, e->force, e->atom_off[a], e->atom_off[b], e->sys_n[a], sa, sb, e->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_get_status(og_engine* e, int32_t* div, int32_t* fails, int32_t* clamped) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (div) HIPCHK(hipMemcpy(div, e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
+    if (fails) HIPCHK(hipMemcpy(fails, e->shake_fail, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
+    if (clamped) HIPCHK(hipMemcpy(clamped, e->clamped, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
+    return OG_OK;
+}
+
+extern "C" int og_profile_enable(og_engine* e, int32_t on) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    prof_collect(e);
+    e->prof = on != 0;
+    for (int k = 0; k < OG_N_KCLASS; ++k) {
+        e->prof_ms[k] = 0.0;
+        e->prof_cnt[k] = 0;
+    }
+    return OG_OK;
+}
+
+extern "C" int og_profile_read(og_engine* e, double* ms, int64_t* count) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    prof_collect(e);
+    for (int k = 0; k < OG_N_KCLASS; ++k) {
+        if (ms) ms[k] = e->prof_ms[k];
+        if (count) count[k] = e->prof_cnt[k];
+    }
+    return OG_OK;
+}

@@ -1,0 +1,437 @@
+// O(N) kernels in FP64: bonded forces/energies, the Born-radius transform,
+// the fused BAOAB front half with SHAKE, and the force finalisation / second
+// half-kick.  These are HBM/latency-bound; they keep origami's FP64 state.
+//
+//   bonded terms     energy/src/forces_bonded.rs:47-236, bonded.rs:59-191
+//   Born radii       energy/src/gb.rs:327-341 (tanh transform + clamp)
+//   BAOAB            dynamics/src/langevin.rs:223-313
+//   SHAKE            dynamics/src/shake.rs:86-125
+//   kinetic energy   dynamics/src/langevin.rs:469-475
+#include "og_device.h"
+#include "og_kernels.h"
+
+#define OG_PI 3.14159265358979323846
+
+namespace {
+
+struct d3 { double x, y, z; };
+__device__ __forceinline__ d3 ld3(const double* p) { return {p[0], p[1], p[2]}; }
+__device__ __forceinline__ d3 sub(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ d3 add(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ d3 mul(d3 a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ d3 dv(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ d3 neg(d3 a) { return {-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ d3 cross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ double nrm(d3 a) { return sqrt(dot(a, a)); }
+
+__device__ __forceinline__ void addf(long long* facc, int atom, d3 f) {
+    og_atomic_add_fix(facc + 3 * (size_t)atom + 0, og_to_fix(f.x));
+    og_atomic_add_fix(facc + 3 * (size_t)atom + 1, og_to_fix(f.y));
+    og_atomic_add_fix(facc + 3 * (size_t)atom + 2, og_to_fix(f.z));
+}
+
+// forces_bonded.rs:124-167
+__device__ bool dihedral_gradient(d3 pa, d3 pb, d3 pc, d3 pd, d3& da, d3& db, d3& dc, d3& dd,
+                                  double& phi) {
+    d3 b1 = sub(pb, pa), b2 = sub(pc, pb), b3 = sub(pd, pc);
+    double b2n = nrm(b2);
+    if (b2n < 1e-9) return false;
+    d3 s = cross(b1, b2), t = cross(b2, b3);
+    double s2 = dot(s, s), t2 = dot(t, t);
+    if (s2 < 1e-18 || t2 < 1e-18) return false;
+    da = mul(s, b2n / s2);
+    dd = mul(neg(t), b2n / t2);
+    double ib2 = 1.0 / (b2n * b2n);
+    double f1 = dot(b1, b2) * ib2, f2 = dot(b3, b2) * ib2;
+    db = add(mul(da, -(f1 + 1.0)), mul(dd, f2));
+    dc = neg(add(add(da, db), dd));
+    d3 m1 = cross(s, dv(b2, b2n));
+    phi = atan2(dot(m1, t), dot(s, t));
+    return true;
+}
+
+// geom/src/measure.rs:17-28
+__device__ double measure_dihedral(d3 a, d3 b, d3 c, d3 d) {
+    d3 b0 = sub(a, b);
+    d3 b1 = sub(c, b);
+    b1 = dv(b1, nrm(b1));
+    d3 b2 = sub(d, c);
+    d3 v = sub(b0, mul(b1, dot(b0, b1)));
+    d3 w = sub(b2, mul(b1, dot(b2, b1)));
+    return atan2(dot(cross(b1, v), w), dot(v, w));
+}
+
+__device__ __forceinline__ double wrap_pi(double x) {
+    while (x > OG_PI) x -= 2.0 * OG_PI;
+    while (x < -OG_PI) x += 2.0 * OG_PI;
+    return x;
+}
+
+template <bool ENERGY>
+__device__ void bonded_body(const og_bonded_args& a) {
+    const int total = a.nb + a.na + a.nd + a.ni;
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
+        if (k < a.nb) {
+            const int ia = a.bonds[2 * k], ib = a.bonds[2 * k + 1];
+            const double K = a.bond_p[2 * k], r0 = a.bond_p[2 * k + 1];
+            d3 d = sub(ld3(a.pos + 3 * ia), ld3(a.pos + 3 * ib));
+            d = neg(d);  // d = x_b - x_a
+            const double r = nrm(d);
+            if (r >= 1e-9) {
+                const double mag = 2.0 * K * (r - r0) * OG_KCAL_TO_KJ;
+                const d3 f = mul(d, mag / r);
+                addf(a.facc, ia, f);
+                addf(a.facc, ib, neg(f));
+            }
+            if (ENERGY) {
+                const double dr = r - r0;
+                og_atomic_add_fix(a.eacc + a.atom_sys[ia] * OG_NE + OG_E_BOND,
+                                  og_to_fix(K * dr * dr * OG_KCAL_TO_KJ));
+            }
+            continue;
+        }
+        int kk = k - a.nb;
+        if (kk < a.na) {
+            const int ia = a.angles[3 * kk], ib = a.angles[3 * kk + 1], ic = a.angles[3 * kk + 2];
+            const double K = a.angle_p[2 * kk], th0 = a.angle_p[2 * kk + 1];
+            const d3 pb = ld3(a.pos + 3 * ib);
+            const d3 u = sub(ld3(a.pos + 3 * ia), pb), v = sub(ld3(a.pos + 3 * ic), pb);
+            const double un = nrm(u), vn = nrm(v);
+            if (un >= 1e-9 && vn >= 1e-9) {
+                const d3 uh = dv(u, un), vh = dv(v, vn);
+                double ct = fmin(fmax(dot(uh, vh), -1.0), 1.0);
+                const double st = sqrt(1.0 - ct * ct);
+                if (st >= 1e-9) {
+                    const double theta = acos(ct);
+                    const double coeff = 2.0 * K * (theta - th0) * OG_KCAL_TO_KJ / st;
+                    const d3 fa = mul(sub(vh, mul(uh, ct)), coeff / un);
+                    const d3 fc = mul(sub(uh, mul(vh, ct)), coeff / vn);
+                    addf(a.facc, ia, fa);
+                    addf(a.facc, ib, neg(add(fa, fc)));
+                    addf(a.facc, ic, fc);
+                }
+                if (ENERGY) {
+                    const double dt = acos(fmin(fmax(dot(uh, vh), -1.0), 1.0)) - th0;
+                    og_atomic_add_fix(a.eacc + a.atom_sys[ia] * OG_NE + OG_E_ANGLE,
+                                      og_to_fix(K * dt * dt * OG_KCAL_TO_KJ));
+                }
+            }
+            continue;
+        }
+        kk -= a.na;
+        const bool is_dih = kk < a.nd;
+        if (!is_dih) kk -= a.nd;
+        const int* q = is_dih ? a.dih + 4 * kk : a.imp + 4 * kk;
+        const d3 pa = ld3(a.pos + 3 * q[0]), pb = ld3(a.pos + 3 * q[1]), pc = ld3(a.pos + 3 * q[2]),
+                 pd = ld3(a.pos + 3 * q[3]);
+        d3 da, db, dc, dd;
+        double phi;
+        if (dihedral_gradient(pa, pb, pc, pd, da, db, dc, dd, phi)) {
+            double dvdphi;
+            if (is_dih) {
+                const double kd = a.dih_p[3 * kk], nn = a.dih_p[3 * kk + 1], delta = a.dih_p[3 * kk + 2];
+                dvdphi = -kd * nn * sin(nn * phi - delta) * OG_KCAL_TO_KJ;
+            } else {
+                dvdphi = 2.0 * a.imp_p[2 * kk] * wrap_pi(phi - a.imp_p[2 * kk + 1]) * OG_KCAL_TO_KJ;
+            }
+            addf(a.facc, q[0], mul(da, -dvdphi));
+            addf(a.facc, q[1], mul(db, -dvdphi));
+            addf(a.facc, q[2], mul(dc, -dvdphi));
+            addf(a.facc, q[3], mul(dd, -dvdphi));
+        }
+        if (ENERGY) {
+            const double chi = measure_dihedral(pa, pb, pc, pd);
+            double e;
+            int slot;
+            if (is_dih) {
+                e = a.dih_p[3 * kk] * (1.0 + cos(a.dih_p[3 * kk + 1] * chi - a.dih_p[3 * kk + 2]));
+                slot = OG_E_DIHEDRAL;
+            } else {
+                const double dom = wrap_pi(chi - a.imp_p[2 * kk + 1]);
+                e = a.imp_p[2 * kk] * dom * dom;
+                slot = OG_E_IMPROPER;
+            }
+            og_atomic_add_fix(a.eacc + a.atom_sys[q[0]] * OG_NE + slot, og_to_fix(e * OG_KCAL_TO_KJ));
+        }
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) og_k_bonded(og_bonded_args a) { bonded_body<false>(a); }
+__global__ void __launch_bounds__(256) og_k_bonded_e(og_bonded_args a) { bonded_body<true>(a); }
+
+void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
+    const int total = a.nb + a.na + a.nd + a.ni;
+    if (total <= 0) return;
+    int g = (total + 255) / 256;
+    if (g > 4096) g = 4096;
+    if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(og_k_bonded, dim3(g), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Born radii from the descreening integral (gb.rs:327-341), GB self energy.
+// ---------------------------------------------------------------------------
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_atoms) return;
+    const double rho = a.rho[i];
+    const double rt = rho - OG_OBC_OFFSET;
+    const double integral = (double)a.iacc[i] * OG_FIX_INV;
+    const double psi = integral * rt;
+    const double arg = 1.0 * psi - 0.8 * psi * psi + 4.85 * psi * psi * psi;
+    const double inv = 1.0 / rt - tanh(arg) / rho;
+    const double floor_r = rt > 0.5 ? rt : 0.5;
+    double R;
+    if (inv <= 0.0 || !isfinite(inv)) {
+        atomicAdd(a.clamped + a.atom_sys[i], 1);
+        R = floor_r;
+    } else {
+        R = 1.0 / inv;
+        R = R > floor_r ? R : floor_r;
+    }
+    a.radii[i] = R;
+    float4 g = a.gb[a.atom_slot[i]];
+    g.z = (float)R;
+    g.w = (float)(0.60056120439322491 / R);  // sqrt(log2(e)/4) / R
+    a.gb[a.atom_slot[i]] = g;
+    if (ENERGY) {
+        const double q = a.charge[i];
+        if (q != 0.0) {
+            const double pref = -0.5 * (1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
+            og_atomic_add_fix(a.eacc + a.atom_sys[i] * OG_NE + OG_E_GB_SELF, og_to_fix(pref * q * q / R));
+        }
+    }
+}
+
+void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s) {
+    if (a.n_atoms <= 0) return;
+    const int g = (a.n_atoms + 255) / 256;
+    if (energy) hipLaunchKernelGGL(og_k_radii<true>, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(og_k_radii<false>, dim3(g), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al., SC'11) and Box-Muller normals.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t k0, uint32_t k1, uint32_t c[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c[0];
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c[0]);
+        const uint32_t lo1 = 0xCD9E8D57u * c[2];
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c[2]);
+        const uint32_t n0 = hi1 ^ c[1] ^ k0;
+        const uint32_t n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+__device__ __forceinline__ double u53(uint32_t hi, uint32_t lo) {
+    const unsigned long long v = (((unsigned long long)hi << 32) | lo) >> 11;
+    return ((double)v + 0.5) * (1.0 / 9007199254740992.0);
+}
+
+// Three standard normals for (system, atom, step).  Counter layout:
+// (atom_local, step, system, sub) with sub in {0, 1}; key = seed.
+__device__ void noise3(uint32_t k0, uint32_t k1, uint32_t sys, uint32_t atom, uint32_t step,
+                       double& g0, double& g1, double& g2) {
+    uint32_t c[4] = {atom, step, sys, 0u};
+    philox4x32_10(k0, k1, c);
+    double u1 = u53(c[0], c[1]), u2 = u53(c[2], c[3]);
+    double r = sqrt(-2.0 * log(u1)), th = 2.0 * OG_PI * u2;
+    g0 = r * cos(th);
+    g1 = r * sin(th);
+    uint32_t d[4] = {atom, step, sys, 1u};
+    philox4x32_10(k0, k1, d);
+    u1 = u53(d[0], d[1]);
+    u2 = u53(d[2], d[3]);
+    r = sqrt(-2.0 * log(u1));
+    th = 2.0 * OG_PI * u2;
+    g2 = r * cos(th);
+}
+
+// ---------------------------------------------------------------------------
+// Fused BAOAB front half, one thread per constraint cluster (a heavy atom
+// with its X-H partners, or a lone atom).  X-H constraints couple only
+// through their heavy atom, so per-cluster Gauss-Seidel SHAKE reproduces
+// shake_iterate's global sweep (converged constraints are skipped there).
+// B, A, SHAKE, O, A, SHAKE; also zeroes this cluster's force and integral
+// accumulators for the force evaluation that follows.
+// ---------------------------------------------------------------------------
+#define OG_MAX_CL 8
+
+__device__ int shake_cluster(const og_cluster_args& a, int c, double* xs, const double* rs,
+                             const int* at, int na) {
+    const int k0 = a.cn_start[c], k1 = a.cn_start[c + 1];
+    if (k1 == k0) return 1;
+    for (int iter = 1; iter <= 64; ++iter) {
+        double max_err = 0.0;
+        for (int k = k0; k < k1; ++k) {
+            const int gi = a.cn_pair[2 * k], gj = a.cn_pair[2 * k + 1];
+            int li = 0, lj = 0;
+            for (int t = 0; t < na; ++t) {
+                if (at[t] == gi) li = t;
+                if (at[t] == gj) lj = t;
+            }
+            const double d = a.cn_len[k];
+            const double rnx = xs[3 * li] - xs[3 * lj], rny = xs[3 * li + 1] - xs[3 * lj + 1],
+                         rnz = xs[3 * li + 2] - xs[3 * lj + 2];
+            const double err = (rnx * rnx + rny * rny + rnz * rnz) - d * d;
+            if (fabs(err) > max_err) max_err = fabs(err);
+            if (fabs(err) < 1e-6) continue;
+            const double rox = rs[3 * li] - rs[3 * lj], roy = rs[3 * li + 1] - rs[3 * lj + 1],
+                         roz = rs[3 * li + 2] - rs[3 * lj + 2];
+            const double imi = 1.0 / a.mass[gi], imj = 1.0 / a.mass[gj];
+            const double denom = 2.0 * (imi + imj) * (rnx * rox + rny * roy + rnz * roz);
+            if (fabs(denom) < 1e-18) continue;
+            const double lam = err / denom;
+            const double dx = rox * lam, dy = roy * lam, dz = roz * lam;
+            xs[3 * li] -= dx * imi; xs[3 * li + 1] -= dy * imi; xs[3 * li + 2] -= dz * imi;
+            xs[3 * lj] += dx * imj; xs[3 * lj + 1] += dy * imj; xs[3 * lj + 2] += dz * imj;
+        }
+        if (max_err < 1e-6) return 1;
+    }
+    return 0;
+}
+
+__global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n_clusters) return;
+    const int s0 = a.cl_start[c], na = a.cl_start[c + 1] - s0;
+    int at[OG_MAX_CL];
+    double xs[3 * OG_MAX_CL], vs[3 * OG_MAX_CL], rs[3 * OG_MAX_CL];
+    const int sys = a.atom_sys[a.cl_atoms[s0]];
+    const bool frozen = a.diverged[sys] != 0;
+    const double hdt = 0.5 * a.dt;
+    const double kbt = OG_KB * a.sys_temp[sys];
+    const double dofc = a.sys_dofcorr[sys];
+    const double om = 1.0 - a.alpha * a.alpha;
+    for (int t = 0; t < na; ++t) {
+        const int g = a.cl_atoms[s0 + t];
+        at[t] = g;
+        for (int d = 0; d < 3; ++d) {
+            xs[3 * t + d] = a.pos[3 * (size_t)g + d];
+            vs[3 * t + d] = a.vel[3 * (size_t)g + d];
+        }
+        a.facc[3 * (size_t)g] = 0; a.facc[3 * (size_t)g + 1] = 0; a.facc[3 * (size_t)g + 2] = 0;
+        a.iacc[g] = 0;
+    }
+    if (frozen) return;
+    int fails = 0;
+    // B: v += F/m * ACCEL * dt/2
+    for (int t = 0; t < na; ++t) {
+        const int g = at[t];
+        const double k = OG_ACCEL / a.mass[g] * hdt;
+        for (int d = 0; d < 3; ++d) vs[3 * t + d] += a.force[3 * (size_t)g + d] * k;
+    }
+    for (int half = 0; half < 2; ++half) {
+        if (half == 1) {  // O
+            for (int t = 0; t < na; ++t) {
+                const int g = at[t];
+                const double sigma = sqrt(om * kbt * dofc * OG_ACCEL / a.mass[g]);
+                double g0, g1, g2;
+                noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], a.step, g0, g1, g2);
+                vs[3 * t] = a.alpha * vs[3 * t] + sigma * g0;
+                vs[3 * t + 1] = a.alpha * vs[3 * t + 1] + sigma * g1;
+                vs[3 * t + 2] = a.alpha * vs[3 * t + 2] + sigma * g2;
+            }
+        }
+        for (int k = 0; k < 3 * na; ++k) {  // A
+            rs[k] = xs[k];
+            xs[k] += vs[k] * hdt;
+        }
+        if (a.use_shake) {
+            if (!shake_cluster(a, c, xs, rs, at, na)) fails++;
+            for (int k = 0; k < 3 * na; ++k) vs[k] = (xs[k] - rs[k]) / hdt;
+        }
+    }
+    for (int t = 0; t < na; ++t) {
+        const int g = at[t];
+        for (int d = 0; d < 3; ++d) {
+            a.pos[3 * (size_t)g + d] = xs[3 * t + d];
+            a.vel[3 * (size_t)g + d] = vs[3 * t + d];
+        }
+    }
+    if (fails) atomicAdd(a.shake_fail + sys, fails);
+}
+
+void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s) {
+    if (a.n_clusters <= 0) return;
+    hipLaunchKernelGGL(og_k_cluster_step, dim3((a.n_clusters + 255) / 256), dim3(256), 0, s, a);
+}
+
+__global__ void __launch_bounds__(256) og_k_zero_acc(long long* facc, long long* iacc, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    facc[3 * (size_t)i] = 0; facc[3 * (size_t)i + 1] = 0; facc[3 * (size_t)i + 2] = 0;
+    iacc[i] = 0;
+}
+
+void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream_t s) {
+    if (n_atoms <= 0) return;
+    hipLaunchKernelGGL(og_k_zero_acc, dim3((n_atoms + 255) / 256), dim3(256), 0, s, facc, iacc, n_atoms);
+}
+
+// ---------------------------------------------------------------------------
+// Finalise forces (fixed point -> FP64), second half-kick, kinetic energy,
+// divergence flag (langevin.rs:309-319).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_atoms) return;
+    double f[3];
+    for (int d = 0; d < 3; ++d) {
+        f[d] = (double)a.facc[3 * (size_t)i + d] * OG_FIX_INV;
+        a.force[3 * (size_t)i + d] = f[d];
+    }
+    if (!a.do_kick) return;
+    const int sys = a.atom_sys[i];
+    if (a.diverged[sys]) return;
+    const double m = a.mass[i];
+    const double k = OG_ACCEL / m * (0.5 * a.dt);
+    double v2 = 0.0;
+    bool finite = true;
+    for (int d = 0; d < 3; ++d) {
+        const double v = a.vel[3 * (size_t)i + d] + f[d] * k;
+        a.vel[3 * (size_t)i + d] = v;
+        v2 += v * v;
+        finite = finite && isfinite(v);
+    }
+    if (!finite) atomicMax(a.diverged + sys, 1);
+    if (a.ke_acc) og_atomic_add_fix(a.ke_acc + sys, og_to_fix(0.5 * m * v2 / OG_ACCEL));
+}
+
+void og_launch_back(const og_back_args& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return;
+    hipLaunchKernelGGL(og_k_back, dim3((a.n_atoms + 255) / 256), dim3(256), 0, s, a);
+}
+
+__global__ void og_k_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
+                          double sa, double sb) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= 3 * n) return;
+    const size_t ia = 3 * (size_t)off_a + k, ib = 3 * (size_t)off_b + k;
+    double t = pos[ia]; pos[ia] = pos[ib]; pos[ib] = t;
+    t = force[ia]; force[ia] = force[ib]; force[ib] = t;
+    const double va = vel[ib] * sa, vb = vel[ia] * sb;
+    vel[ia] = va;
+    vel[ib] = vb;
+}
+
+void og_launch_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
+                    double scale_a, double scale_b, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(og_k_swap, dim3((3 * n + 255) / 256), dim3(256), 0, s, pos, vel, force,
+                       off_a, off_b, n, scale_a, scale_b);
+}

@@ -1,0 +1,73 @@
+// Shared device-side definitions for the origami MI355X engine (gfx950).
+//
+// Layout conventions used by every kernel:
+//   * Atoms of all systems in a batch are concatenated ("atom index").
+//   * Each system is cut into 64-atom tiles (one tile = one wave64).  The
+//     last tile of a system may be partial; tiles never straddle systems.
+//   * Tile-local FP32 coordinates are stored relative to the FP64 position
+//     of the tile's first atom, so pair offsets keep ~1e-6 Å precision on
+//     chains that span ~1000 Å.
+//   * Forces, descreening integrals and energies are accumulated as int64
+//     fixed point (2^32 units per kJ/mol/Å, per Å^-1, per kJ/mol): integer
+//     addition is associative, so results are bit-reproducible regardless
+//     of wave scheduling.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define OG_TILE 64
+#define OG_FIX_SCALE 4294967296.0        // 2^32
+#define OG_FIX_INV (1.0 / 4294967296.0)
+
+// Physical constants (origami units: Å, kJ/mol, fs, K, Da).
+#define OG_KCAL_TO_KJ 4.184
+#define OG_COULOMB_KCAL 332.0637
+#define OG_EPS_WATER 78.5
+#define OG_OBC_OFFSET 0.09
+#define OG_ACCEL 1.0e-4
+#define OG_KB 8.314462618e-3
+
+// Flags shared with the host.
+#define OG_F_MODE_A 1u        // origami integrator cutoffs (10 Å LJ/RF-Coulomb/GB, 20 Å descreen)
+#define OG_F_SASA 2u          // add analytical SASA forces
+#define OG_F_ENERGY 4u        // accumulate energies
+#define OG_F_SURROGATE 8u     // Born radii from the learned surrogate
+
+struct og_work {              // one tile pair (I <= J) of one system
+    int32_t ti, tj;           // global tile indices
+    int32_t mask;             // index into the exclusion-mask table, -1 if none
+    int32_t sys;              // system index
+};
+
+__device__ __forceinline__ long long og_to_fix(double x) {
+    return __double2ll_rn(x * OG_FIX_SCALE);
+}
+
+__device__ __forceinline__ void og_atomic_add_fix(long long* p, long long v) {
+    atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 cross-lane sum.  Row (16-lane) stages use DPP; the two cross-row
+// stages use ds_swizzle/bpermute through __shfl_xor.  Every lane ends with
+// the full sum.  Deterministic: fixed combination order.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float og_dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__device__ __forceinline__ float og_wave_sum(float v) {
+    v += og_dpp<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += og_dpp<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += og_dpp<0x124>(v);  // row_ror:4
+    v += og_dpp<0x128>(v);  // row_ror:8
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    return v;
+}
+
+__device__ __forceinline__ double og_wave_sum_d(double v) {
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}

@@ -1,0 +1,109 @@
+// Host-visible kernel launchers and argument blocks shared by the engine
+// (engine.cpp) and the kernel translation units (*.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+struct og_work;
+
+struct og_pair_args {
+    const double* pos;               // [N*3] FP64 master positions (atom index space)
+    const float4* par;               // [P] (se, rm, se14, rm14) per padded slot
+    float4* gb;                      // [P] (rho~, S = scale*rho~, R, sqrt(log2e/4)/R)
+    const float* qs;                 // [P] charge * sqrt(k_C * 4.184)
+    const int* tile_atom0;           // [T] first atom index of each tile
+    const int* tile_cnt;             // [T] atoms in each tile (1..64)
+    const og_work* work;             // [n_work] tile pairs
+    int n_work;
+    const unsigned long long* masks; // [n_masked*128]: lanes' 1-2/1-3 bits then 1-4 bits
+    long long* facc;                 // [N*3] force accumulator (fixed point)
+    long long* iacc;                 // [N]   descreening integral accumulator
+    long long* eacc;                 // [S*OG_NE] energy accumulators
+    float rc2_nb, rc2_gb, rc2_ds;    // squared cutoffs (Mode A)
+    float inv_rc3, kappa, crf;       // reaction-field constants (Mode A)
+};
+
+// energy slots per system
+enum {
+    OG_E_BOND = 0, OG_E_ANGLE, OG_E_DIHEDRAL, OG_E_IMPROPER, OG_E_LJ, OG_E_COUL,
+    OG_E_GB_SELF, OG_E_GB_PAIR, OG_E_SASA, OG_E_KIN, OG_NE
+};
+
+struct og_bonded_args {
+    const double* pos;
+    long long* facc;
+    long long* eacc;
+    const int* atom_sys;
+    int nb, na, nd, ni;
+    const int* bonds; const double* bond_p;       // (K, r0)
+    const int* angles; const double* angle_p;     // (K, θ0)
+    const int* dih; const double* dih_p;          // (k, n, δ)
+    const int* imp; const double* imp_p;          // (K, ψ0)
+};
+
+struct og_radii_args {
+    const long long* iacc;
+    const int* atom_sys;
+    const int* atom_slot;            // atom → padded slot
+    const double* rho;               // intrinsic radius ρ
+    const double* charge;
+    float4* gb;
+    double* radii;                   // [N] FP64 effective radii
+    int* clamped;                    // [S]
+    long long* eacc;
+    int n_atoms;
+};
+
+struct og_cluster_args {
+    double* pos; double* vel; double* ref; const double* force;
+    long long* facc; long long* iacc;
+    const double* mass;
+    const int* atom_sys;
+    const int* atom_local;           // index within its system (RNG counter)
+    const int* cl_start;             // [C+1] atoms of cluster c: cl_atoms[cl_start[c]..]
+    const int* cl_atoms;
+    const int* cn_start;             // [C+1] constraints of cluster c
+    const int* cn_pair;              // [2*K] atom pairs
+    const double* cn_len;            // [K]
+    int n_clusters;
+    const double* sys_temp;          // [S]
+    const double* sys_dofcorr;       // [S]
+    int* shake_fail;                 // [S]
+    const int* diverged;             // [S]
+    double dt, alpha;
+    int use_shake;
+    uint32_t key0, key1;
+    uint32_t step;
+};
+
+struct og_back_args {
+    double* vel; double* force;
+    const long long* facc;
+    const double* mass;
+    const int* atom_sys;
+    int n_atoms;
+    double dt;
+    int do_kick;
+    long long* ke_acc;               // [S] or null
+    int* diverged;                   // [S]
+};
+
+void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s);
+void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s);
+
+struct og_energy64_args {            // FP64 per-atom data for the energy pass
+    const double* charge;
+    const double* eps; const double* rmin;      // kcal/mol, Å
+    const double* eps14; const double* rmin14;
+    const double* radii;                        // effective Born radii
+    double rc2_nb, kappa, crf;                  // Mode A cutoff and reaction field
+};
+void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, bool mode_a,
+                             hipStream_t s);
+void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s);
+void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s);
+void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s);
+void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream_t s);
+void og_launch_back(const og_back_args& a, hipStream_t s);
+void og_launch_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
+                    double scale_a, double scale_b, hipStream_t s);

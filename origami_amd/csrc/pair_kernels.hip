@@ -1,0 +1,365 @@
+// All-pairs tile kernels: GB descreening integral and the fused
+// LJ + Coulomb + GB-pair force (optionally with energies).
+//
+// Work decomposition: one wave64 per tile pair (I <= J) of one system,
+// grid-stride over a flattened work list that covers every system of the
+// batch.  Lane l owns i-atom I*64+l; the j loop is wave-uniform, j data is
+// broadcast with v_readlane (SGPR operands), and the j-side reaction
+// (Newton's third law) is reduced across the 64 lanes and parked in lane j's
+// accumulator.  Per tile pair every lane flushes one FP32 partial per
+// component into the int64 fixed-point accumulators.
+//
+// Physics follows origami's SoA kernels:
+//   descreening   energy/src/gb.rs:271-283 (pair), :347-369 (h)
+//   LJ + Coulomb  energy/src/forces_nonbonded.rs:196-232
+//   GB pair force energy/src/forces_gb.rs:72-112
+//   energies      energy/src/nonbonded.rs:97-174, energy/src/gb.rs:146-182
+// Mode B (all pairs) is the fast path; Mode A adds origami's integrator
+// cutoffs (10 Å LJ + reaction-field Coulomb, 10 Å GB force, 20 Å descreen).
+#include "og_device.h"
+#include "og_kernels.h"
+
+namespace {
+
+constexpr float kGbK = (float)(1.0 - 1.0 / OG_EPS_WATER);
+constexpr float kHalfLn2 = 0.34657359027997264f;
+
+struct LaneData {
+    float x, y, z, q;
+    float4 par;   // se, rm, se14, rm14
+    float4 gb;    // rho~, S, R, invRq
+};
+
+__device__ __forceinline__ LaneData load_lane(const og_pair_args& a, int tile, int atom0, int cnt,
+                                              int lane, double ox, double oy, double oz) {
+    LaneData d;
+    const int slot = tile * OG_TILE + lane;
+    if (lane < cnt) {
+        const double* p = a.pos + 3 * (size_t)(atom0 + lane);
+        d.x = (float)(p[0] - ox);
+        d.y = (float)(p[1] - oy);
+        d.z = (float)(p[2] - oz);
+    } else {
+        // Padding slot: far away and (via zero parameters) inert.
+        d.x = 1.0e6f + 97.0f * (float)lane;
+        d.y = 1.0e6f;
+        d.z = 1.0e6f;
+    }
+    d.q = a.qs[slot];
+    d.par = a.par[slot];
+    d.gb = a.gb[slot];
+    return d;
+}
+
+__device__ __forceinline__ float rl(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Descreening integral term h(r, rho~_i, S_j) of energy/src/gb.rs:347-369,
+// written with one reciprocal and one log:  1/L - 1/U = (U-L)/(LU),
+// 1/U^2 - 1/L^2 = -(1/L - 1/U)(U+L)/(LU),  ln(L/U) = ln(L^2/(LU)).
+__device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, float S) {
+    const float U = r + S;
+    const float rmS = r - S;
+    const bool overlap = rmS < rt;
+    const float L = overlap ? rt : rmS;
+    const float UmL = overlap ? U - L : 2.0f * S;
+    const float UpL = overlap ? U + L : 2.0f * r;
+    const float w = __builtin_amdgcn_rcpf(L * U);
+    const float t = UmL * w;
+    const float s2q = S * S * (0.25f * inv_r);
+    const float lg = __builtin_amdgcn_logf(L * L * w);   // log2
+    float h = fmaf(0.5f, t, -(t * UpL * w) * s2q);
+    h = fmaf(lg, kHalfLn2 * inv_r, h);
+    return (U <= rt) ? 0.0f : h;
+}
+
+template <bool MODEA, bool DIAG>
+__device__ __forceinline__ void descreen_tile(const og_pair_args& a, const LaneData& di,
+                                              const LaneData& dj, int lane, int cnt_j,
+                                              float& acc_i, float& acc_j) {
+    for (int jj = 0; jj < cnt_j; ++jj) {
+        const float xj = rl(dj.x, jj), yj = rl(dj.y, jj), zj = rl(dj.z, jj);
+        const float rtj = rl(dj.gb.x, jj), Sj = rl(dj.gb.y, jj);
+        const float dx = xj - di.x, dy = yj - di.y, dz = zj - di.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        float hij = descreen_h(r, inv_r, di.gb.x, Sj);
+        float hji = descreen_h(r, inv_r, rtj, di.gb.y);
+        bool ok = r2 >= 1e-18f;
+        if (MODEA) ok = ok && (r2 <= a.rc2_ds);
+        if (DIAG) ok = ok && (jj > lane);
+        hij = ok ? hij : 0.0f;
+        hji = ok ? hji : 0.0f;
+        acc_i += hij;
+        const float red = og_wave_sum(hji);
+        if (lane == jj) acc_j += red;
+    }
+}
+
+template <bool MODEA>
+__device__ __forceinline__ void descreen_body(const og_pair_args& a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (int w = wave; w < a.n_work; w += n_waves) {
+        const og_work it = a.work[w];
+        const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
+        const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
+        const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],
+                     oz = a.pos[3 * (size_t)a0i + 2];
+        const LaneData di = load_lane(a, it.ti, a0i, ci, lane, ox, oy, oz);
+        float acc_i = 0.0f, acc_j = 0.0f;
+        if (it.ti == it.tj) {
+            descreen_tile<MODEA, true>(a, di, di, lane, cj, acc_i, acc_j);
+        } else {
+            const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
+            descreen_tile<MODEA, false>(a, di, dj, lane, cj, acc_i, acc_j);
+        }
+        if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix((double)acc_i));
+        if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix((double)acc_j));
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) og_k_descreen_b(og_pair_args a) { descreen_body<false>(a); }
+__global__ void __launch_bounds__(256) og_k_descreen_a(og_pair_args a) { descreen_body<true>(a); }
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Fused force tile.
+// ---------------------------------------------------------------------------
+struct ForceAcc {
+    float fx, fy, fz;       // i-side
+    float gx, gy, gz;       // j-side (lane j)
+};
+
+template <bool MODEA, bool MASKED, bool DIAG>
+__device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData& di,
+                                           const LaneData& dj, int lane, int cnt_j,
+                                           unsigned long long xmask, unsigned long long m14,
+                                           ForceAcc& acc) {
+    for (int jj = 0; jj < cnt_j; ++jj) {
+        const float xj = rl(dj.x, jj), yj = rl(dj.y, jj), zj = rl(dj.z, jj);
+        const float qj = rl(dj.q, jj);
+        const float sej = rl(dj.par.x, jj), rmj = rl(dj.par.y, jj);
+        const float Rj = rl(dj.gb.z, jj), iRj = rl(dj.gb.w, jj);
+        const float dx = xj - di.x, dy = yj - di.y, dz = zj - di.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float inv_r2 = inv_r * inv_r;
+        float rmin, eij;
+        bool excl = false;
+        if (MASKED) {
+            const float se14j = rl(dj.par.z, jj), rm14j = rl(dj.par.w, jj);
+            excl = (xmask >> jj) & 1ull;
+            const bool one4 = (m14 >> jj) & 1ull;
+            rmin = one4 ? di.par.w + rm14j : di.par.y + rmj;
+            eij = one4 ? di.par.z * se14j : di.par.x * sej;
+        } else {
+            rmin = di.par.y + rmj;
+            eij = di.par.x * sej;
+        }
+        const float t2 = rmin * rmin * inv_r2;
+        const float t6 = t2 * t2 * t2;
+        const float lj = eij * inv_r2 * (t6 - t6 * t6);
+        const float qq = di.q * qj;
+        float coul;
+        if (MODEA) coul = -qq * (inv_r2 * inv_r - a.inv_rc3);
+        else coul = -qq * inv_r2 * inv_r;
+        float cnb = lj + coul;
+        if (MODEA) cnb = (r2 <= a.rc2_nb) ? cnb : 0.0f;
+        if (MASKED) cnb = excl ? 0.0f : cnb;
+        // GB pair (frozen radii): f^2 = r^2 + R_i R_j exp(-r^2 / (4 R_i R_j))
+        const float Rp = di.gb.z * Rj;
+        const float e = __builtin_amdgcn_exp2f(-r2 * (di.gb.w * iRj));
+        const float f2 = fmaf(Rp, e, r2);
+        const float inv_f = __builtin_amdgcn_rsqf(f2);
+        const float qgb = kGbK * qq;
+        float gbc = qgb * fmaf(-0.25f, e, 1.0f) * (inv_f * inv_f * inv_f);
+        if (MODEA) gbc = (r2 <= a.rc2_gb) ? gbc : 0.0f;
+        float c = cnb + gbc;
+        bool ok = r2 >= 1e-18f;
+        if (DIAG) ok = ok && (jj > lane);
+        c = ok ? c : 0.0f;
+        const float fx = c * dx, fy = c * dy, fz = c * dz;
+        acc.fx += fx;
+        acc.fy += fy;
+        acc.fz += fz;
+        const float sx = og_wave_sum(fx);
+        const float sy = og_wave_sum(fy);
+        const float sz = og_wave_sum(fz);
+        if (lane == jj) {
+            acc.gx -= sx;
+            acc.gy -= sy;
+            acc.gz -= sz;
+        }
+    }
+}
+
+template <bool MODEA>
+__device__ __forceinline__ void force_body(const og_pair_args& a) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (int w = wave; w < a.n_work; w += n_waves) {
+        const og_work it = a.work[w];
+        const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
+        const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
+        const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],
+                     oz = a.pos[3 * (size_t)a0i + 2];
+        const LaneData di = load_lane(a, it.ti, a0i, ci, lane, ox, oy, oz);
+        ForceAcc acc = {0, 0, 0, 0, 0, 0};
+        unsigned long long xm = 0, m14 = 0;
+        if (it.mask >= 0) {
+            xm = a.masks[(size_t)it.mask * 128 + lane];
+            m14 = a.masks[(size_t)it.mask * 128 + 64 + lane];
+        }
+        if (it.ti == it.tj) {
+            force_tile<MODEA, true, true>(a, di, di, lane, cj, xm, m14, acc);
+        } else {
+            const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
+            if (it.mask >= 0)
+                force_tile<MODEA, true, false>(a, di, dj, lane, cj, xm, m14, acc);
+            else
+                force_tile<MODEA, false, false>(a, di, dj, lane, cj, xm, m14, acc);
+        }
+        if (lane < ci) {
+            long long* f = a.facc + 3 * (size_t)(a0i + lane);
+            og_atomic_add_fix(f + 0, og_to_fix((double)acc.fx));
+            og_atomic_add_fix(f + 1, og_to_fix((double)acc.fy));
+            og_atomic_add_fix(f + 2, og_to_fix((double)acc.fz));
+        }
+        if (lane < cj) {
+            long long* f = a.facc + 3 * (size_t)(a0j + lane);
+            og_atomic_add_fix(f + 0, og_to_fix((double)acc.gx));
+            og_atomic_add_fix(f + 1, og_to_fix((double)acc.gy));
+            og_atomic_add_fix(f + 2, og_to_fix((double)acc.gz));
+        }
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) og_k_force_b(og_pair_args a) { force_body<false>(a); }
+__global__ void __launch_bounds__(256) og_k_force_a(og_pair_args a) { force_body<true>(a); }
+
+static int pair_grid(int n_work) {
+    int blocks = (n_work + 3) / 4;
+    const int cap = 256 * 16;
+    if (blocks > cap) blocks = cap;
+    return blocks < 1 ? 1 : blocks;
+}
+
+void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s) {
+    if (a.n_work <= 0) return;
+    const int g = pair_grid(a.n_work);
+    if (mode_a) hipLaunchKernelGGL(og_k_descreen_a, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(og_k_descreen_b, dim3(g), dim3(256), 0, s, a);
+}
+
+void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
+    if (a.n_work <= 0) return;
+    const int g = pair_grid(a.n_work);
+    if (mode_a) hipLaunchKernelGGL(og_k_force_a, dim3(g), dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(og_k_force_b, dim3(g), dim3(256), 0, s, a);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 pair energies (LJ, Coulomb, GB pair), used for energy totals, REMD
+// exchange and the L-BFGS line search.  Same tile schedule; all arithmetic in
+// double, evaluated as origami writes it:
+//   LJ       eps_ij ((Rmin/r)^12 - 2 (Rmin/r)^6)         nonbonded.rs:410-415
+//   Coulomb  k q_i q_j (1/r + kappa r^2 - c)  (Mode B: k q_i q_j / r)  nonbonded.rs:327-329
+//   GB pair  2 (-1/2)(1 - 1/eps_w) k q_i q_j / f_GB, uncut  gb.rs:160-173
+// ---------------------------------------------------------------------------
+namespace {
+
+__device__ __forceinline__ double rld(double v, int j) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <bool MODEA>
+__device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    const double kc = OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
+    const double pref = -(1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
+    for (int w = wave; w < a.n_work; w += n_waves) {
+        const og_work it = a.work[w];
+        const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
+        const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
+        const bool vi = lane < ci, vj = lane < cj;
+        const int gi = a0i + (vi ? lane : 0), gj = a0j + (vj ? lane : 0);
+        const double xi = a.pos[3 * (size_t)gi], yi = a.pos[3 * (size_t)gi + 1], zi = a.pos[3 * (size_t)gi + 2];
+        const double xjv = a.pos[3 * (size_t)gj], yjv = a.pos[3 * (size_t)gj + 1], zjv = a.pos[3 * (size_t)gj + 2];
+        const double qi = e.charge[gi], qjv = e.charge[gj];
+        const double epi = e.eps[gi], rmi = e.rmin[gi], ep14i = e.eps14[gi], rm14i = e.rmin14[gi];
+        const double epjv = e.eps[gj], rmjv = e.rmin[gj], ep14jv = e.eps14[gj], rm14jv = e.rmin14[gj];
+        const double Ri = e.radii[gi], Rjv = e.radii[gj];
+        unsigned long long xm = 0, m14 = 0;
+        if (it.mask >= 0) {
+            xm = a.masks[(size_t)it.mask * 128 + lane];
+            m14 = a.masks[(size_t)it.mask * 128 + 64 + lane];
+        }
+        double elj = 0.0, ec = 0.0, eg = 0.0;
+        for (int jj = 0; jj < cj; ++jj) {
+            const double dx = xi - rld(xjv, jj), dy = yi - rld(yjv, jj), dz = zi - rld(zjv, jj);
+            const double r2 = dx * dx + dy * dy + dz * dz;
+            bool ok = vi;
+            if (it.ti == it.tj) ok = ok && (jj > lane);
+            if (!ok) continue;
+            const double qq = qi * rld(qjv, jj);
+            const bool excl = (xm >> jj) & 1ull;
+            bool nb = !excl;
+            if (MODEA) nb = nb && (r2 <= (double)e.rc2_nb);
+            if (nb) {
+                const bool one4 = (m14 >> jj) & 1ull;
+                const double rmij = one4 ? rm14i + rld(rm14jv, jj) : rmi + rld(rmjv, jj);
+                const double eij = sqrt(one4 ? ep14i * rld(ep14jv, jj) : epi * rld(epjv, jj));
+                const double r = sqrt(r2);
+                const double t = rmij / r;
+                const double t2 = t * t;
+                const double t6 = t2 * t2 * t2;
+                elj += eij * (t6 * t6 - 2.0 * t6);
+                if (qq != 0.0) {
+                    if (MODEA) ec += qq * (1.0 / r + e.kappa * r * r - e.crf);
+                    else ec += qq * (1.0 / r);
+                }
+            }
+            if (qq != 0.0) {
+                const double rp = Ri * rld(Rjv, jj);
+                const double f = sqrt(r2 + rp * exp(-r2 / (4.0 * rp)));
+                eg += qq / f;
+            }
+        }
+        elj = og_wave_sum_d(elj * OG_KCAL_TO_KJ);
+        ec = og_wave_sum_d(ec * kc);
+        eg = og_wave_sum_d(eg * pref);
+        if (lane == 0) {
+            long long* acc = a.eacc + (size_t)it.sys * OG_NE;
+            og_atomic_add_fix(acc + OG_E_LJ, og_to_fix(elj));
+            og_atomic_add_fix(acc + OG_E_COUL, og_to_fix(ec));
+            og_atomic_add_fix(acc + OG_E_GB_PAIR, og_to_fix(eg));
+        }
+    }
+}
+
+}  // namespace
+
+__global__ void __launch_bounds__(256) og_k_energy64_b(og_pair_args a, og_energy64_args e) { energy_body<false>(a, e); }
+__global__ void __launch_bounds__(256) og_k_energy64_a(og_pair_args a, og_energy64_args e) { energy_body<true>(a, e); }
+
+void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, bool mode_a, hipStream_t s) {
+    if (a.n_work <= 0) return;
+    const int g = pair_grid(a.n_work);
+    if (mode_a) hipLaunchKernelGGL(og_k_energy64_a, dim3(g), dim3(256), 0, s, a, e);
+    else hipLaunchKernelGGL(og_k_energy64_b, dim3(g), dim3(256), 0, s, a, e);
+}

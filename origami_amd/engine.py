@@ -1,0 +1,419 @@
+"""Python host mirror of origami's force/dynamics interface over the C ABI.
+
+The names follow origami's own entry points so parity tests read like
+origami's tests:
+
+  ``Engine``                       ForceScratch (energy/src/scratch.rs:105-262)
+  ``Engine.total_force``           total_force_with_scratch (energy/src/forces.rs:78-128)
+  ``Engine.total_energy``          total_energy_with_options (dynamics/src/energy_eval.rs:20-34)
+  ``Engine.born_radii``            compute_born_radii_into_scratch (energy/src/gb.rs:203-342)
+  ``run_langevin``                 dynamics/src/langevin.rs:128-367
+  ``run_remd``                     dynamics/src/remd.rs:136-498
+
+Everything computes on the GPU through ``liborigami_gpu.so``; there is no CPU
+fallback.  Importing this module without the built library raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import system as S
+
+_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_lib", "liborigami_gpu.so")
+
+MODE_A = 1
+SASA = 2
+ENERGY = 4
+SURROGATE = 8
+
+ENERGY_TERMS = ["bond", "angle", "dihedral", "improper", "lj", "coulomb", "gb_self", "gb_pair",
+                "sasa", "kinetic"]
+KERNEL_CLASSES = ["cluster", "bonded", "descreen", "radii", "force", "sasa", "back", "surrogate",
+                  "other"]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [("n_atoms", C.c_int32), ("pos", _dp), ("charge", _dp), ("rmin_half", _dp),
+                ("epsilon", _dp), ("rmin_half_14", _dp), ("epsilon_14", _dp), ("gb_rho", _dp),
+                ("gb_scale", _dp), ("sasa_radius", _dp), ("sasa_gamma", _dp), ("mass", _dp),
+                ("n_excl", C.c_int32), ("excl", _ip), ("n_14", C.c_int32), ("pair14", _ip),
+                ("n_bonds", C.c_int32), ("bonds", _ip), ("bond_params", _dp),
+                ("n_angles", C.c_int32), ("angles", _ip), ("angle_params", _dp),
+                ("n_dihedrals", C.c_int32), ("dihedrals", _ip), ("dihedral_params", _dp),
+                ("n_impropers", C.c_int32), ("impropers", _ip), ("improper_params", _dp),
+                ("n_constraints", C.c_int32), ("constraints", _ip), ("constraint_len", _dp)]
+
+
+class DynamicsParams(C.Structure):
+    _fields_ = [("dt_fs", C.c_double), ("friction_ps_inv", C.c_double), ("seed", C.c_uint64),
+                ("constrain_h_bonds", C.c_int32), ("flags", C.c_uint32),
+                ("sasa_gamma_scale", C.c_double)]
+
+
+class EngineError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP engine library; raise if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise EngineError("HIP engine library not built: %s (run python -m origami_amd.build)"
+                              % _LIB_PATH)
+        L = C.CDLL(_LIB_PATH)
+        vp = C.c_void_p
+        L.og_last_error.restype = C.c_char_p
+        L.og_engine_create.argtypes = [C.POINTER(SystemDesc), C.c_int32, C.POINTER(vp)]
+        L.og_engine_destroy.argtypes = [vp]
+        L.og_engine_destroy.restype = None
+        L.og_engine_info.argtypes = [vp, C.POINTER(C.c_int64)]
+        for nm in ("og_set_positions", "og_get_positions", "og_set_velocities",
+                   "og_get_velocities", "og_get_forces"):
+            getattr(L, nm).argtypes = [vp, _dp]
+        L.og_forces.argtypes = [vp, _dp, _dp, C.c_uint32, C.c_double, _dp]
+        L.og_energies.argtypes = [vp, _dp, C.c_uint32, _dp]
+        L.og_born_radii.argtypes = [vp, _dp, _dp]
+        L.og_dynamics_setup.argtypes = [vp, C.POINTER(DynamicsParams), _dp]
+        L.og_run.argtypes = [vp, C.c_int32, _dp]
+        L.og_set_temperature.argtypes = [vp, C.c_int32, C.c_double]
+        L.og_swap_systems.argtypes = [vp, C.c_int32, C.c_int32]
+        L.og_get_status.argtypes = [vp, _ip, _ip, _ip]
+        L.og_profile_enable.argtypes = [vp, C.c_int32]
+        L.og_profile_read.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise EngineError("origami engine error %d: %s" % (rc, lib().og_last_error().decode()))
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+class Engine:
+    """A device-resident batch of one or more systems (chains / replicas)."""
+
+    def __init__(self, systems: Sequence[S.ResolvedSystem]):
+        if isinstance(systems, S.ResolvedSystem):
+            systems = [systems]
+        L = lib()
+        self.systems = list(systems)
+        self.n_systems = len(self.systems)
+        self.sizes = [s.n_atoms for s in self.systems]
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n_atoms = int(self.offsets[-1])
+        self._keep = []
+        descs = (SystemDesc * self.n_systems)()
+
+        def d(x, w=None):
+            a = np.ascontiguousarray(x, dtype=np.float64)
+            self._keep.append(a)
+            return _d(a)
+
+        def i(x):
+            a = np.ascontiguousarray(x, dtype=np.int32)
+            self._keep.append(a)
+            return _i(a)
+
+        for k, s in enumerate(self.systems):
+            D = descs[k]
+            D.n_atoms = s.n_atoms
+            D.pos, D.charge = d(s.pos), d(s.charge)
+            D.rmin_half, D.epsilon = d(s.rmin_half), d(s.epsilon)
+            D.rmin_half_14, D.epsilon_14 = d(s.rmin_half_14), d(s.epsilon_14)
+            D.gb_rho, D.gb_scale = d(s.gb_rho), d(s.gb_scale)
+            D.sasa_radius, D.sasa_gamma, D.mass = d(s.sasa_radius), d(s.sasa_gamma), d(s.mass)
+            D.n_excl, D.excl = len(s.excl_pairs), i(s.excl_pairs)
+            D.n_14, D.pair14 = len(s.pair14), i(s.pair14)
+            D.n_bonds, D.bonds, D.bond_params = len(s.bonds), i(s.bonds), d(s.bond_params)
+            D.n_angles, D.angles, D.angle_params = len(s.angles), i(s.angles), d(s.angle_params)
+            D.n_dihedrals, D.dihedrals, D.dihedral_params = (len(s.dihedrals), i(s.dihedrals),
+                                                             d(s.dihedral_params))
+            D.n_impropers, D.impropers, D.improper_params = (len(s.impropers), i(s.impropers),
+                                                             d(s.improper_params))
+            D.n_constraints, D.constraints, D.constraint_len = (len(s.constraints),
+                                                                i(s.constraints),
+                                                                d(s.constraint_len))
+        h = C.c_void_p()
+        _check(L.og_engine_create(descs, self.n_systems, C.byref(h)))
+        self._h = h
+        self._keep = []
+        self.temperatures = np.zeros(self.n_systems)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().og_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- info / state -------------------------------------------------------
+    def info(self):
+        out = (C.c_int64 * 8)()
+        _check(lib().og_engine_info(self._h, out))
+        keys = ["atoms", "systems", "tiles", "tile_pairs", "masked_tile_pairs", "clusters",
+                "pairs", "sasa_work_atoms"]
+        return dict(zip(keys, [int(x) for x in out]))
+
+    def _state(self, getter):
+        out = np.zeros((self.n_atoms, 3))
+        _check(getter(self._h, _d(out)))
+        return out
+
+    def positions(self):
+        return self._state(lib().og_get_positions)
+
+    def velocities(self):
+        return self._state(lib().og_get_velocities)
+
+    def forces(self):
+        return self._state(lib().og_get_forces)
+
+    def set_positions(self, xyz):
+        a = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_set_positions(self._h, _d(a)))
+
+    def set_velocities(self, v):
+        a = np.ascontiguousarray(v, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_set_velocities(self._h, _d(a)))
+
+    def split(self, arr):
+        return [arr[self.offsets[k]:self.offsets[k + 1]] for k in range(self.n_systems)]
+
+    # -- forces / energies ----------------------------------------------------
+    def total_force(self, xyz=None, mode_a=False, include_sasa=False, sasa_gamma_scale=1.0,
+                    energies=False):
+        flags = (MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
+        f = np.zeros((self.n_atoms, 3))
+        e = np.zeros((self.n_systems, len(ENERGY_TERMS))) if energies else None
+        x = None
+        if xyz is not None:
+            x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_forces(self._h, _d(x) if x is not None else None, _d(f), flags,
+                               float(sasa_gamma_scale), _d(e) if e is not None else None))
+        if energies:
+            return f, [dict(zip(ENERGY_TERMS, row)) for row in e]
+        return f
+
+    def total_energy(self, xyz=None, mode_a=False):
+        e = np.zeros((self.n_systems, len(ENERGY_TERMS)))
+        x = None
+        if xyz is not None:
+            x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_energies(self._h, _d(x) if x is not None else None,
+                                 MODE_A if mode_a else 0, _d(e)))
+        out = []
+        for row in e:
+            dct = dict(zip(ENERGY_TERMS, row.tolist()))
+            dct["gb"] = dct["gb_self"] + dct["gb_pair"]
+            dct["total"] = (dct["bond"] + dct["angle"] + dct["dihedral"] + dct["improper"]
+                            + dct["lj"] + dct["coulomb"] + dct["gb"] + dct["sasa"])
+            out.append(dct)
+        return out
+
+    def born_radii(self):
+        r = np.zeros(self.n_atoms)
+        integ = np.zeros(self.n_atoms)
+        _check(lib().og_born_radii(self._h, _d(r), _d(integ)))
+        return r, integ
+
+    # -- dynamics ---------------------------------------------------------------
+    def dynamics_setup(self, temperatures, dt_fs=1.0, friction_ps_inv=1.0, seed=0,
+                       constrain_h_bonds=False, mode_a=False, include_sasa=False,
+                       sasa_gamma_scale=1.0):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperatures, dtype=np.float64),
+                                                 (self.n_systems,)))
+        self.temperatures = t.copy()
+        p = DynamicsParams(dt_fs, friction_ps_inv, seed & ((1 << 64) - 1),
+                           1 if constrain_h_bonds else 0,
+                           (MODE_A if mode_a else 0) | (SASA if include_sasa else 0),
+                           sasa_gamma_scale)
+        self.dt_fs = dt_fs
+        _check(lib().og_dynamics_setup(self._h, C.byref(p), _d(t)))
+
+    def run(self, n_steps, record_ke=False):
+        ke = np.zeros((n_steps, self.n_systems)) if record_ke and n_steps > 0 else None
+        rc = lib().og_run(self._h, int(n_steps), _d(ke) if ke is not None else None)
+        if rc == -4:
+            raise EngineError("diverged: " + lib().og_last_error().decode())
+        _check(rc)
+        return ke
+
+    def set_temperature(self, system, t):
+        _check(lib().og_set_temperature(self._h, int(system), float(t)))
+        self.temperatures[system] = t
+
+    def swap(self, a, b):
+        _check(lib().og_swap_systems(self._h, int(a), int(b)))
+
+    def status(self):
+        n = self.n_systems
+        div, fails, cl = (np.zeros(n, np.int32) for _ in range(3))
+        _check(lib().og_get_status(self._h, _i(div), _i(fails), _i(cl)))
+        return {"diverged": div, "shake_failures": fails, "clamped": cl}
+
+    # -- profiling --------------------------------------------------------------
+    def profile(self, on=True):
+        _check(lib().og_profile_enable(self._h, 1 if on else 0))
+
+    def profile_read(self):
+        ms = np.zeros(len(KERNEL_CLASSES))
+        cnt = np.zeros(len(KERNEL_CLASSES), np.int64)
+        _check(lib().og_profile_read(self._h, _d(ms), cnt.ctypes.data_as(C.POINTER(C.c_int64))))
+        return {k: (float(ms[i]), int(cnt[i])) for i, k in enumerate(KERNEL_CLASSES)}
+
+
+# ---------------------------------------------------------------------------
+# Drivers
+# ---------------------------------------------------------------------------
+def maxwell_boltzmann(sys: S.ResolvedSystem, temperature_k: float, rng: S.Xoshiro256pp,
+                      remove_com: bool = True):
+    """Initial velocities (langevin.rs:487-513; remd.rs:206 uses no COM removal)."""
+    kbt = S.BOLTZMANN_KJ * temperature_k
+    v = np.zeros((sys.n_atoms, 3))
+    for i, m in enumerate(sys.mass):
+        sigma = math.sqrt(kbt * S.ACCEL_FACTOR / m)
+        v[i, 0] = sigma * rng.gaussian()
+        v[i, 1] = sigma * rng.gaussian()
+        v[i, 2] = sigma * rng.gaussian()
+    if remove_com:
+        p = (v * sys.mass[:, None]).sum(0)
+        v -= p / sys.mass.sum()
+    return v
+
+
+def kinetic_temperature(ke_kj, n_atoms):
+    return 2.0 * ke_kj / (3.0 * n_atoms * S.BOLTZMANN_KJ)
+
+
+@dataclass
+class LangevinSummary:
+    steps_run: int
+    temperature_mean_k: float
+    temperature_stddev_k: float
+    equipartition_ratio: float
+    final_kinetic_energy_kj_mol: float
+    atoms_count: int
+    diverged: bool
+    shake_failures: int
+
+
+def run_langevin(sys: S.ResolvedSystem, steps: int, dt_fs=1.0, temperature_k=310.0,
+                 friction_ps_inv=1.0, seed=0, randomise_initial_velocities=True,
+                 constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0,
+                 mode_a=False, engine: Optional[Engine] = None):
+    """BAOAB Langevin for one system on the GPU (langevin.rs:128-367)."""
+    eng = engine or Engine([sys])
+    if randomise_initial_velocities:
+        eng.set_velocities(maxwell_boltzmann(sys, temperature_k, S.Xoshiro256pp(seed)))
+    eng.dynamics_setup([temperature_k], dt_fs=dt_fs, friction_ps_inv=friction_ps_inv, seed=seed,
+                       constrain_h_bonds=constrain_h_bonds, mode_a=mode_a,
+                       include_sasa=include_sasa, sasa_gamma_scale=sasa_gamma_scale)
+    diverged = False
+    try:
+        ke = eng.run(steps, record_ke=True)
+    except EngineError:
+        diverged = True
+        ke = np.zeros((0, 1))
+    st = eng.status()
+    t = kinetic_temperature(ke[:, 0], sys.n_atoms) if len(ke) else np.zeros(0)
+    ke_final = float(ke[-1, 0]) if len(ke) else 0.0
+    dof = 3.0 * sys.n_atoms
+    return eng, LangevinSummary(
+        steps_run=len(t), temperature_mean_k=float(t.mean()) if len(t) else 0.0,
+        temperature_stddev_k=float(t.std()) if len(t) else 0.0,
+        equipartition_ratio=ke_final / (0.5 * dof * S.BOLTZMANN_KJ * temperature_k),
+        final_kinetic_energy_kj_mol=ke_final, atoms_count=sys.n_atoms,
+        diverged=diverged or bool(st["diverged"][0]), shake_failures=int(st["shake_failures"][0]))
+
+
+@dataclass
+class RemdSummary:
+    n_replicas: int
+    atoms_count: int
+    swap_attempts: List[int]
+    swap_accepts: List[int]
+    temperatures: List[float]
+    final_potential_energy: List[float] = field(default_factory=list)
+
+    def acceptance_ratios(self):
+        return [a / t if t else 0.0 for a, t in zip(self.swap_accepts, self.swap_attempts)]
+
+
+def geometric_ladder(t_min, t_max, n):
+    if n == 1:
+        return [t_min]
+    return [t_min * (t_max / t_min) ** (k / (n - 1)) for k in range(n)]
+
+
+def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs: float,
+             swap_interval_fs: float, dt_fs=1.0, friction_ps_inv=2.0, seed=0,
+             constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0, mode_a=False,
+             engine: Optional[Engine] = None):
+    """Parallel tempering with every replica in one device batch (remd.rs:136-498).
+
+    Replicas advance together in the batched step; swap decisions use the
+    replicas' potential energies, origami's swap RNG (seed + 0xDEADBEEF) and
+    Metropolis rule on alternating adjacent pairs, then exchange positions,
+    velocities and forces with the sqrt(T_a/T_b) velocity rescale.
+    """
+    temps = sorted(float(t) for t in temperatures)
+    n_rep = len(temps)
+    eng = engine or Engine([sys] * n_rep)
+    v0 = []
+    for i, t in enumerate(temps):
+        rng = S.Xoshiro256pp((seed + i * 0xC0FFEE) & ((1 << 64) - 1))
+        v0.append(maxwell_boltzmann(sys, t, rng, remove_com=False))
+    eng.set_velocities(np.concatenate(v0))
+    eng.dynamics_setup(temps, dt_fs=dt_fs, friction_ps_inv=friction_ps_inv, seed=seed,
+                       constrain_h_bonds=constrain_h_bonds, mode_a=mode_a,
+                       include_sasa=include_sasa, sasa_gamma_scale=sasa_gamma_scale)
+    steps_per_swap = int(round(swap_interval_fs / dt_fs))
+    total_steps = int(round(total_time_fs / dt_fs))
+    n_rounds = -(-total_steps // max(steps_per_swap, 1))
+    attempts = [0] * (n_rep - 1)
+    accepts = [0] * (n_rep - 1)
+    swap_rng = S.Xoshiro256pp((seed + 0xDEADBEEF) & ((1 << 64) - 1))
+    for rnd in range(n_rounds):
+        n = min(steps_per_swap, total_steps - rnd * steps_per_swap)
+        if n <= 0:
+            break
+        eng.run(n)
+        energies = [e["total"] for e in eng.total_energy(mode_a=mode_a)]
+        for pair in range(rnd % 2, n_rep - 1, 2):
+            a, b = pair, pair + 1
+            attempts[pair] += 1
+            beta_a = 1.0 / (S.BOLTZMANN_KJ * temps[a])
+            beta_b = 1.0 / (S.BOLTZMANN_KJ * temps[b])
+            delta = (beta_a - beta_b) * (energies[a] - energies[b])
+            p_acc = 1.0 if delta >= 0.0 else math.exp(delta)
+            if swap_rng.next_f64() < p_acc:
+                accepts[pair] += 1
+                eng.swap(a, b)
+                energies[a], energies[b] = energies[b], energies[a]
+    final = [e["total"] for e in eng.total_energy(mode_a=mode_a)]
+    return eng, RemdSummary(n_rep, sys.n_atoms, attempts, accepts, temps, final)

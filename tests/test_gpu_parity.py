@@ -1,0 +1,249 @@
+"""GPU parity: the HIP engine against the FP64 oracle on identical inputs.
+
+Bar (from the engine's contract): FP32 pair arithmetic with int64
+fixed-point accumulation must stay within 1e-5 relative force RMS of the
+FP64 oracle; FP64 paths (bonded, SHAKE, integrator, SASA) within 1e-9.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from origami_amd import system as S
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RMS_TOL = 1e-5
+
+
+def rel_rms(a, b):
+    return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+@pytest.fixture(scope="module")
+def E():
+    from origami_amd import engine
+    return engine
+
+
+def test_library_is_native(E):
+    import ctypes
+    lib = E.lib()
+    assert isinstance(lib, ctypes.CDLL)
+    assert lib._name.endswith("liborigami_gpu.so")
+
+
+@pytest.mark.parametrize("mode_a", [False, True])
+def test_total_force_c1(E, c1_system, mode_a):
+    s = c1_system
+    orc = O.Oracle(s)
+    cut = O.MODE_A if mode_a else O.MODE_B
+    fref, terms = orc.total_force(s.pos, terms=True, **cut)
+    eng = E.Engine([s])
+    f = eng.total_force(mode_a=mode_a)
+    err = rel_rms(f, fref)
+    print("C1 mode_a=%s rel force RMS %.3e" % (mode_a, err))
+    assert err < FORCE_RMS_TOL
+    # Newton's third law: every term is pairwise
+    assert np.abs(f.sum(0)).max() < 1e-6 * np.abs(fref).max()
+
+
+def test_born_radii_and_integral(E, c2_system):
+    s = c2_system
+    orc = O.Oracle(s)
+    integ_ref, radii_ref, clamped = orc.born_radii(s.pos, math.inf)
+    eng = E.Engine([s])
+    eng.total_force()
+    radii, integ = eng.born_radii()
+    assert np.array_equal(radii, s.gb_rho - 0.09)       # R_i == rho~_i exactly
+    assert np.allclose(radii, radii_ref, rtol=0, atol=0)
+    rel = np.abs(integ - integ_ref) / np.abs(integ_ref)
+    print("descreening integral max rel err %.3e" % rel.max())
+    assert rel.max() < 1e-4
+    assert eng.status()["clamped"][0] == 0
+
+
+@pytest.mark.parametrize("mode_a", [False, True])
+def test_total_force_c2(E, c2_system, mode_a):
+    s = c2_system
+    orc = O.Oracle(s)
+    cut = O.MODE_A if mode_a else O.MODE_B
+    fref = orc.total_force(s.pos, **cut)
+    eng = E.Engine([s])
+    f = eng.total_force(mode_a=mode_a)
+    err = rel_rms(f, fref)
+    print("C2 mode_a=%s rel force RMS %.3e" % (mode_a, err))
+    assert err < FORCE_RMS_TOL
+
+
+def jittered(s, seed, scale=0.02):
+    """Generic geometry: the builder's exact symmetries removed."""
+    return s.pos + np.random.default_rng(seed).normal(scale=scale, size=s.pos.shape)
+
+
+def test_sasa_forces_c2_generic_geometry(E, c2_system):
+    s = c2_system
+    pos = jittered(s, 2)
+    orc = O.Oracle(s)
+    fs, stats = orc.sasa_forces(pos, 0.25)
+    eng = E.Engine([s])
+    f0 = eng.total_force(pos)
+    f1 = eng.total_force(pos, include_sasa=True, sasa_gamma_scale=0.25)
+    err = np.abs((f1 - f0) - fs).max()
+    print("SASA max abs diff %.3e (max |F| %.3e), stats %s" % (err, np.abs(fs).max(), stats))
+    assert err < 1e-8       # FP64 kernel; int64 fixed-point resolution 2.3e-10
+    fref = orc.total_force(pos) + fs
+    assert rel_rms(f1, fref) < FORCE_RMS_TOL
+
+
+def test_sasa_total_force_c2_built_geometry(E, c2_system):
+    # The freshly built chain is exactly symmetric, so a handful of cap
+    # triple points sit on origami's 1e-12 topology tolerances and single
+    # SASA terms there depend on last-ulp rounding; the total force still
+    # meets the bar.
+    s = c2_system
+    orc = O.Oracle(s)
+    fref = orc.total_force(s.pos) + orc.sasa_forces(s.pos, 0.25)[0]
+    eng = E.Engine([s])
+    f1 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    err = rel_rms(f1, fref)
+    print("C2 + SASA (built geometry) rel force RMS %.3e" % err)
+    assert err < FORCE_RMS_TOL
+
+
+def test_energies_c2(E, c2_system):
+    s = c2_system
+    orc = O.Oracle(s)
+    for mode_a in (False, True):
+        cut = O.MODE_A if mode_a else O.MODE_B
+        ref = orc.total_energy(s.pos, cut_nb=cut["cut_nb"], cut_descreen=cut["cut_descreen"])
+        eng = E.Engine([s])
+        got = eng.total_energy(mode_a=mode_a)[0]
+        for k in ("bond", "angle", "dihedral", "improper"):
+            assert abs(got[k] - ref[k]) <= 1e-8 * max(1.0, abs(ref[k])), k
+        print("mode_a=%s" % mode_a, {k: (got[k], ref[k]) for k in ref if k in got})
+        for k in ("lj", "coulomb"):
+            assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+        assert abs(got["gb_self"] - ref["gb_self"]) <= 1e-9 * abs(ref["gb_self"])
+        assert abs(got["gb_pair"] - ref["gb_pair"]) <= 1e-4 * abs(ref["gb_pair"])
+        assert abs(got["total"] - ref["total"]) <= 1e-4 * max(1.0, abs(ref["total"]))
+
+
+def test_uneven_batch_matches_single(E):
+    rng = S.Xoshiro256pp(4)
+    systems = []
+    for k in range(5):
+        L = 10 + int(rng.next_f64() * 60)
+        systems.append(S.resolve(S.build_chain(S.random_sequence(rng, L))))
+    eng = E.Engine(systems)
+    pos = [jittered(s, 10 + k) for k, s in enumerate(systems)]
+    f = eng.total_force(np.concatenate(pos), include_sasa=True, sasa_gamma_scale=0.25)
+    for s, p, fs_ in zip(systems, pos, eng.split(f)):
+        orc = O.Oracle(s)
+        fref = orc.total_force(p) + orc.sasa_forces(p, 0.25)[0]
+        assert rel_rms(fs_, fref) < FORCE_RMS_TOL
+
+
+def test_bit_reproducible(E, c2_system):
+    eng = E.Engine([c2_system])
+    a = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    b = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("shake", [False, True])
+def test_baoab_step_matches_oracle(E, c1_system, shake):
+    s = c1_system
+    orc = O.Oracle(s)
+    eng = E.Engine([s])
+    v0 = E.maxwell_boltzmann(s, 310.0, S.Xoshiro256pp(5))
+    eng.set_velocities(v0)
+    dt = 2.0 if shake else 1.0
+    seed = 0x1234567890ABCDEF
+    eng.dynamics_setup([310.0], dt_fs=dt, friction_ps_inv=2.0, seed=seed, constrain_h_bonds=shake)
+    f0 = eng.forces()
+    eng.run(1)
+    x1, v1, f1 = eng.positions(), eng.velocities(), eng.forces()
+    xi = O.philox_normals(seed, 0, s.n_atoms, 0)
+    fo = orc.total_force(s.pos)
+    assert rel_rms(f0, fo) < FORCE_RMS_TOL
+    # integrate the oracle from the same state with the engine's own start
+    # forces and noise, so the step itself is what is compared
+    px, pv, pf, fails = orc.baoab_step(s.pos, v0, f0, xi, dt, 310.0, 2.0, shake)
+    assert fails == 0
+    assert np.abs(x1 - px).max() < 1e-9
+    # velocities before the closing half-kick (which uses each side's own F(x1))
+    kick = (S.ACCEL_FACTOR / s.mass * 0.5 * dt)[:, None]
+    assert np.abs((v1 - f1 * kick) - (pv - pf * kick)).max() < 1e-9
+    assert rel_rms(f1, pf) < FORCE_RMS_TOL
+
+
+def test_thermostat_holds_temperature(E):
+    s = S.build_system(["Ala", "Ala", "Ala"])
+    eng, summ = E.run_langevin(s, 2000, dt_fs=1.0, temperature_k=310.0, friction_ps_inv=2.0,
+                               seed=7)
+    assert not summ.diverged
+    assert abs(summ.temperature_mean_k - 310.0) < 60.0
+    x = eng.positions()
+    assert np.isfinite(x).all() and np.abs(x).max() < 1000.0
+
+
+@pytest.fixture(scope="module")
+def c1_minimised(E, c1_system):
+    from origami_amd.minimize import minimize
+    import copy
+    eng = E.Engine([c1_system])
+    res = minimize(eng, max_steps=300, mode_a=False)[0]
+    s = copy.copy(c1_system)
+    s.pos = eng.positions()
+    return s, res
+
+
+def test_lbfgs_lowers_energy(c1_minimised):
+    s, res = c1_minimised
+    print(res)
+    assert res.final_energy < res.initial_energy - 1000.0
+    assert res.line_search_failures <= 2
+
+
+def test_shake_dt2_no_failures(E, c1_minimised):
+    c1_system, _ = c1_minimised
+    eng, summ = E.run_langevin(c1_system, 500, dt_fs=2.0, temperature_k=310.0,
+                               friction_ps_inv=2.0, seed=11, constrain_h_bonds=True)
+    assert not summ.diverged
+    assert summ.shake_failures == 0
+    assert abs(summ.temperature_mean_k - 310.0) < 80.0
+    # constraints hold
+    x = eng.positions()
+    d = np.linalg.norm(x[c1_system.constraints[:, 0]] - x[c1_system.constraints[:, 1]], axis=1)
+    assert np.abs(d ** 2 - c1_system.constraint_len ** 2).max() < 1e-6
+
+
+def test_remd_runs_and_swaps(E, c3_system):
+    import copy
+    from origami_amd.minimize import minimize
+    e0 = E.Engine([c3_system])
+    minimize(e0, max_steps=300, mode_a=False)
+    s = copy.copy(c3_system)
+    s.pos = e0.positions()
+    temps = E.geometric_ladder(280.0, 395.0, 8)
+    eng, summ = E.run_remd(s, temps, total_time_fs=4000.0, swap_interval_fs=200.0,
+                           dt_fs=2.0, friction_ps_inv=2.0, seed=3, constrain_h_bonds=True)
+    assert sum(summ.swap_attempts) > 0
+    assert max(summ.acceptance_ratios()) > 0.1
+    assert np.isfinite(eng.positions()).all()
+
+
+def test_trp_cage_minimise_acceptance(E):
+    # dynamics/tests/trp_cage_minimise.rs: extended Trp-cage, origami's
+    # minimiser settings (10 Å cutoffs) -> below 1000 kJ/mol within 500 steps
+    # with the bond term under 200 kJ/mol.
+    from origami_amd.minimize import minimize
+    s = S.build_system(list("NLYIQWLKDGGPSSGRPPPS"))
+    eng = E.Engine([s])
+    res = minimize(eng, max_steps=500, mode_a=True)[0]
+    e = eng.total_energy(mode_a=True)[0]
+    print(res, e)
+    assert res.final_energy < 1000.0
+    assert e["bond"] < 200.0

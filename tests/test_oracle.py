@@ -1,0 +1,318 @@
+"""Pins for the FP64 oracle: published known answers, closed forms,
+invariants and finite differences chosen so that a dropped term, a wrong
+sign or a swapped index fails at least one of them."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from origami_amd import system as S
+
+KJ = 4.184
+KC = 332.0637
+
+
+def make_sys(pos, q=None, eps=None, rmin=None, rho=None, scale=None, mass=None, sasa_r=None,
+             sasa_g=None, excl=(), p14=(), bonds=(), bond_p=(), cons=(), cons_len=()):
+    pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
+    n = len(pos)
+    z = np.zeros(n)
+
+    def arr(x, default):
+        return np.asarray(default if x is None else x, dtype=np.float64) * np.ones(n)
+
+    i2 = lambda x: np.asarray(x, dtype=np.int32).reshape(-1, 2)
+    return S.ResolvedSystem(
+        n_atoms=n, sequence=[], names=[""] * n, residue_index=np.zeros(n, np.int32),
+        element=np.zeros(n, np.int32), atom_type=np.zeros(n, np.int32), pos=pos,
+        charge=arr(q, 0.0), rmin_half=arr(rmin, 1.0), epsilon=arr(eps, 0.0),
+        rmin_half_14=arr(rmin, 1.0), epsilon_14=arr(eps, 0.0), gb_rho=arr(rho, 1.7),
+        gb_scale=arr(scale, 0.72), sasa_radius=arr(sasa_r, 3.1), sasa_gamma=arr(sasa_g, 0.0),
+        mass=arr(mass, 12.011), excl_pairs=i2(excl), pair14=i2(p14), bonds=i2(bonds),
+        bond_params=np.asarray(bond_p, dtype=np.float64).reshape(-1, 2),
+        angles=np.zeros((0, 3), np.int32), angle_params=np.zeros((0, 2)),
+        dihedrals=np.zeros((0, 4), np.int32), dihedral_params=np.zeros((0, 3)),
+        dihedral_first=np.zeros(0, np.int32), impropers=np.zeros((0, 4), np.int32),
+        improper_params=np.zeros((0, 2)), constraints=i2(cons),
+        constraint_len=np.asarray(cons_len, dtype=np.float64))
+
+
+# ---------------------------------------------------------------- RNGs --
+def test_philox_known_answers():
+    # Random123 kat_vectors for philox4x32-10
+    assert O.philox4x32((0, 0), (0, 0, 0, 0)) == [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]
+    assert O.philox4x32((0xFFFFFFFF,) * 2, (0xFFFFFFFF,) * 4) == [0x408F276D, 0x41C83B0E,
+                                                                  0xA20BC7C6, 0x6D5451FD]
+    assert O.philox4x32((0xA4093822, 0x299F31D0), (0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344)) \
+        == [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]
+
+
+def test_philox_normals_moments():
+    g = O.philox_normals(12345, 3, 200000, 7).reshape(-1)
+    assert abs(g.mean()) < 0.01 and abs(g.var() - 1.0) < 0.01
+    assert not np.array_equal(O.philox_normals(1, 0, 10, 0), O.philox_normals(1, 0, 10, 1))
+    assert not np.array_equal(O.philox_normals(1, 0, 10, 0), O.philox_normals(1, 1, 10, 0))
+
+
+def test_xoshiro_matches_host_stream():
+    r = S.Xoshiro256pp(987654321)
+    assert O.xoshiro_u64(987654321, 64) == [r.next_u64() for _ in range(64)]
+
+
+# ------------------------------------------------------ LJ and Coulomb --
+def test_coulomb_closed_form():
+    # nonbonded.rs:472-489: +1/-1 at 5 Å -> -277.871 kJ/mol (bare Coulomb)
+    s = make_sys([[0, 0, 0], [5, 0, 0]], q=[1.0, -1.0])
+    e = O.Oracle(s).nonbonded_energy(s.pos, math.inf)
+    assert abs(e["coulomb"] - (-277.871)) < 0.01
+    assert abs(e["coulomb"] - (-KC * KJ / 5.0)) < 1e-9
+
+
+def test_reaction_field_vanishes_at_cutoff():
+    # nonbonded.rs:21-26: V_RF(Rc) = F_RF(Rc) = 0
+    for r in (10.0 - 1e-9, 9.5):
+        s = make_sys([[0, 0, 0], [r, 0, 0]], q=[0.5, 0.7])
+        orc = O.Oracle(s)
+        e = orc.nonbonded_energy(s.pos, 10.0)["coulomb"]
+        f = orc.nonbonded_forces(s.pos, 10.0)
+        exact = KC * 0.35 * (1 / r + r * r / 2000.0 - 0.15) * KJ
+        assert abs(e - exact) < 1e-9
+        if r > 9.99:
+            assert abs(e) < 1e-7 and np.abs(f).max() < 1e-7
+    s = make_sys([[0, 0, 0], [10.5, 0, 0]], q=[0.5, 0.7])
+    assert np.abs(O.Oracle(s).nonbonded_forces(s.pos, 10.0)).max() == 0.0
+
+
+def test_lj_minimum_and_exclusions():
+    # V = eps((R/r)^12 - 2(R/r)^6): minimum -eps at r = R, zero force there
+    s = make_sys([[0, 0, 0], [3.0, 0, 0]], eps=[0.2, 0.05], rmin=[1.4, 1.6])
+    orc = O.Oracle(s)
+    assert abs(orc.nonbonded_energy(s.pos, math.inf)["lj"] + math.sqrt(0.01) * KJ) < 1e-12
+    assert np.abs(orc.nonbonded_forces(s.pos, math.inf)).max() < 1e-12
+    s2 = make_sys([[0, 0, 0], [2.0, 0, 0]], q=[1, 1], eps=[0.2, 0.05], excl=[(0, 1)])
+    o2 = O.Oracle(s2)
+    assert o2.nonbonded_energy(s2.pos, math.inf)["lj"] == 0.0
+    assert np.abs(o2.nonbonded_forces(s2.pos, math.inf)).max() == 0.0
+
+
+def test_nonbonded_finite_difference_both_modes(c1_system):
+    s = c1_system
+    orc = O.Oracle(s)
+    for cut in (10.0, math.inf):
+        f = orc.nonbonded_forces(s.pos, cut)
+        h = 1e-5
+        for i in (0, 57, 113, 250):
+            for ax in range(3):
+                p = s.pos.copy()
+                p[i, ax] += h
+                ep = orc.nonbonded_energy(p, cut)
+                p[i, ax] -= 2 * h
+                em = orc.nonbonded_energy(p, cut)
+                num = -((ep["lj"] + ep["coulomb"]) - (em["lj"] + em["coulomb"])) / (2 * h)
+                assert abs(num - f[i, ax]) < 1e-3 * max(1.0, abs(f[i, ax])), (cut, i, ax)
+
+
+# --------------------------------------------------------------- GB --
+def test_descreening_closed_form_and_sign():
+    # gb.rs:399-404: j inside i -> 0
+    assert O.pairwise_descreening(0.5, 2.0, 0.5) == 0.0
+    # non-overlap branch: h = (1/r)[x/(1-x^2) - x^3/(1-x^2)^2 - atanh x], x = S/r
+    for r, rho, S_ in ((3.0, 1.61, 1.2), (7.5, 1.11, 1.0), (20.0, 1.46, 1.31)):
+        x = S_ / r
+        exact = (x / (1 - x * x) - x ** 3 / (1 - x * x) ** 2 - math.atanh(x)) / r
+        assert abs(O.pairwise_descreening(r, rho, S_) - exact) < 1e-12 * max(1.0, abs(exact) * 1e3)
+    # h <= 0 everywhere on a dense scan (the property behind R_i == rho~_i)
+    for rho in (1.11, 1.46, 1.61, 1.71):
+        for S_ in (0.94, 1.04, 1.17, 1.31, 1.64):
+            for r in np.linspace(0.01, 30.0, 600):
+                assert O.pairwise_descreening(float(r), rho, S_) <= 0.0
+
+
+def test_born_radii_equal_rho_tilde(c1_system, c2_system):
+    for s in (c1_system, c2_system):
+        for cut in (20.0, math.inf):
+            integ, radii, clamped = O.Oracle(s).born_radii(s.pos, cut)
+            assert clamped == 0
+            assert (integ < 0).all()
+            assert np.array_equal(radii, s.gb_rho - 0.09)
+
+
+def test_gb_self_and_pair_closed_forms():
+    pref = -0.5 * (1 - 1 / 78.5) * KC * KJ
+    s = make_sys([[0, 0, 0]], q=[1.0], rho=[1.55])
+    r = np.array([1.46])
+    e = O.Oracle(s).gb_energy(s.pos, r)
+    assert abs(e["self"] - pref / 1.46) < 1e-9 and e["pair"] == 0.0
+    d = 4.2
+    s2 = make_sys([[0, 0, 0], [d, 0, 0]], q=[0.4, -0.6])
+    R = np.array([1.61, 1.11])
+    e2 = O.Oracle(s2).gb_energy(s2.pos, R)
+    f = math.sqrt(d * d + R.prod() * math.exp(-d * d / (4 * R.prod())))
+    assert abs(e2["pair"] - 2 * pref * (-0.24) / f) < 1e-9
+    # frozen-radii force = -dE/dr of the pair term
+    fo = O.Oracle(s2).gb_forces(s2.pos, R, math.inf)
+    h = 1e-6
+    fp = math.sqrt((d + h) ** 2 + R.prod() * math.exp(-(d + h) ** 2 / (4 * R.prod())))
+    fm = math.sqrt((d - h) ** 2 + R.prod() * math.exp(-(d - h) ** 2 / (4 * R.prod())))
+    dE = 2 * pref * (-0.24) * (1 / fp - 1 / fm) / (2 * h)
+    assert abs(fo[1, 0] - (-dE)) < 1e-6 and abs(fo[0, 0] + fo[1, 0]) < 1e-12
+
+
+def test_total_force_finite_difference_mode_b(c1_system):
+    # forces.rs:200-238 (there with the GB pair energy at matching cutoff)
+    s = c1_system
+    orc = O.Oracle(s)
+    f = orc.total_force(s.pos)
+    h = 1e-5
+    for i in range(0, s.n_atoms, 23):
+        for ax in range(3):
+            p = s.pos.copy()
+            p[i, ax] += h
+            ep = orc.total_energy(p)["total"]
+            p[i, ax] -= 2 * h
+            em = orc.total_energy(p)["total"]
+            assert abs(-(ep - em) / (2 * h) - f[i, ax]) < 1e-3 * max(1.0, abs(f[i, ax]))
+
+
+def test_total_force_invariants(c2_system):
+    s = c2_system
+    orc = O.Oracle(s)
+    f = orc.total_force(s.pos)
+    assert np.abs(f.sum(0)).max() < 1e-8 * np.abs(f).max()
+    f2 = orc.total_force(s.pos + np.array([3.25, -1.5, 7.0]))
+    assert np.abs(f2 - f).max() < 1e-8 * np.abs(f).max()
+
+
+def test_bonded_forces_finite_difference_and_zero_at_r0():
+    s = S.build_system(["Lys", "Trp", "Asp"])
+    orc = O.Oracle(s)
+    f = orc.bonded_forces(s.pos)
+    assert np.abs(f.sum(0)).max() < 1e-8
+    h = 1e-6
+    E = lambda p: sum(orc.bonded_energy(p).values())
+    for i in range(0, s.n_atoms, 5):
+        for ax in range(3):
+            p = s.pos.copy()
+            p[i, ax] += h
+            ep = E(p)
+            p[i, ax] -= 2 * h
+            assert abs(-(ep - E(p)) / (2 * h) - f[i, ax]) < 1e-3 * max(1.0, abs(f[i, ax]))
+    b = make_sys([[0, 0, 0], [1.5, 0, 0]], bonds=[(0, 1)], bond_p=[(300.0, 1.5)])
+    assert np.abs(O.Oracle(b).bonded_forces(b.pos)).max() == 0.0
+
+
+# ------------------------------------------------------------- SASA --
+def test_sasa_single_and_disjoint_caps():
+    # area.rs:418-447
+    a = O.psa_area_from_caps([[0, 0, 1]], [0.3], 1.0)
+    assert abs(a - 2 * math.pi * 1.3) < 1e-9
+    r = 1.5
+    a = O.psa_area_from_caps([[0, 0, 1], [0, 0, -1]], [0.7, 0.6], r)
+    assert abs(a - (4 * math.pi * r * r - 2 * math.pi * r * r * 0.3 - 2 * math.pi * r * r * 0.4)) < 1e-9
+    a = O.psa_area_from_caps([[0, 0, 1], [1, 0, 0]], [0.3, 0.3], 1.0)
+    assert 0 < a < 2 * math.pi * 1.3
+
+
+def test_sasa_isolated_atom_and_lens():
+    # mod.rs:120-138: isolated carbon 4 pi (1.7 + 1.4)^2
+    tot, per = O.psa_area([[0, 0, 0]], [3.1])
+    assert abs(tot - 4 * math.pi * 3.1 ** 2) < 1e-6
+    # two spheres: exposed area of each = 2 pi R^2 (1 + cos a) by the lens formula
+    d, R = 3.0, 3.1
+    tot, per = O.psa_area([[0, 0, 0], [d, 0, 0]], [R, R])
+    ca = d / (2 * R)
+    assert abs(per[0] - 2 * math.pi * R * R * (1 + ca)) < 1e-9
+
+
+def test_sasa_two_atom_gradient():
+    # analytical.rs:999-1046: radii 3, 3 at d = 3, vs central differences
+    pos = np.array([[0, 0, 0], [3.0, 0, 0]], dtype=float)
+    rad = np.array([3.0, 3.0])
+    g, na = O.psa_atom_gradient(pos, rad, 0, [1])
+    assert na == 1
+    # exact: A_0 = 2 pi R^2 (1 + d / 2R)  ->  dA_0/dx_1 = pi R,  dA_0/dx_0 = -pi R
+    assert abs(g[1, 0] - math.pi * 3.0) < 1e-9 and abs(g[0, 0] + math.pi * 3.0) < 1e-9
+    assert np.abs(g[:, 1:]).max() < 1e-12
+
+
+def _sasa_fd_checks(seq, jitter, seed=0):
+    s = S.build_system(seq)
+    pos = s.pos + np.random.default_rng(seed).normal(scale=jitter, size=s.pos.shape)
+    R = s.sasa_radius
+    tot, per = O.psa_area(pos, R)
+    worst, n_checked, clamped_with_force = 0.0, 0, 0
+    for i in range(s.n_atoms):
+        nb = [j for j in range(s.n_atoms)
+              if j != i and np.linalg.norm(pos[j] - pos[i]) <= R[i] + R[j]]
+        g, na = O.psa_atom_gradient(pos, R, i, nb)
+        if na == 0:
+            continue
+        if not (0.0 < per[i] < 4 * math.pi * R[i] ** 2):
+            clamped_with_force += int(np.abs(g).max() > 0)
+            continue
+        for x in [i] + nb:
+            for ax in range(3):
+                h = 1e-6
+                p = pos.copy()
+                p[x, ax] += h
+                ap = O.psa_area(p, R)[1][i]
+                p[x, ax] -= 2 * h
+                am = O.psa_area(p, R)[1][i]
+                worst = max(worst, abs((ap - am) / (2 * h) - g[x, ax]))
+                n_checked += 1
+    return worst, n_checked, clamped_with_force
+
+
+def test_sasa_gradient_is_derivative_of_area():
+    # analytical.rs:888-993 checks 12 atoms of Ala3 at 1e-2; on generic
+    # geometry the restated gradient equals the central difference of the
+    # restated area (itself pinned by the closed forms above) for every atom
+    # whose area lies strictly inside (0, 4 pi R^2).
+    worst, n, _ = _sasa_fd_checks(["Ala", "Leu", "Ala"], 0.05)
+    assert n > 2000 and worst < 1e-6
+
+
+def test_sasa_gradient_ignores_area_clamp():
+    # origami's area clamps A_i to [0, 4 pi R^2] (area.rs:123-125) but the
+    # gradient differentiates the unclamped Gauss-Bonnet sum (analytical.rs:
+    # 315-421): an atom whose sum goes negative reports A_i = 0 yet still
+    # receives SASA forces.  Restated as is.
+    _, _, clamped = _sasa_fd_checks(["Ala", "Leu", "Ala"], 0.05)
+    assert clamped >= 1
+
+
+def test_sasa_forces_newton_third_law():
+    s = S.build_system(["Lys", "Phe", "Val", "Trp"])
+    pos = s.pos + np.random.default_rng(3).normal(scale=0.05, size=s.pos.shape)
+    f, st = O.Oracle(s).sasa_forces(pos, 1.0)
+    assert st["atoms_with_arcs"] > 0
+    assert np.abs(f.sum(0)).max() < 1e-9 * max(1.0, np.abs(f).max())
+
+
+# ------------------------------------------------------- integrator --
+def test_shake_restores_constraints(c1_system):
+    s = c1_system
+    orc = O.Oracle(s)
+    ref = s.pos.copy()
+    # bring the reference onto the constraint manifold first
+    ref, it = orc.shake(ref, s.pos)
+    assert it > 0
+    moved = ref + np.random.default_rng(1).normal(scale=0.01, size=ref.shape)
+    out, it = orc.shake(moved, ref)
+    assert 0 < it <= 64
+    d2 = ((out[s.constraints[:, 0]] - out[s.constraints[:, 1]]) ** 2).sum(1)
+    assert np.abs(d2 - s.constraint_len ** 2).max() < 1e-6
+
+
+def test_baoab_without_friction_conserves_energy():
+    s = S.build_system(["Ala", "Ala", "Ala"])
+    orc = O.Oracle(s)
+    pos, v = s.pos.copy(), np.zeros_like(s.pos)
+    f = orc.total_force(pos)
+    xi = np.zeros_like(pos)
+    e0 = orc.total_energy(pos)["total"] + orc.kinetic_energy(v)
+    for _ in range(40):
+        pos, v, f, _ = orc.baoab_step(pos, v, f, xi, 0.25, 300.0, 0.0, False)
+    e1 = orc.total_energy(pos)["total"] + orc.kinetic_energy(v)
+    assert abs(e1 - e0) < 1e-3 * abs(e0)
