@@ -239,7 +239,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<double> pos(3 * (size_t)N), mass(N), rho(N), charge(N), sr(N), sg(N);
     std::vector<double> epsv(N), rminv(N), eps14v(N), rmin14v(N);
     std::vector<int> a_sys(N), a_loc(N), a_slot(N);
-    std::vector<int> tile_atom0, tile_cnt;
+    std::vector<int> tile_atom0, tile_cnt, sys_tile_off{0};
     std::vector<og_work> work;
     std::vector<unsigned long long> masks;
     std::vector<float4> par, gbv;
@@ -262,6 +262,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
             tile_atom0.push_back(off + t * OG_TILE);
             tile_cnt.push_back(std::min(OG_TILE, n - t * OG_TILE));
         }
+        sys_tile_off.push_back(t0 + nt);
         for (int k = 0; k < n; ++k) {
             const int g = off + k;
             for (int c = 0; c < 3; ++c) pos[3 * (size_t)g + c] = d.pos[3 * k + c];
@@ -428,7 +429,8 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->shake_fail, zi));
     HIPCHK(upload(&e->diverged, zi));
     HIPCHK(upload(&e->clamped, zi));
-    int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg);
+    int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg, sys_tile_off,
+                           e->tile_atom0, e->tile_cnt, e->n_tiles);
     if (rc != OG_OK) return fail(rc, "SASA set-up failed: %s", og_sasa_error());
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipDeviceSynchronize());
@@ -525,10 +527,19 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
     }
     if (flags & OG_SASA) {
         ProfScope p(e, OG_K_SASA);
-        int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, e->eacc, sasa_scale, energy, e->stream);
+        int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
     }
     HIPCHK(hipGetLastError());
+    return OG_OK;
+}
+
+static int check_sasa_overflow(og_engine* e, uint32_t flags) {
+    if (!(flags & OG_SASA)) return OG_OK;
+    int ovf = 0;
+    if (og_sasa_overflow(&e->sasa, &ovf) != 0) return fail(OG_ERR_HIP, "reading SASA status failed");
+    if (ovf) return fail(OG_ERR_ARG, "%d SASA atom evaluations exceeded the per-atom capacity "
+                         "(128 caps / 256 arcs / 256 vertices) and were skipped", ovf);
     return OG_OK;
 }
 
@@ -565,6 +576,7 @@ extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) prof_collect(e);
+    if ((rc = check_sasa_overflow(e, flags))) return rc;
     if (energies) {
         if ((rc = read_energies(e, energies))) return rc;
     }
@@ -686,6 +698,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) prof_collect(e);
+    if (int rc = check_sasa_overflow(e, p.flags)) return rc;
     std::vector<int> div(e->n_sys);
     HIPCHK(hipMemcpy(div.data(), e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
     for (int s = 0; s < e->n_sys; ++s)

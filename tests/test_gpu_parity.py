@@ -209,7 +209,8 @@ def test_lbfgs_lowers_energy(c1_minimised):
 
 def test_shake_dt2_no_failures(E, c1_minimised):
     c1_system, _ = c1_minimised
-    eng, summ = E.run_langevin(c1_system, 500, dt_fs=2.0, temperature_k=310.0,
+    # dynamics/tests/shake_dt2.rs: 1000 steps at dt = 2 fs, T within 80 K
+    eng, summ = E.run_langevin(c1_system, 1000, dt_fs=2.0, temperature_k=310.0,
                                friction_ps_inv=2.0, seed=11, constrain_h_bonds=True)
     assert not summ.diverged
     assert summ.shake_failures == 0
