@@ -1,13 +1,19 @@
-// Analytical SASA force kernel (PowerSasa) — host interface.
+// Analytical SASA force kernels (PowerSasa) — host interface.
 #pragma once
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
+// Pool capacity: arcs (and, separately, vertices) per SASA atom on average
+// over a batch.  Protein atoms carry ~5-20; exceeding it is reported as an
+// error, never as silently missing forces.
+#define OG_SASA_POOL_ARCS 64
+
 struct og_sasa_data {
     int n_work = 0;               // atoms with gamma != 0
-    int blocks = 0;               // launch width of the wave-per-atom kernel
-    int stride = 0;               // waves launched (= per-wave scratch slots)
+    int n_cu = 0;
+    int blocks = 0;               // launch width of the arrangement kernel
+    int stride = 0;               // waves launched (= per-wave staging slots)
     int n_tiles = 0;
     double max_radius = 0.0;      // largest SASA radius (neighbour culling)
     int* work = nullptr;          // [n_work] global atom index
@@ -18,9 +24,22 @@ struct og_sasa_data {
     double4* tile_bound = nullptr;    // [n_tiles] bounding sphere, refreshed per launch
     double* radius = nullptr;     // [N] vdW + probe
     double* gamma = nullptr;      // [N] kJ/mol/Å^2 (unscaled)
-    double* scratch = nullptr;    // per-wave FP64 scratch slots
-    int* iscratch = nullptr;      // per-wave int scratch slots
+    double* scratch = nullptr;    // per-wave FP64 staging slots
+    int* iscratch = nullptr;      // per-wave int staging slots
     int* overflow = nullptr;      // [1] count of atoms whose arrangement exceeded capacity
+    int* counters = nullptr;      // [4] pool fill: arcs, vertices, cap owners
+    int4* rec = nullptr;          // [n_work] (arc_off, vtx_off, owner_off, na | nv<<9 | nc<<18)
+    // pool
+    int arc_cap = 0, vtx_cap = 0, own_cap = 0;
+    int4* arc_atoms = nullptr;    // (i, k, l, m) atoms; l = m = -1 for arcs without vertices
+    double4* arc_par = nullptr;   // (theta, start sign, end sign, -)
+    unsigned* arc_code = nullptr; // local roles k, l, m as bytes (1 + cap; 0xFF absent)
+    double* arc_terms = nullptr;  // [12 per arc] role x axis
+    int4* vtx_atoms = nullptr;    // (i, in, out, -)
+    double* vtx_sign = nullptr;
+    unsigned* vtx_code = nullptr;
+    double* vtx_terms = nullptr;  // [9 per vertex]
+    int* owners = nullptr;        // cap owners per atom
 };
 
 // sys_tile_off[s] .. sys_tile_off[s+1] are system s's tiles (64-atom, never
@@ -32,5 +51,6 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
 void og_sasa_free(og_sasa_data* d);
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
                    hipStream_t s);
+// Reads and clears the overflow count.
 int og_sasa_overflow(og_sasa_data* d, int* count);
 const char* og_sasa_error();

@@ -102,6 +102,19 @@ OG_HD int intersect_circles(const Cap& c1, const Cap& c2, v3& p, v3& q) {
     return 0;
 }
 
+// Number of points intersect_circles would return (same arithmetic, same
+// decisions): 2, 1 (tangent) or 0.
+OG_HD int intersect_count(const Cap& c1, const Cap& c2) {
+    const double sigma = dot(c1.axis, c2.axis);
+    const double denom = 1.0 - sigma * sigma;
+    if (fabs(denom) < 1e-12) return 0;
+    const double a = (c1.ca - sigma * c2.ca) / denom;
+    const double b = (c2.ca - sigma * c1.ca) / denom;
+    const double c_sq = 1.0 - a * a - b * b - 2.0 * a * b * sigma;
+    if (c_sq < -1e-9) return 0;
+    return fabs(c_sq) <= 1e-9 ? 1 : 2;
+}
+
 OG_HD double angular_position(v3 pt, const Cap& c) {
     const v3 h = helper_axis(c.axis);
     const v3 u = normalize(h - c.axis * dot(h, c.axis));

@@ -1,10 +1,11 @@
 // Analytical hydrophobic-SASA forces (PowerSasa, Klenin et al. 2011 spherical
-// Gauss-Bonnet), FP64, one wave64 per atom with nonzero surface tension.
+// Gauss-Bonnet), FP64.
 //
-// Per atom i: overlapping neighbours -> buried caps on i's unit sphere ->
-// accessible-region boundary (arcs on cap circles joined at vertices) ->
-// forward-mode derivatives of A_i / R_i^2 = 2*pi*chi - sum_arcs cos(a) theta
-// - sum_vertices eps, scattered as F_x -= gamma_i * dA_i/dr_x.
+// Per atom i with nonzero surface tension: overlapping neighbours -> buried
+// caps on i's unit sphere -> accessible-region boundary (arcs on cap circles
+// joined at vertices) -> forward-mode derivatives of
+// A_i / R_i^2 = 2*pi*chi - sum_arcs cos(a) theta - sum_vertices eps,
+// scattered as F_x -= gamma_i * dA_i/dr_x.
 //
 // Restates origami:
 //   caps / boundary      energy/src/powersasa/arrangement.rs:69-289
@@ -14,32 +15,48 @@
 // Topology decisions use origami's FP64 tolerances unchanged (1e-12, 1e-9,
 // 1e-10) and no FMA contraction (sasa_core.h), which is why this stays FP64.
 //
-// Parallel structure (wave = one atom, lanes cooperate):
-//   caps      lanes over the 64 atoms of each candidate tile (tiles culled by
-//             bounding spheres), ballot-compacted in ascending atom order;
-//   boundary  caps k in order; for each, lanes over the other caps m compute
-//             the transitions (push order m, then p before q), a bitonic sort
-//             on (angle, push index) reproduces the stable sort, duplicates
-//             are dropped against the last kept point, lanes over segments do
-//             the midpoint test, and ballot compaction emits arcs/vertices in
-//             origami's (k, segment) order.  A cap whose whole circle lies
-//             inside another cap by a margin of 1e-9 (>> the 1e-12 test
-//             tolerance) emits nothing in origami and is skipped up front;
-//   gradient  lanes over (arc|vertex, role, axis) items evaluate one JVP term
-//             each; lanes over (atom, axis) then sum their terms in arc order
-//             and vertex order, which is the order origami sums them in
-//             (terms of arcs/vertices not involving an atom are exact zeros).
-// Per-wave LDS holds the caps and the current cap's transitions; arcs,
-// vertices and gradient terms live in a per-wave global scratch slot.
+// Three kernels per force evaluation:
+//
+//  og_k_sasa_arrange  one wave64 per atom.  Caps from ballot-compacted
+//      neighbour candidates (ascending atom order; far tiles culled by their
+//      bounding spheres).  The boundary is built for many caps at once: the
+//      caps are cut into chunks whose transitions fit in LDS; within a chunk
+//      lanes run over (cap k, cap m) pairs in origami's push order (k, then m
+//      ascending, p before q), a rank sort on (angle, push index) inside each
+//      cap's segment reproduces the stable angular sort, each cap's
+//      transitions are deduplicated against the last kept point, and lanes
+//      run over "slots" (boundary segments, or the probe point of a cap with
+//      no transitions) for the midpoint / probe burial tests.  Emission is
+//      ballot-compacted in slot order, which is origami's (k, segment) order.
+//      Vertex root signs and arc->vertex links follow, and the atom's arcs,
+//      vertices and cap owners are appended to a pool.
+//  og_k_sasa_terms    thread per (arc|vertex, role, axis): one JVP term each.
+//  og_k_sasa_scatter  one wave per atom, lane per affected atom x: the terms
+//      of every arc (in arc order) and then every vertex (in vertex order)
+//      that involve x are summed, which is the order origami sums them in
+//      (terms of arcs/vertices not involving x are exact zeros there), and
+//      added to the int64 fixed-point force accumulator.
+//
+// The pool placement depends on which wave allocates first, but every
+// value depends only on the atom's own arrangement, so results are
+// bit-reproducible.
 #include <string>
 
 #include "og_device.h"
 #include "og_sasa.h"
 #include "sasa_core.h"
 
-#define SW_WAVES 2                 // waves per block
-#define SW_MAXA MAXA
-#define SW_MAXV MAXV
+#define SA_WAVES 2                    // waves per block (arrangement, scatter)
+#define SA_TMAX 256                   // transitions per chunk (>= 2 * (MAXC - 1))
+
+// per-wave global staging (arcs and vertices of the atom in progress)
+#define WD_ARC 0                      // MAXA x (sx, sy, sz, ex, ey, ez, theta)
+#define WD_VTX (WD_ARC + 7 * MAXA)    // MAXV x (px, py, pz)
+#define WD_VSG (WD_VTX + 3 * MAXV)    // MAXV root signs
+#define WD_TOTAL (WD_VSG + MAXV)
+#define WI_ARC 0                      // MAXA x (cap, full)
+#define WI_VTX (WI_ARC + 2 * MAXA)    // MAXV x (cap in, cap out)
+#define WI_TOTAL (WI_VTX + 2 * MAXV)
 
 static std::string g_sasa_err;
 const char* og_sasa_error() { return g_sasa_err.c_str(); }
@@ -48,27 +65,20 @@ using namespace og_sasa;
 
 namespace {
 
-struct WaveLds {
+struct ArrLds {
     double cx[MAXC], cy[MAXC], cz[MAXC], cc[MAXC];   // caps: axis, cos(alpha)
     int own[MAXC];                                   // cap owner (global atom)
-    double ta[MAXT], tx[MAXT], ty[MAXT], tz[MAXT];   // transitions of cap k (push order)
-    int to[MAXT];                                    // other cap of each transition
-    unsigned char perm[256];                         // sorted order
-    unsigned char kept[MAXT];                        // kept (deduplicated) transitions, sorted
-    unsigned int xarc[SW_MAXA];                      // arc roles k, l, m as local atom bytes
-    unsigned int xvtx[SW_MAXV];                      // vertex roles in, out as local atom bytes
-    int scalar;
+    unsigned char hint[MAXC];                        // a cap that likely buries circle k
+    unsigned char skip[MAXC];                        // circle k provably contributes nothing
+    unsigned char act[MAXC];                         // caps still to process, ascending
+    unsigned short cnt[MAXC];                        // transitions of cap k
+    unsigned short seg[MAXC + 1];                    // chunk: segment start per chunk cap
+    double ta[SA_TMAX];                              // chunk: transition angles (push order)
+    double tx[SA_TMAX], ty[SA_TMAX], tz[SA_TMAX];    // chunk: transition points
+    unsigned char tk[SA_TMAX], tm[SA_TMAX], tq[SA_TMAX];  // chunk cap, other cap, p/q/probe
+    unsigned char perm[SA_TMAX];                     // sorted order within segments
+    unsigned char rm[SA_TMAX];                       // sorted position removed by dedup
 };
-
-// per-wave global scratch slot layout (doubles then ints)
-#define SD_ARC 0                                     // MAXA x 7 (sx,sy,sz,ex,ey,ez,theta)
-#define SD_VTX (SD_ARC + 7 * SW_MAXA)                // MAXV x 4 (px,py,pz,sign)
-#define SD_TA (SD_VTX + 4 * SW_MAXV)                 // MAXA x 12 arc terms (role*3+axis)
-#define SD_TV (SD_TA + 12 * SW_MAXA)                 // MAXV x 9 vertex terms
-#define SD_TOTAL (SD_TV + 9 * SW_MAXV)
-#define SI_ARC 0                                     // MAXA x 4 (cap, full, sv, ev)
-#define SI_VTX (SI_ARC + 4 * SW_MAXA)                // MAXV x 2 (in, out)
-#define SI_TOTAL (SI_VTX + 2 * SW_MAXV)
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -81,450 +91,554 @@ __device__ __forceinline__ unsigned long long lanemask_lt(int lane) {
 
 __device__ __forceinline__ int popc(unsigned long long m) { return __popcll(m); }
 
-__device__ __forceinline__ Cap cap_at(const WaveLds& L, int k) {
+__device__ __forceinline__ double rld(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)b, l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ Cap cap_at(const ArrLds& L, int k) {
     return {mk(L.cx[k], L.cy[k], L.cz[k]), L.cc[k]};
 }
 
-// Sort key comparison (angle, push index): strict weak order that matches a
-// stable sort by angle.
-__device__ __forceinline__ bool key_less(const WaveLds& L, int a, int b, int M) {
-    if (b >= M) return a < M;          // padding sorts last
-    if (a >= M) return false;
-    const double x = L.ta[a], y = L.ta[b];
-    return x < y || (x == y && a < b);
+// Lower bound of dot(x, axis_m) over the rotations x of point p about
+// axis_k (ct, st: cosine and sine between the axes) clears cap m's
+// containment test with margin; points near the axis are not trusted.
+__device__ __forceinline__ bool bound_inside(v3 p, const Cap& ck, double ct, double st, double ca_m) {
+    const double h = dot(p, ck.axis);
+    const double rn = norm(p - ck.axis * h);
+    return rn >= 1e-3 && h * ct - rn * st > ca_m + 1e-11;
+}
+
+// Arrangement of one work atom; returns its pool record (empty when the atom
+// contributes nothing or exceeded capacity).
+__device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos, int w, int lane,
+                             unsigned long long lt, double* wd, int* wi) {
+    const int4 empty = make_int4(0, 0, 0, 0);
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    const int ia = d.work[w];
+    const v3 pi = ldp(pos, ia);
+    const double ri = d.radius[ia];
+
+    // ---------------- caps (build_atom_cache + build_caps) ----------------
+    // Candidates in ascending atom order; an enclosing neighbour ends the
+    // scan (no accessible surface) unless the cap count overflowed first.
+    int nc = 0;
+    bool enclosed = false, overflow = false;
+    for (int t = d.tile_begin[w]; t < d.tile_end[w]; ++t) {
+        const double4 tb = d.tile_bound[t];
+        const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
+        const double reach = ri + d.max_radius + tb.w;
+        if (dxc * dxc + dyc * dyc + dzc * dzc > reach * reach) continue;
+        const int j = d.tile_atom0[t] + lane;
+        const bool valid = lane < d.tile_cnt[t] && j != ia;
+        bool is_cap = false, enc = false;
+        v3 axis = mk(0, 0, 0);
+        double ca = 0.0;
+        if (valid) {
+            const v3 pj = ldp(pos, j);
+            const double rj = d.radius[j];
+            const double dd = norm(pj - pi);
+            if (dd <= ri + rj) {
+                if (dd + ri <= rj) {
+                    enc = true;
+                } else if (!(dd + rj <= ri || dd >= ri + rj)) {
+                    ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
+                    if (ca >= -1.0 && ca <= 1.0) {
+                        is_cap = true;
+                        axis = (pj - pi) / dd;
+                    }
+                }
+            }
+        }
+        unsigned long long m = __ballot(is_cap);
+        const unsigned long long em = __ballot(enc);
+        if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
+        if (nc + popc(m) > MAXC) { overflow = true; break; }
+        if ((m >> lane) & 1ull) {
+            const int p = nc + popc(m & lt);
+            L.cx[p] = axis.x; L.cy[p] = axis.y; L.cz[p] = axis.z; L.cc[p] = ca;
+            L.own[p] = j;
+        }
+        nc += popc(m);
+        if (em) { enclosed = true; break; }
+    }
+    wave_sync();
+    if (overflow) {
+        if (lane == 0) atomicAdd(d.overflow, 1);
+        return empty;
+    }
+    if (enclosed || nc == 0) return empty;
+
+    // ---------------- per-cap transition counts and burial hints ----------
+    // hint[k]: the first cap m that (up to rounding) contains circle k
+    // entirely.  It is tested first in the burial tests below (that changes
+    // only the order in which an any() is evaluated), and it drives the exact
+    // skip test that follows.
+    for (int k = lane; k < nc; k += 64) {
+        const Cap ck = cap_at(L, k);
+        const double sk = sqrt(fmax(1.0 - ck.ca * ck.ca, 0.0));
+        int cnt = 0, hint = 0xFF;
+        for (int m = 0; m < nc; ++m) {
+            if (m == k) continue;
+            const Cap cm = cap_at(L, m);
+            cnt += intersect_count(ck, cm);
+            if (hint == 0xFF) {
+                const double ct = dot(ck.axis, cm.axis);
+                const double st = norm(cross(ck.axis, cm.axis));
+                if (ct * ck.ca - st * sk > cm.ca) hint = m;
+            }
+        }
+        L.cnt[k] = (unsigned short)cnt;
+        L.hint[k] = (unsigned char)hint;
+        L.skip[k] = hint != 0xFF ? 1 : 0;
+    }
+    wave_sync();
+
+    // ---------------- exact skip of caps buried by their hint cap ---------
+    // Every point origami tests on circle k is a segment midpoint (a rotation
+    // of a transition point about axis_k, which keeps its height h and
+    // radius rn) or, without transitions, the probe point.  Such a point x
+    // satisfies dot(x, axis_m) >= h*ct - rn*st (ct, st: cosine and sine
+    // between the axes).  When that bound clears cap m's containment test by
+    // 1e-11 (>> rounding) for every transition point, or for the probe,
+    // circle k contributes no arc and is dropped.
+    const int nper = nc > 1 ? nc - 1 : 1;
+    for (int base = 0; base < nc * nper; base += 64) {
+        const int p = base + lane;
+        if (p < nc * nper) {
+            const int k = p / nper, mi = p % nper;
+            const int h = L.hint[k];
+            if (h != 0xFF && L.skip[k]) {
+                const Cap ck = cap_at(L, k), chint = cap_at(L, h);
+                const double ct = dot(ck.axis, chint.axis);
+                const double st = norm(cross(ck.axis, chint.axis));
+                bool ok = true;
+                if (L.cnt[k] == 0) {
+                    if (mi == 0) ok = bound_inside(sample_circle_point(ck), ck, ct, st, chint.ca);
+                } else if (nc > 1) {
+                    const int m = mi < k ? mi : mi + 1;
+                    v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
+                    const int r = intersect_circles(ck, cap_at(L, m), P, Q);
+                    if (r <= 1) ok = bound_inside(P, ck, ct, st, chint.ca);
+                    if (r == 0) ok = ok && bound_inside(Q, ck, ct, st, chint.ca);
+                }
+                if (!ok) L.skip[k] = 0;
+            }
+        }
+    }
+    wave_sync();
+    // active caps, ascending
+    int n_act = 0;
+    for (int base = 0; base < nc; base += 64) {
+        const int k = base + lane;
+        const bool a = k < nc && !L.skip[k];
+        const unsigned long long am = __ballot(a);
+        if (a) L.act[n_act + popc(am & lt)] = (unsigned char)k;
+        n_act += popc(am);
+    }
+    wave_sync();
+
+    // ---------------- boundary (find_boundary), chunk by chunk ------------
+    int na = 0, nv = 0;
+    int a0 = 0;
+    while (a0 < n_act && !overflow) {
+        int a1 = a0, tb = 0;
+        while (a1 < n_act) {
+            const int c = L.cnt[L.act[a1]];
+            const int e = c > 0 ? c : 1;      // a cap without transitions holds its probe entry
+            if (tb + e > SA_TMAX) break;
+            tb += e;
+            ++a1;
+        }
+        if (a1 == a0) { overflow = true; break; }   // cannot happen: cnt <= 2 (MAXC - 1)
+        const int nck = a1 - a0;
+
+        // transitions in push order (cap, then m ascending, p before q); a cap
+        // without transitions gets one probe entry (tq = 2)
+        const int npairs = nck * nper;
+        int T = 0;
+        for (int base = 0; base < npairs; base += 64) {
+            const int p = base + lane;
+            int cnt = 0, kl = 0, k = 0, m = 0, first = 0, probe = 0;
+            v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
+            if (p < npairs) {
+                kl = p / nper;
+                k = L.act[a0 + kl];
+                const int mi = p % nper;
+                first = mi == 0;
+                if (L.cnt[k] == 0) {
+                    if (first) { probe = 1; cnt = 1; P = sample_circle_point(cap_at(L, k)); }
+                } else if (nc > 1) {
+                    m = mi < k ? mi : mi + 1;
+                    const int r = intersect_circles(cap_at(L, k), cap_at(L, m), P, Q);
+                    cnt = r == 0 ? 2 : (r == 1 ? 1 : 0);
+                }
+            }
+            const unsigned long long b1 = __ballot(cnt >= 1), b2 = __ballot(cnt == 2);
+            const int off = T + popc(b1 & lt) + popc(b2 & lt);
+            if (first && off <= SA_TMAX) L.seg[kl] = (unsigned short)off;
+            if (cnt >= 1 && off < SA_TMAX) {
+                L.ta[off] = probe ? 0.0 : angular_position(P, cap_at(L, k));
+                L.tx[off] = P.x; L.ty[off] = P.y; L.tz[off] = P.z;
+                L.tk[off] = (unsigned char)kl; L.tm[off] = (unsigned char)m;
+                L.tq[off] = (unsigned char)(probe ? 2 : 0);
+                if (cnt == 2 && off + 1 < SA_TMAX) {
+                    L.ta[off + 1] = angular_position(Q, cap_at(L, k));
+                    L.tx[off + 1] = Q.x; L.ty[off + 1] = Q.y; L.tz[off + 1] = Q.z;
+                    L.tk[off + 1] = (unsigned char)kl; L.tm[off + 1] = (unsigned char)m; L.tq[off + 1] = 1;
+                }
+            }
+            T += popc(b1) + popc(b2);
+        }
+        if (T > SA_TMAX) { overflow = true; break; }
+        if (lane == 0) L.seg[nck] = (unsigned short)T;
+        for (int x = lane; x < T; x += 64) L.perm[x] = (unsigned char)x;
+        wave_sync();
+        // every cap holds at least one entry (counts come from the same arithmetic)
+        {
+            bool empty_seg = false;
+            for (int kl = lane; kl < nck; kl += 64) empty_seg = empty_seg || L.seg[kl] >= L.seg[kl + 1];
+            if (__ballot(empty_seg)) { overflow = true; break; }
+        }
+
+        // stable angular sort inside each cap's segment: rank on (angle, push index)
+        for (int x = lane; x < T; x += 64) {
+            const int kl = L.tk[x];
+            const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+            const double ax = L.ta[x];
+            int r = 0;
+            for (int y = s0; y < s1; ++y) {
+                const double ay = L.ta[y];
+                r += (ay < ax || (ay == ax && y < x)) ? 1 : 0;
+            }
+            if (s0 + r < s1) L.perm[s0 + r] = (unsigned char)x;
+        }
+        wave_sync();
+
+        // dedup against the last kept point (Vec::dedup_by).  rm[y] marks a
+        // removed sorted position.  While no removal happens before the last
+        // position of a segment, "last kept" is the sorted predecessor and
+        // lanes decide independently; otherwise one lane per cap redoes it.
+        bool chain = false;
+        for (int y = lane; y < T; y += 64) {
+            const int kl = L.tk[L.perm[y]];
+            const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+            bool rem = false;
+            if (y > s0) {
+                const int a = L.perm[y], b = L.perm[y - 1];
+                const v3 pa = mk(L.tx[a], L.ty[a], L.tz[a]), pb = mk(L.tx[b], L.ty[b], L.tz[b]);
+                rem = norm2(pa - pb) < 1e-12;
+            }
+            L.rm[y] = rem ? 1 : 0;
+            chain = chain || (rem && y + 1 < s1);
+        }
+        if (__ballot(chain)) {
+            wave_sync();
+            for (int kl = lane; kl < nck; kl += 64) {
+                const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                int last = -1;
+                for (int y = s0; y < s1; ++y) {
+                    const int a = L.perm[y];
+                    bool rem = false;
+                    if (last >= 0) {
+                        const v3 pa = mk(L.tx[a], L.ty[a], L.tz[a]), pb = mk(L.tx[last], L.ty[last], L.tz[last]);
+                        rem = norm2(pa - pb) < 1e-12;
+                    }
+                    L.rm[y] = rem ? 1 : 0;
+                    if (!rem) last = a;
+                }
+            }
+        }
+        wave_sync();
+
+        // one slot per kept sorted position: segment to the next kept
+        // position (cyclic), or the probe test for a probe entry
+        for (int base = 0; base < T; base += 64) {
+            const int y = base + lane;
+            bool emit = false, segm = false;
+            double theta = 0.0;
+            v3 st = mk(0, 0, 0), en = mk(0, 0, 0);
+            int kk = 0, oth = 0;
+            if (y < T && !L.rm[y]) {
+                const int x = L.perm[y];
+                const int kl = L.tk[x];
+                kk = L.act[a0 + kl];
+                const Cap ck = cap_at(L, kk);
+                const int hint = L.hint[kk];
+                const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
+                if (L.tq[x] == 2) {
+                    // whole circle: a boundary loop unless its probe point is buried
+                    bool buried = hint != 0xFF && contains_strict(cap_at(L, hint), apt);
+                    for (int o = 0; o < nc && !buried; ++o)
+                        buried = (o != kk) && contains_strict(cap_at(L, o), apt);
+                    emit = !buried;
+                    st = apt; en = apt; theta = -two_pi;
+                } else {
+                    segm = true;
+                    const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                    int yn = y + 1 < s1 ? y + 1 : s0;
+                    while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
+                    const int bx = L.perm[yn];
+                    const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
+                    if (!(ccw < 1e-9)) {
+                        const v3 mid = midpoint_on_circle(apt, ck, ccw / 2.0);
+                        const int oa = L.tm[x], ob = L.tm[bx];
+                        bool bad = hint != 0xFF && contains_strict(cap_at(L, hint), mid);
+                        bad = bad || contains_strict(cap_at(L, oa), mid) || contains_strict(cap_at(L, ob), mid);
+                        for (int o = 0; o < nc && !bad; ++o)
+                            bad = (o != kk) && contains_strict(cap_at(L, o), mid);
+                        emit = !bad;
+                        if (emit) {
+                            st = mk(L.tx[bx], L.ty[bx], L.tz[bx]);
+                            en = apt;
+                            theta = -ccw;
+                            oth = oa;
+                        }
+                    }
+                }
+            }
+            const unsigned long long em = __ballot(emit), es = __ballot(emit && segm);
+            if (na + popc(em) > MAXA || nv + popc(es) > MAXV) { overflow = true; break; }
+            if (emit) {
+                const int A = na + popc(em & lt);
+                double* Ad = wd + WD_ARC + 7 * A;
+                Ad[0] = st.x; Ad[1] = st.y; Ad[2] = st.z;
+                Ad[3] = en.x; Ad[4] = en.y; Ad[5] = en.z;
+                Ad[6] = theta;
+                wi[WI_ARC + 2 * A] = kk;
+                wi[WI_ARC + 2 * A + 1] = segm ? 0 : 1;
+                if (segm) {
+                    const int V = nv + popc(es & lt);
+                    double* Vd = wd + WD_VTX + 3 * V;
+                    Vd[0] = en.x; Vd[1] = en.y; Vd[2] = en.z;
+                    wi[WI_VTX + 2 * V] = kk;
+                    wi[WI_VTX + 2 * V + 1] = oth;
+                }
+            }
+            na += popc(em);
+            nv += popc(es);
+        }
+        wave_sync();
+        a0 = a1;
+    }
+    if (overflow) {
+        if (lane == 0) atomicAdd(d.overflow, 1);
+        return empty;
+    }
+    // FullyBuried and the other arc-less outcomes contribute no gradient
+    if (na == 0) return empty;
+
+    // ---------------- pool allocation ----------------
+    int abase = 0, vbase = 0, obase = 0;
+    if (lane == 0) {
+        abase = atomicAdd(d.counters + 0, na);
+        vbase = atomicAdd(d.counters + 1, nv);
+        obase = atomicAdd(d.counters + 2, nc);
+    }
+    abase = __shfl(abase, 0, 64);
+    vbase = __shfl(vbase, 0, 64);
+    obase = __shfl(obase, 0, 64);
+    if (abase + na > d.arc_cap || vbase + nv > d.vtx_cap || obase + nc > d.own_cap) {
+        if (lane == 0) atomicAdd(d.overflow, 1);
+        return empty;
+    }
+    for (int c = lane; c < nc; c += 64) d.owners[obase + c] = L.own[c];
+
+    // ---------------- vertex root signs ----------------
+    for (int V = lane; V < nv; V += 64) {
+        const int cin = wi[WI_VTX + 2 * V], cout = wi[WI_VTX + 2 * V + 1];
+        const double* Vd = wd + WD_VTX + 3 * V;
+        const v3 ref = mk(Vd[0], Vd[1], Vd[2]);
+        v3 vp, vm;
+        double sg = 1.0;
+        if (vertex_point(cap_at(L, cin), cap_at(L, cout), 1.0, vp) &&
+            vertex_point(cap_at(L, cin), cap_at(L, cout), -1.0, vm))
+            sg = norm2(vp - ref) <= norm2(vm - ref) ? 1.0 : -1.0;
+        wd[WD_VSG + V] = sg;
+        d.vtx_atoms[vbase + V] = make_int4(ia, L.own[cin], L.own[cout], 0);
+        d.vtx_sign[vbase + V] = sg;
+        d.vtx_code[vbase + V] = 0xFF0000u | ((unsigned)(1 + cout) << 8) | (unsigned)(1 + cin);
+    }
+    wave_sync();
+
+    // ---------------- arc -> vertex links, arc records ----------------
+    for (int base = 0; base < na; base += 64) {
+        const int A = base + lane;
+        const bool valid = A < na;
+        int kc = 0, full = 1;
+        v3 st = mk(0, 0, 0), en = mk(0, 0, 0);
+        double theta = 0.0;
+        if (valid) {
+            kc = wi[WI_ARC + 2 * A];
+            full = wi[WI_ARC + 2 * A + 1];
+            const double* Ad = wd + WD_ARC + 7 * A;
+            st = mk(Ad[0], Ad[1], Ad[2]);
+            en = mk(Ad[3], Ad[4], Ad[5]);
+            theta = Ad[6];
+        }
+        int sv = -1, ev = -1;
+        for (int vb = 0; vb < nv; vb += 64) {
+            v3 pv = mk(0, 0, 0);
+            if (vb + lane < nv) {
+                const double* Vd = wd + WD_VTX + 3 * (vb + lane);
+                pv = mk(Vd[0], Vd[1], Vd[2]);
+            }
+            const int lim = nv - vb < 64 ? nv - vb : 64;
+            for (int j = 0; j < lim; ++j) {
+                const v3 q = mk(rld(pv.x, j), rld(pv.y, j), rld(pv.z, j));
+                if (sv < 0 && norm2(q - st) < 1e-10) sv = vb + j;
+                if (ev < 0 && norm2(q - en) < 1e-10) ev = vb + j;
+            }
+        }
+        if (valid) {
+            int4 at = make_int4(ia, L.own[kc], -1, -1);
+            double ssg = 1.0, esg = 1.0;
+            unsigned code = 0xFFFF00u | (unsigned)(1 + kc);
+            if (!full && sv >= 0 && ev >= 0) {
+                const int s_in = wi[WI_VTX + 2 * sv], s_out = wi[WI_VTX + 2 * sv + 1];
+                const int e_in = wi[WI_VTX + 2 * ev], e_out = wi[WI_VTX + 2 * ev + 1];
+                const double svs = wd[WD_VSG + sv], evs = wd[WD_VSG + ev];
+                int lc, mc;
+                if (s_in == kc) { lc = s_out; ssg = svs; } else { lc = s_in; ssg = -svs; }
+                if (e_in == kc) { mc = e_out; esg = evs; } else { mc = e_in; esg = -evs; }
+                at.z = L.own[lc];
+                at.w = L.own[mc];
+                code = ((unsigned)(mc == lc ? 0xFF : 1 + mc) << 16) | ((unsigned)(1 + lc) << 8) |
+                       (unsigned)(1 + kc);
+            }
+            d.arc_atoms[abase + A] = at;
+            d.arc_par[abase + A] = make_double4(theta, ssg, esg, 0.0);
+            d.arc_code[abase + A] = code;
+        }
+    }
+    return make_int4(abase, vbase, obase, na | (nv << 9) | (nc << 18));
 }
 
 }  // namespace
 
-__global__ void __launch_bounds__(64 * SW_WAVES) og_k_sasa_wave(og_sasa_data d, const double* pos,
-                                                                long long* facc, double gamma_scale) {
-    __shared__ WaveLds lds_all[SW_WAVES];
+__global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_arrange(og_sasa_data d, const double* pos) {
+    __shared__ ArrLds lds_all[SA_WAVES];
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    WaveLds& L = lds_all[wib];
-    const int slot = blockIdx.x * SW_WAVES + wib;
-    const int n_slots = gridDim.x * SW_WAVES;
-    double* sd = d.scratch + (size_t)slot * SD_TOTAL;
-    int* si = d.iscratch + (size_t)slot * SI_TOTAL;
+    ArrLds& L = lds_all[wib];
+    const int slot = blockIdx.x * SA_WAVES + wib;
+    const int n_slots = gridDim.x * SA_WAVES;
+    double* wd = d.scratch + (size_t)slot * WD_TOTAL;
+    int* wi = d.iscratch + (size_t)slot * WI_TOTAL;
     const unsigned long long lt = lanemask_lt(lane);
-    const double two_pi = 2.0 * 3.14159265358979323846;
-
     for (int w = slot; w < d.n_work; w += n_slots) {
-        const int ia = d.work[w];
-        const double gam = d.gamma[ia] * gamma_scale;
-        if (gam == 0.0) continue;
-        const v3 pi = ldp(pos, ia);
-        const double ri = d.radius[ia];
-
-        // ---------------- caps (build_atom_cache + build_caps) ----------------
-        // Candidates in ascending atom order; an enclosing neighbour ends the
-        // scan (no accessible surface) unless the cap count overflowed first.
-        int nc = 0;
-        bool enclosed = false, overflow = false;
-        for (int t = d.tile_begin[w]; t < d.tile_end[w]; ++t) {
-            const double4 tb = d.tile_bound[t];
-            const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
-            const double reach = ri + d.max_radius + tb.w;
-            if (dxc * dxc + dyc * dyc + dzc * dzc > reach * reach) continue;
-            const int j = d.tile_atom0[t] + lane;
-            const bool valid = lane < d.tile_cnt[t] && j != ia;
-            bool is_cap = false, enc = false;
-            v3 axis = mk(0, 0, 0);
-            double ca = 0.0;
-            if (valid) {
-                const v3 pj = ldp(pos, j);
-                const double rj = d.radius[j];
-                const double dd = norm(pj - pi);
-                if (dd <= ri + rj) {
-                    if (dd + ri <= rj) {
-                        enc = true;
-                    } else if (!(dd + rj <= ri || dd >= ri + rj)) {
-                        ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
-                        if (ca >= -1.0 && ca <= 1.0) {
-                            is_cap = true;
-                            axis = (pj - pi) / dd;
-                        }
-                    }
-                }
-            }
-            unsigned long long m = __ballot(is_cap);
-            const unsigned long long em = __ballot(enc);
-            if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
-            if (nc + popc(m) > MAXC) { overflow = true; break; }
-            if (is_cap && ((m >> lane) & 1ull)) {
-                const int p = nc + popc(m & lt);
-                L.cx[p] = axis.x; L.cy[p] = axis.y; L.cz[p] = axis.z; L.cc[p] = ca;
-                L.own[p] = j;
-            }
-            nc += popc(m);
-            if (em) { enclosed = true; break; }
-        }
+        const int4 rec = arrange_atom(L, d, pos, w, lane, lt, wd, wi);
+        if (lane == 0) d.rec[w] = rec;
         wave_sync();
-        if (overflow) {
-            if (lane == 0) atomicAdd(d.overflow, 1);
-            continue;
-        }
-        if (enclosed || nc == 0) continue;
+    }
+}
 
-        // ---------------- boundary (find_boundary) ----------------
-        int na = 0, nv = 0;
-        for (int k = 0; k < nc && !overflow; ++k) {
-            const Cap ck = cap_at(L, k);
-            // transitions of circle k, in origami's push order (cap m ascending, p then q)
-            int M = 0;
-            for (int base = 0; base < nc; base += 64) {
-                const int m = base + lane;
-                int cnt = 0;
-                v3 p = mk(0, 0, 0), q = mk(0, 0, 0);
-                if (m < nc && m != k) {
-                    const int r = intersect_circles(ck, cap_at(L, m), p, q);
-                    cnt = r == 0 ? 2 : (r == 1 ? 1 : 0);
-                }
-                const unsigned long long b1 = __ballot(cnt >= 1), b2 = __ballot(cnt == 2);
-                const int off = M + popc(b1 & lt) + popc(b2 & lt);
-                if (cnt >= 1) {
-                    L.ta[off] = angular_position(p, ck);
-                    L.tx[off] = p.x; L.ty[off] = p.y; L.tz[off] = p.z; L.to[off] = m;
-                }
-                if (cnt == 2) {
-                    L.ta[off + 1] = angular_position(q, ck);
-                    L.tx[off + 1] = q.x; L.ty[off + 1] = q.y; L.tz[off + 1] = q.z; L.to[off + 1] = m;
-                }
-                M += popc(b1) + popc(b2);
-            }
-            wave_sync();
-            // Exact early-out: every point origami tests on circle k (segment
-            // midpoints are rotations of transition points about axis_k, which
-            // keep their height h and radius rn; the probe is on the circle)
-            // satisfies dot(x, axis_m) >= h*ct - rn*st with ct, st the cosine
-            // and sine between the axes.  When that bound clears cap m's
-            // containment test for all of them, circle k contributes nothing.
-            {
-                double hmin = ck.ca, hmax = ck.ca;
-                double rmx = sqrt(fmax(1.0 - ck.ca * ck.ca, 0.0));
-                for (int x = lane; x < M; x += 64) {
-                    const v3 p = mk(L.tx[x], L.ty[x], L.tz[x]);
-                    const double h = dot(p, ck.axis);
-                    const double rn = norm(p - ck.axis * h);
-                    hmin = fmin(hmin, h);
-                    hmax = fmax(hmax, h);
-                    rmx = fmax(rmx, rn);
-                }
-                for (int o = 32; o >= 1; o >>= 1) {
-                    hmin = fmin(hmin, __shfl_xor(hmin, o, 64));
-                    hmax = fmax(hmax, __shfl_xor(hmax, o, 64));
-                    rmx = fmax(rmx, __shfl_xor(rmx, o, 64));
-                }
-                bool inside = false;
-                for (int base = 0; base < nc; base += 64) {
-                    const int m = base + lane;
-                    bool cont = false;
-                    if (m < nc && m != k) {
-                        const v3 am = mk(L.cx[m], L.cy[m], L.cz[m]);
-                        const double ct = dot(ck.axis, am);
-                        const double st = norm(cross(ck.axis, am));
-                        const double lo = fmin(hmin * ct, hmax * ct) - rmx * st;
-                        cont = lo > L.cc[m] + 1e-11;
-                    }
-                    if (__ballot(cont)) inside = true;
-                }
-                if (inside) continue;
-            }
-            // stable sort of the M transitions by angle (bitonic on (angle, index))
-            int P = 1;
-            while (P < M) P <<= 1;
-            for (int x = lane; x < P; x += 64) L.perm[x] = (unsigned char)x;
-            wave_sync();
-            for (int k2 = 2; k2 <= P; k2 <<= 1) {
-                for (int j2 = k2 >> 1; j2 > 0; j2 >>= 1) {
-                    for (int x = lane; x < P; x += 64) {
-                        const int y = x ^ j2;
-                        if (y > x) {
-                            const int a = L.perm[x], b = L.perm[y];
-                            const bool asc = (x & k2) == 0;
-                            const bool swap = asc ? key_less(L, b, a, M) : key_less(L, a, b, M);
-                            if (swap) { L.perm[x] = (unsigned char)b; L.perm[y] = (unsigned char)a; }
-                        }
-                    }
-                    wave_sync();
-                }
-            }
-            // Dedup against the last kept point (Vec::dedup_by).  When no
-            // removal happens before the last position, "last kept" is always
-            // the sorted predecessor and lanes decide independently.
-            bool chain = false;
-            for (int base = 0; base < M; base += 64) {
-                const int x = base + lane;
-                bool c = false;
-                if (x >= 1 && x + 1 < M) {
-                    const int a = L.perm[x], b = L.perm[x - 1];
-                    const v3 pa = mk(L.tx[a], L.ty[a], L.tz[a]), pb = mk(L.tx[b], L.ty[b], L.tz[b]);
-                    c = norm2(pa - pb) < 1e-12;
-                }
-                if (__ballot(c)) chain = true;
-            }
-            int nk = 0;
-            if (!chain) {
-                for (int base = 0; base < M; base += 64) {
-                    const int x = base + lane;
-                    bool keep = false;
-                    if (x < M) {
-                        keep = true;
-                        if (x > 0) {
-                            const int a = L.perm[x], b = L.perm[x - 1];
-                            const v3 pa = mk(L.tx[a], L.ty[a], L.tz[a]), pb = mk(L.tx[b], L.ty[b], L.tz[b]);
-                            keep = !(norm2(pa - pb) < 1e-12);
-                        }
-                    }
-                    const unsigned long long km = __ballot(keep);
-                    if (keep) L.kept[nk + popc(km & lt)] = L.perm[x];
-                    nk += popc(km);
-                }
-            } else {
-                if (lane == 0) {
-                    int wk = 0;
-                    for (int x = 0; x < M; ++x) {
-                        const int a = L.perm[x];
-                        if (wk > 0) {
-                            const int b = L.kept[wk - 1];
-                            const v3 pa = mk(L.tx[a], L.ty[a], L.tz[a]), pb = mk(L.tx[b], L.ty[b], L.tz[b]);
-                            if (norm2(pa - pb) < 1e-12) continue;
-                        }
-                        L.kept[wk++] = (unsigned char)a;
-                    }
-                    L.scalar = wk;
-                }
-                wave_sync();
-                nk = L.scalar;
-            }
-            wave_sync();
-            if (nk == 0) {
-                // whole circle: boundary loop unless the probe point is buried
-                const v3 probe = sample_circle_point(ck);
-                bool buried = false;
-                for (int base = 0; base < nc; base += 64) {
-                    const int o = base + lane;
-                    const bool b = o < nc && o != k && contains_strict(cap_at(L, o), probe);
-                    if (__ballot(b)) buried = true;
-                }
-                if (!buried) {
-                    if (na >= SW_MAXA) { overflow = true; break; }
-                    if (lane == 0) {
-                        double* A = sd + SD_ARC + 7 * na;
-                        A[0] = probe.x; A[1] = probe.y; A[2] = probe.z;
-                        A[3] = probe.x; A[4] = probe.y; A[5] = probe.z;
-                        A[6] = -two_pi;
-                        int* AI = si + SI_ARC + 4 * na;
-                        AI[0] = k; AI[1] = 1; AI[2] = -1; AI[3] = -1;
-                    }
-                    ++na;
-                }
-                continue;
-            }
-            for (int base = 0; base < nk; base += 64) {
-                const int x = base + lane;
-                bool emit = false;
-                double ccw = 0.0;
-                int ax_ = 0, by_ = 0;
-                if (x < nk) {
-                    ax_ = L.kept[x];
-                    by_ = L.kept[(x + 1) % nk];
-                    ccw = wrap_2pi_positive(L.ta[by_] - L.ta[ax_]);
-                    if (!(ccw < 1e-9)) {
-                        const v3 apt = mk(L.tx[ax_], L.ty[ax_], L.tz[ax_]);
-                        const v3 mid = midpoint_on_circle(apt, ck, ccw / 2.0);
-                        // any() over the other caps; the two caps bounding the
-                        // segment are the likeliest to contain it, test them first
-                        const int oa = L.to[ax_], ob = L.to[by_];
-                        bool bad = contains_strict(cap_at(L, oa), mid) || contains_strict(cap_at(L, ob), mid);
-                        for (int o = 0; o < nc && !bad; ++o)
-                            bad = (o != k) && contains_strict(cap_at(L, o), mid);
-                        emit = !bad;
-                    }
-                }
-                const unsigned long long em = __ballot(emit);
-                if (na + popc(em) > SW_MAXA || nv + popc(em) > SW_MAXV) { overflow = true; break; }
-                if (emit) {
-                    const int A = na + popc(em & lt);
-                    double* Ad = sd + SD_ARC + 7 * A;
-                    Ad[0] = L.tx[by_]; Ad[1] = L.ty[by_]; Ad[2] = L.tz[by_];
-                    Ad[3] = L.tx[ax_]; Ad[4] = L.ty[ax_]; Ad[5] = L.tz[ax_];
-                    Ad[6] = -ccw;
-                    int* AI = si + SI_ARC + 4 * A;
-                    AI[0] = k; AI[1] = 0; AI[2] = -1; AI[3] = -1;
-                    const int V = nv + popc(em & lt);
-                    double* Vd = sd + SD_VTX + 4 * V;
-                    Vd[0] = L.tx[ax_]; Vd[1] = L.ty[ax_]; Vd[2] = L.tz[ax_]; Vd[3] = 1.0;
-                    int* VI = si + SI_VTX + 2 * V;
-                    VI[0] = k; VI[1] = L.to[ax_];
-                }
-                na += popc(em);
-                nv += popc(em);
-            }
-            wave_sync();
-        }
-        if (overflow) {
-            if (lane == 0) atomicAdd(d.overflow, 1);
-            continue;
-        }
-        if (na == 0) continue;
-        wave_sync();
-
-        // ---------------- cache: vertex signs, arc links ----------------
-        for (int V = lane; V < nv; V += 64) {
-            const int* VI = si + SI_VTX + 2 * V;
-            double* Vd = sd + SD_VTX + 4 * V;
-            const Cap cin = cap_at(L, VI[0]), cout = cap_at(L, VI[1]);
-            const v3 ref = mk(Vd[0], Vd[1], Vd[2]);
-            v3 vp, vm;
-            double sg = 1.0;
-            if (vertex_point(cin, cout, 1.0, vp) && vertex_point(cin, cout, -1.0, vm))
-                sg = norm2(vp - ref) <= norm2(vm - ref) ? 1.0 : -1.0;
-            Vd[3] = sg;
-            L.xvtx[V] = (unsigned)(0xFF0000u | ((unsigned)(1 + VI[1]) << 8) | (unsigned)(1 + VI[0]));
-        }
-        for (int A = lane; A < na; A += 64) {
-            int* AI = si + SI_ARC + 4 * A;
-            int sv = -1, ev = -1;
-            if (!AI[1]) {
-                const double* Ad = sd + SD_ARC + 7 * A;
-                const v3 st = mk(Ad[0], Ad[1], Ad[2]), en = mk(Ad[3], Ad[4], Ad[5]);
-                for (int V = 0; V < nv && (sv < 0 || ev < 0); ++V) {
-                    const double* Vd = sd + SD_VTX + 4 * V;
-                    const v3 pv = mk(Vd[0], Vd[1], Vd[2]);
-                    if (sv < 0 && norm2(pv - st) < 1e-10) sv = V;
-                    if (ev < 0 && norm2(pv - en) < 1e-10) ev = V;
-                }
-                AI[2] = sv;
-                AI[3] = ev;
-            }
-        }
-        wave_sync();
-        // arc roles as local atom indices (0 = atom i, 1 + c = owner of cap c);
-        // byte 0xFF = role absent or already covered by an earlier role
-        for (int A = lane; A < na; A += 64) {
-            const int* AI = si + SI_ARC + 4 * A;
-            const int kc = AI[0];
-            unsigned xl = 0xFF, xm = 0xFF;
-            if (!(AI[1] || AI[2] < 0 || AI[3] < 0)) {
-                const int* SV = si + SI_VTX + 2 * AI[2];
-                const int* EV = si + SI_VTX + 2 * AI[3];
-                const int lc = SV[0] == kc ? SV[1] : SV[0];
-                const int mc = EV[0] == kc ? EV[1] : EV[0];
-                xl = 1 + lc;
-                xm = mc == lc ? 0xFF : 1 + mc;
-            }
-            L.xarc[A] = (xm << 16) | (xl << 8) | (unsigned)(1 + kc);
-        }
-        wave_sync();
-
-        // ---------------- gradient terms, one (item, role, axis) per lane ----
-        const v3 zero = mk(0, 0, 0);
-        for (int it = lane; it < 12 * na + 9 * nv; it += 64) {
-            if (it < 12 * na) {
-                const int A = it / 12, role = (it % 12) / 3, axis = it % 3;
-                const unsigned xr = L.xarc[A];
-                const int x = role == 0 ? 0 : (int)((xr >> (8 * (role - 1))) & 0xFF);
-                double term = 0.0;
-                if (x != 0xFF) {
-                    const int* AI = si + SI_ARC + 4 * A;
-                    const double theta = sd[SD_ARC + 7 * A + 6];
-                    const int kc = AI[0];
-                    const int ka = L.own[kc];
-                    const v3 pk = ldp(pos, ka);
-                    const double rk = d.radius[ka];
-                    const v3 b = unit(axis);
-                    const v3 dri = x == 0 ? b : zero;
-                    const v3 drk = x == 1 + kc ? b : zero;
-                    const double dcos = cos_alpha_directional(pi, pk, ri, rk, dri, drk);
-                    if ((xr >> 8 & 0xFF) == 0xFF) {
-                        term = theta * dcos;
-                    } else {
-                        const int sv = AI[2], ev = AI[3];
-                        const int* SV = si + SI_VTX + 2 * sv;
-                        const int* EV = si + SI_VTX + 2 * ev;
-                        const double svs = sd[SD_VTX + 4 * sv + 3], evs = sd[SD_VTX + 4 * ev + 3];
-                        int lc, mc;
-                        double ssg, esg;
-                        if (SV[0] == kc) { lc = SV[1]; ssg = svs; } else { lc = SV[0]; ssg = -svs; }
-                        if (EV[0] == kc) { mc = EV[1]; esg = evs; } else { mc = EV[0]; esg = -evs; }
-                        const int la = L.own[lc], ma = L.own[mc];
-                        const v3 pl = ldp(pos, la), pm = ldp(pos, ma);
-                        const double rl = d.radius[la], rm = d.radius[ma];
-                        const v3 drl = x == 1 + lc ? b : zero;
-                        const v3 drm = x == 1 + mc ? b : zero;
-                        const double dik = norm(pk - pi);
-                        const double cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
-                        double dth = 0.0;
-                        if (!arc_theta_jvp(pi, pk, pl, pm, ri, rk, rl, rm, ssg, esg, dri, drk, drl, drm, dth))
-                            dth = 0.0;
-                        term = theta * dcos + cak * dth;
-                    }
-                }
-                sd[SD_TA + 12 * A + 3 * role + axis] = term;
-            } else {
-                const int jt = it - 12 * na;
-                const int V = jt / 9, role = (jt % 9) / 3, axis = jt % 3;
-                const int* VI = si + SI_VTX + 2 * V;
-                const int kcap = VI[0], lcap = VI[1];
-                const int x = role == 0 ? 0 : role == 1 ? 1 + kcap : 1 + lcap;
-                const int ka = L.own[kcap], la = L.own[lcap];
-                const v3 pk = ldp(pos, ka), pl = ldp(pos, la);
+// One JVP term per thread: arc items (12 per arc: role i, k, l, m x axis)
+// then vertex items (9 per vertex: role i, in, out x axis).
+__global__ void __launch_bounds__(256) og_k_sasa_terms(og_sasa_data d, const double* pos) {
+    const int n_a = min(d.counters[0], d.arc_cap);
+    const int n_v = min(d.counters[1], d.vtx_cap);
+    const long long total = 12ll * n_a + 9ll * n_v;
+    const v3 zero = mk(0, 0, 0);
+    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
+         it += (long long)gridDim.x * blockDim.x) {
+        if (it < 12ll * n_a) {
+            const int A = (int)(it / 12), role = (int)(it % 12) / 3, axis = (int)(it % 3);
+            const int4 at = d.arc_atoms[A];
+            const int xa = role == 0 ? at.x : role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
+            double term = 0.0;
+            if (xa >= 0) {
+                const double4 pr = d.arc_par[A];
+                const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y);
+                const double ri = d.radius[at.x], rk = d.radius[at.y];
                 const v3 b = unit(axis);
-                const v3 dri = x == 0 ? b : zero;
-                const v3 drk = x == 1 + kcap ? b : zero;
-                const v3 drl = x == 1 + lcap ? b : zero;
-                double de = 0.0;
-                if (!vertex_epsilon_jvp(pi, pk, pl, ri, d.radius[ka], d.radius[la], sd[SD_VTX + 4 * V + 3],
-                                        dri, drk, drl, de))
-                    de = 0.0;
-                sd[SD_TV + 9 * V + 3 * role + axis] = de;
+                const v3 dri = xa == at.x ? b : zero;
+                const v3 drk = xa == at.y ? b : zero;
+                const double dcos = cos_alpha_directional(pi, pk, ri, rk, dri, drk);
+                if (at.z < 0) {
+                    term = pr.x * dcos;
+                } else {
+                    const v3 pl = ldp(pos, at.z), pm = ldp(pos, at.w);
+                    const double rl = d.radius[at.z], rm = d.radius[at.w];
+                    const v3 drl = xa == at.z ? b : zero;
+                    const v3 drm = xa == at.w ? b : zero;
+                    const double dik = norm(pk - pi);
+                    const double cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
+                    double dth = 0.0;
+                    if (!arc_theta_jvp(pi, pk, pl, pm, ri, rk, rl, rm, pr.y, pr.z, dri, drk, drl, drm, dth))
+                        dth = 0.0;
+                    term = pr.x * dcos + cak * dth;
+                }
             }
+            d.arc_terms[it] = term;
+        } else {
+            const long long jt = it - 12ll * n_a;
+            const int V = (int)(jt / 9), role = (int)(jt % 9) / 3, axis = (int)(jt % 3);
+            const int4 at = d.vtx_atoms[V];
+            const int xa = role == 0 ? at.x : role == 1 ? at.y : at.z;
+            const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y), pl = ldp(pos, at.z);
+            const v3 b = unit(axis);
+            const v3 dri = xa == at.x ? b : zero;
+            const v3 drk = xa == at.y ? b : zero;
+            const v3 drl = xa == at.z ? b : zero;
+            double de = 0.0;
+            if (!vertex_epsilon_jvp(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z], d.vtx_sign[V],
+                                    dri, drk, drl, de))
+                de = 0.0;
+            d.vtx_terms[jt] = de;
         }
-        wave_sync();
+    }
+}
 
-        // ---------------- per-atom sums in origami's order, scatter ----------
-        // Lane x sums, for its three axes, the terms of every arc (in arc
-        // order) and then every vertex (in vertex order) that involve it.
-        const double r_sq = ri * ri;
-        for (int x = lane; x <= nc; x += 64) {
-            double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
-            bool any = false;
-            for (int A = 0; A < na; ++A) {
-                const unsigned xr = L.xarc[A];
-                int role = -1;
-                if (x == 0) role = 0;
-                else if (x == (int)(xr & 0xFF)) role = 1;
-                else if (x == (int)(xr >> 8 & 0xFF)) role = 2;
-                else if (x == (int)(xr >> 16 & 0xFF)) role = 3;
-                if (role >= 0) {
-                    const double* T = sd + SD_TA + 12 * A + 3 * role;
-                    sa0 += T[0]; sa1 += T[1]; sa2 += T[2];
-                    any = true;
-                }
-            }
-            for (int V = 0; V < nv; ++V) {
-                const unsigned xr = L.xvtx[V];
-                int role = -1;
-                if (x == 0) role = 0;
-                else if (x == (int)(xr & 0xFF)) role = 1;
-                else if (x == (int)(xr >> 8 & 0xFF)) role = 2;
-                if (role >= 0) {
-                    const double* T = sd + SD_TV + 9 * V + 3 * role;
-                    sv0 += T[0]; sv1 += T[1]; sv2 += T[2];
-                    any = true;
-                }
-            }
-            if (!any) continue;
-            const int atom = x == 0 ? ia : L.own[x - 1];
-            const double s_a[3] = {sa0, sa1, sa2}, s_v[3] = {sv0, sv1, sv2};
-            for (int axis = 0; axis < 3; ++axis) {
-                const double g = (-s_a[axis] - s_v[axis]) * r_sq;
-                const double f = -(g * gam);
-                if (f != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + axis, og_to_fix(f));
+__global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data d, long long* facc,
+                                                                   double gamma_scale) {
+    const int lane = threadIdx.x & 63;
+    const int w = blockIdx.x * SA_WAVES + (threadIdx.x >> 6);
+    if (w >= d.n_work) return;
+    const int4 rec = d.rec[w];
+    const int na = rec.w & 511, nv = (rec.w >> 9) & 511, nc = rec.w >> 18;
+    if (na == 0) return;
+    const int ia = d.work[w];
+    const double gam = d.gamma[ia] * gamma_scale;
+    const double ri = d.radius[ia];
+    const double r_sq = ri * ri;
+    for (int x = lane; x <= nc; x += 64) {
+        double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
+        bool any = false;
+        for (int A = rec.x; A < rec.x + na; ++A) {
+            const unsigned xr = d.arc_code[A];
+            int role = -1;
+            if (x == 0) role = 0;
+            else if (x == (int)(xr & 0xFF)) role = 1;
+            else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
+            else if (x == (int)((xr >> 16) & 0xFF)) role = 3;
+            if (role >= 0) {
+                const double* T = d.arc_terms + 12ll * A + 3 * role;
+                sa0 += T[0]; sa1 += T[1]; sa2 += T[2];
+                any = true;
             }
         }
-        wave_sync();
+        for (int V = rec.y; V < rec.y + nv; ++V) {
+            const unsigned xr = d.vtx_code[V];
+            int role = -1;
+            if (x == 0) role = 0;
+            else if (x == (int)(xr & 0xFF)) role = 1;
+            else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
+            if (role >= 0) {
+                const double* T = d.vtx_terms + 9ll * V + 3 * role;
+                sv0 += T[0]; sv1 += T[1]; sv2 += T[2];
+                any = true;
+            }
+        }
+        if (!any) continue;
+        const int atom = x == 0 ? ia : d.owners[rec.z + x - 1];
+        const double g0 = (-sa0 - sv0) * r_sq, g1 = (-sa1 - sv1) * r_sq, g2 = (-sa2 - sv2) * r_sq;
+        const double f0 = -(g0 * gam), f1 = -(g1 * gam), f2 = -(g2 * gam);
+        if (f0 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 0, og_to_fix(f0));
+        if (f1 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 1, og_to_fix(f1));
+        if (f2 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 2, og_to_fix(f2));
     }
 }
 
@@ -546,10 +660,15 @@ __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const
     if (lane == 0) out[t] = make_double4(cx, cy, cz, sqrt(r2) * (1.0 + 1e-12) + 1e-9);
 }
 
+__global__ void og_k_sasa_reset(int* counters) {
+    if (threadIdx.x < 4) counters[threadIdx.x] = 0;
+}
+
 int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys,
                   const std::vector<int>& atom_off, const std::vector<double>& radius,
                   const std::vector<double>& gamma, const std::vector<int>& sys_tile_off,
                   const int* tile_atom0_dev, const int* tile_cnt_dev, int n_tiles) {
+    (void)atom_off;
     std::vector<int> work, tb, te;
     double rmax = 0.0;
     for (int i = 0; i < n_atoms; ++i) rmax = std::max(rmax, radius[i]);
@@ -564,16 +683,24 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
     d->n_tiles = n_tiles;
     d->tile_atom0 = tile_atom0_dev;
     d->tile_cnt = tile_cnt_dev;
-    int dev = 0, n_cu = 256;
+    int dev = 0;
+    d->n_cu = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
-            n_cu = prop.multiProcessorCount;
+            d->n_cu = prop.multiProcessorCount;
     }
-    int blocks = (d->n_work + SW_WAVES - 1) / SW_WAVES;
-    blocks = std::min(blocks, n_cu * 8);
+    int blocks = (d->n_work + SA_WAVES - 1) / SA_WAVES;
+    blocks = std::min(blocks, d->n_cu * 8);
     d->blocks = std::max(blocks, 1);
-    d->stride = d->blocks * SW_WAVES;
+    d->stride = d->blocks * SA_WAVES;
+    // pool capacity: OG_SASA_POOL_ARCS arcs and vertices per SASA atom on average
+    const long long per = OG_SASA_POOL_ARCS;
+    d->arc_cap = (int)std::min<long long>(per * std::max(d->n_work, 1), 1ll << 30);
+    d->vtx_cap = d->arc_cap;
+    d->own_cap = (int)std::min<long long>((long long)MAXC * std::max(d->n_work, 1), 1ll << 30);
+
+    auto fail = [](const char* m) { g_sasa_err = m; return -2; };
     auto up = [](auto** dst, const auto& v) -> bool {
         using T = typename std::remove_reference<decltype(v)>::type::value_type;
         size_t bytes = std::max<size_t>(v.size(), 1) * sizeof(T);
@@ -582,33 +709,36 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
             return false;
         return true;
     };
+    auto zalloc = [](auto** dst, size_t count) -> bool {
+        using T = typename std::remove_reference<decltype(**dst)>::type;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        if (hipMalloc((void**)dst, bytes) != hipSuccess) return false;
+        return hipMemset(*dst, 0, bytes) == hipSuccess;
+    };
     if (!up(&d->work, work) || !up(&d->tile_begin, tb) || !up(&d->tile_end, te) ||
-        !up(&d->radius, radius) || !up(&d->gamma, gamma)) {
-        g_sasa_err = "device allocation failed";
-        return -2;
-    }
-    if (hipMalloc((void**)&d->tile_bound, std::max(n_tiles, 1) * sizeof(double4)) != hipSuccess) {
-        g_sasa_err = "device allocation failed";
-        return -2;
-    }
+        !up(&d->radius, radius) || !up(&d->gamma, gamma))
+        return fail("device allocation failed");
+    if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 4) ||
+        !zalloc(&d->rec, (size_t)d->n_work))
+        return fail("device allocation failed");
     if (d->n_work > 0) {
-        if (hipMalloc((void**)&d->scratch, (size_t)SD_TOTAL * d->stride * sizeof(double)) != hipSuccess ||
-            hipMalloc((void**)&d->iscratch, (size_t)SI_TOTAL * d->stride * sizeof(int)) != hipSuccess) {
-            g_sasa_err = "SASA scratch allocation failed";
-            return -2;
-        }
-    }
-    if (hipMalloc((void**)&d->overflow, sizeof(int)) != hipSuccess ||
-        hipMemset(d->overflow, 0, sizeof(int)) != hipSuccess) {
-        g_sasa_err = "device allocation failed";
-        return -2;
+        if (!zalloc(&d->scratch, (size_t)WD_TOTAL * d->stride) ||
+            !zalloc(&d->iscratch, (size_t)WI_TOTAL * d->stride) ||
+            !zalloc(&d->arc_atoms, (size_t)d->arc_cap) || !zalloc(&d->arc_par, (size_t)d->arc_cap) ||
+            !zalloc(&d->arc_code, (size_t)d->arc_cap) || !zalloc(&d->arc_terms, 12 * (size_t)d->arc_cap) ||
+            !zalloc(&d->vtx_atoms, (size_t)d->vtx_cap) || !zalloc(&d->vtx_sign, (size_t)d->vtx_cap) ||
+            !zalloc(&d->vtx_code, (size_t)d->vtx_cap) || !zalloc(&d->vtx_terms, 9 * (size_t)d->vtx_cap) ||
+            !zalloc(&d->owners, (size_t)d->own_cap))
+            return fail("SASA pool allocation failed");
     }
     return 0;
 }
 
 void og_sasa_free(og_sasa_data* d) {
     void* p[] = {d->work, d->tile_begin, d->tile_end, d->radius, d->gamma, d->tile_bound,
-                 d->scratch, d->iscratch, d->overflow};
+                 d->scratch, d->iscratch, d->overflow, d->counters, d->rec,
+                 d->arc_atoms, d->arc_par, d->arc_code, d->arc_terms,
+                 d->vtx_atoms, d->vtx_sign, d->vtx_code, d->vtx_terms, d->owners};
     for (void* q : p)
         if (q) hipFree(q);
     *d = og_sasa_data{};
@@ -616,11 +746,14 @@ void og_sasa_free(og_sasa_data* d) {
 
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
                    hipStream_t s) {
-    if (d->n_work == 0) return 0;
+    if (d->n_work == 0 || gamma_scale == 0.0) return 0;
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles), dim3(64), 0, s, pos, d->tile_atom0,
                        d->tile_cnt, d->n_tiles, d->tile_bound);
-    hipLaunchKernelGGL(og_k_sasa_wave, dim3(d->blocks), dim3(64 * SW_WAVES), 0, s, *d, pos, facc,
-                       gamma_scale);
+    hipLaunchKernelGGL(og_k_sasa_reset, dim3(1), dim3(64), 0, s, d->counters);
+    hipLaunchKernelGGL(og_k_sasa_arrange, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
+    hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
+                       0, s, *d, facc, gamma_scale);
     if (hipGetLastError() != hipSuccess) {
         g_sasa_err = "kernel launch failed";
         return -2;
@@ -630,5 +763,6 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
 
 int og_sasa_overflow(og_sasa_data* d, int* count) {
     if (hipMemcpy(count, d->overflow, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -2;
+    if (*count != 0 && hipMemset(d->overflow, 0, sizeof(int)) != hipSuccess) return -2;
     return 0;
 }
