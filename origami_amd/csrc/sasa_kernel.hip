@@ -66,11 +66,12 @@ using namespace og_sasa;
 namespace {
 
 struct ArrLds {
-    double cx[MAXC], cy[MAXC], cz[MAXC], cc[MAXC];   // caps: axis, cos(alpha)
+    double4 cap4[MAXC];                              // caps: axis, cos(alpha)
     int own[MAXC];                                   // cap owner (global atom)
     unsigned char hint[MAXC];                        // a cap that likely buries circle k
     unsigned char skip[MAXC];                        // circle k provably contributes nothing
     unsigned char act[MAXC];                         // caps still to process, ascending
+    unsigned char hl[MAXC];                          // caps with a hint, ascending
     unsigned short cnt[MAXC];                        // transitions of cap k
     unsigned short seg[MAXC + 1];                    // chunk: segment start per chunk cap
     double ta[SA_TMAX];                              // chunk: transition angles (push order)
@@ -99,7 +100,29 @@ __device__ __forceinline__ double rld(double v, int l) {
 }
 
 __device__ __forceinline__ Cap cap_at(const ArrLds& L, int k) {
-    return {mk(L.cx[k], L.cy[k], L.cz[k]), L.cc[k]};
+    const double4 c = L.cap4[k];
+    return {mk(c.x, c.y, c.z), c.w};
+}
+
+__device__ __forceinline__ Cap as_cap(double4 c) { return {mk(c.x, c.y, c.z), c.w}; }
+
+// any(o != skip_k && contains_strict(cap o, p)) over the nc caps, four caps
+// per step so the LDS loads overlap.  Written with a loop-carried flag: the
+// same loop with early returns from inside the unrolled body gave wrong
+// results (hipcc, ROCm 7.2, gfx950) when inlined into lane-divergent code.
+__device__ __forceinline__ bool any_contains(const ArrLds& L, int nc, int skip_k, v3 p) {
+    int o = 0;
+    bool r = false;
+    for (; o + 4 <= nc && !r; o += 4) {
+        const double4 c0 = L.cap4[o], c1 = L.cap4[o + 1], c2 = L.cap4[o + 2], c3 = L.cap4[o + 3];
+        const bool b0 = o != skip_k && contains_strict(as_cap(c0), p);
+        const bool b1 = o + 1 != skip_k && contains_strict(as_cap(c1), p);
+        const bool b2 = o + 2 != skip_k && contains_strict(as_cap(c2), p);
+        const bool b3 = o + 3 != skip_k && contains_strict(as_cap(c3), p);
+        r = b0 | b1 | b2 | b3;
+    }
+    for (; o < nc && !r; ++o) r = (o != skip_k) && contains_strict(cap_at(L, o), p);
+    return r;
 }
 
 // Lower bound of dot(x, axis_m) over the rotations x of point p about
@@ -158,7 +181,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
         if (nc + popc(m) > MAXC) { overflow = true; break; }
         if ((m >> lane) & 1ull) {
             const int p = nc + popc(m & lt);
-            L.cx[p] = axis.x; L.cy[p] = axis.y; L.cz[p] = axis.z; L.cc[p] = ca;
+            L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
             L.own[p] = j;
         }
         nc += popc(m);
@@ -204,13 +227,22 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     // between the axes).  When that bound clears cap m's containment test by
     // 1e-11 (>> rounding) for every transition point, or for the probe,
     // circle k contributes no arc and is dropped.
+    int n_h = 0;
+    for (int base = 0; base < nc; base += 64) {
+        const int k = base + lane;
+        const bool hh = k < nc && L.hint[k] != 0xFF;
+        const unsigned long long hm = __ballot(hh);
+        if (hh) L.hl[n_h + popc(hm & lt)] = (unsigned char)k;
+        n_h += popc(hm);
+    }
+    wave_sync();
     const int nper = nc > 1 ? nc - 1 : 1;
-    for (int base = 0; base < nc * nper; base += 64) {
+    for (int base = 0; base < n_h * nper; base += 64) {
         const int p = base + lane;
-        if (p < nc * nper) {
-            const int k = p / nper, mi = p % nper;
+        if (p < n_h * nper) {
+            const int k = L.hl[p / nper], mi = p % nper;
             const int h = L.hint[k];
-            if (h != 0xFF && L.skip[k]) {
+            if (L.skip[k]) {
                 const Cap ck = cap_at(L, k), chint = cap_at(L, h);
                 const double ct = dot(ck.axis, chint.axis);
                 const double st = norm(cross(ck.axis, chint.axis));
@@ -308,11 +340,20 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             const int kl = L.tk[x];
             const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
             const double ax = L.ta[x];
-            int r = 0;
-            for (int y = s0; y < s1; ++y) {
-                const double ay = L.ta[y];
-                r += (ay < ax || (ay == ax && y < x)) ? 1 : 0;
+            int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
+            int y = s0;
+            for (; y + 4 <= s1; y += 4) {
+                const double y0 = L.ta[y], y1 = L.ta[y + 1], y2 = L.ta[y + 2], y3 = L.ta[y + 3];
+                r0 += (y0 < ax || (y0 == ax && y < x)) ? 1 : 0;
+                r1 += (y1 < ax || (y1 == ax && y + 1 < x)) ? 1 : 0;
+                r2 += (y2 < ax || (y2 == ax && y + 2 < x)) ? 1 : 0;
+                r3 += (y3 < ax || (y3 == ax && y + 3 < x)) ? 1 : 0;
             }
+            for (; y < s1; ++y) {
+                const double ay = L.ta[y];
+                r0 += (ay < ax || (ay == ax && y < x)) ? 1 : 0;
+            }
+            const int r = r0 + r1 + r2 + r3;
             if (s0 + r < s1) L.perm[s0 + r] = (unsigned char)x;
         }
         wave_sync();
@@ -370,9 +411,8 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
                 const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
                 if (L.tq[x] == 2) {
                     // whole circle: a boundary loop unless its probe point is buried
-                    bool buried = hint != 0xFF && contains_strict(cap_at(L, hint), apt);
-                    for (int o = 0; o < nc && !buried; ++o)
-                        buried = (o != kk) && contains_strict(cap_at(L, o), apt);
+                    const bool buried = (hint != 0xFF && contains_strict(cap_at(L, hint), apt)) ||
+                                        any_contains(L, nc, kk, apt);
                     emit = !buried;
                     st = apt; en = apt; theta = -two_pi;
                 } else {
@@ -385,10 +425,9 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
                     if (!(ccw < 1e-9)) {
                         const v3 mid = midpoint_on_circle(apt, ck, ccw / 2.0);
                         const int oa = L.tm[x], ob = L.tm[bx];
-                        bool bad = hint != 0xFF && contains_strict(cap_at(L, hint), mid);
-                        bad = bad || contains_strict(cap_at(L, oa), mid) || contains_strict(cap_at(L, ob), mid);
-                        for (int o = 0; o < nc && !bad; ++o)
-                            bad = (o != kk) && contains_strict(cap_at(L, o), mid);
+                        const bool bad = (hint != 0xFF && contains_strict(cap_at(L, hint), mid)) ||
+                                         contains_strict(cap_at(L, oa), mid) ||
+                                         contains_strict(cap_at(L, ob), mid) || any_contains(L, nc, kk, mid);
                         emit = !bad;
                         if (emit) {
                             st = mk(L.tx[bx], L.ty[bx], L.tz[bx]);
