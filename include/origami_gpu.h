@@ -148,6 +148,48 @@ enum og_kernel_class {
 int og_profile_enable(og_engine* e, int32_t on);
 int og_profile_read(og_engine* e, double* ms, int64_t* count);
 
+/* ---- Learned Born-radius surrogate --------------------------------------
+ * No counterpart in origami: it stands in for the descreening pass of
+ * compute_born_radii_into_scratch (crates/energy/src/gb.rs:203-342) and is
+ * trained on the radii that pass produces along the engine's trajectories.
+ * Descriptor per atom: 5 element classes (H, C, N, O, S/P by mass) x 32
+ * radial bins of neighbour counts over 0-12 Å (0.375 Å bins, /8) plus
+ * (rho - 1.5)*4; 161 features.  MLP 161 -> 128 -> 128 -> 1 (SiLU), BF16
+ * operands on MFMA with FP32 accumulation; R = 1.5 + 0.25 y.
+ * Parameters (41345 floats): W1 [192][128] input-major (rows 161..191
+ * zero), b1 [128], W2 [128][128] input-major, b2 [128], w3 [128], b3. */
+
+/* Allocate the surrogate (training set of capacity_rows rows, weights drawn
+ * from a seeded N(0, 1/fan_in)). */
+int og_surrogate_setup(og_engine* e, int64_t capacity_rows, uint64_t seed);
+/* Append one row per atom: descriptor and plain Born radius at the current
+ * positions (OG_MODE_A in flags selects the 20 Å descreening cutoff). */
+int og_surrogate_collect(og_engine* e, uint32_t flags, int64_t* rows_total);
+/* n_steps Adam steps (beta 0.9 / 0.999) on batches sampled from the
+ * collected rows; loss_per_step (optional) receives the mean squared error
+ * in units of (0.25 Å)^2.  batch: multiple of 32, at most 8192. */
+int og_surrogate_train(og_engine* e, int32_t n_steps, int32_t batch, double lr, uint64_t seed,
+                       float* loss_per_step);
+/* Use surrogate radii for every system under OG_SURROGATE.  Every
+ * check_every force evaluations (0 = never) the plain radii are computed as
+ * well and a system whose max |R_sur - R_plain| / R_plain exceeds tolerance
+ * returns to plain GB from the next evaluation on. */
+int og_surrogate_enable(og_engine* e, int32_t check_every, double tolerance);
+/* Surrogate radii of every atom at xyz (or the current positions if NULL). */
+int og_surrogate_radii(og_engine* e, const double* xyz, double* radii);
+/* Descriptors of every atom at xyz (or current), n_atoms x 161 floats. */
+int og_surrogate_descriptors(og_engine* e, const double* xyz, float* x);
+/* Read (get) and/or replace (set) the FP32 parameters. */
+int og_surrogate_params(og_engine* e, float* get, const float* set);
+/* Gradient of the mean squared loss over an explicit batch (x: n x 161,
+ * target radii in Å; n a multiple of 32, at most 8192) with the GPU's own
+ * forward and backward kernels; no update. */
+int og_surrogate_gradient(og_engine* e, const float* x, const double* r_target, int32_t n, float* grad,
+                          double* loss);
+/* Per system: 1 while it uses surrogate radii, and the last check's max
+ * relative deviation from the plain radii. */
+int og_surrogate_status(og_engine* e, int32_t* surrogate_mode, float* deviation);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,6 +19,7 @@
 #include "og_device.h"
 #include "og_kernels.h"
 #include "og_sasa.h"
+#include "og_sur.h"
 
 static thread_local std::string g_err;
 
@@ -77,6 +78,9 @@ struct og_engine {
     double *sys_temp = nullptr, *sys_dofc = nullptr;
     int *shake_fail = nullptr, *diverged = nullptr, *clamped = nullptr;
     og_sasa_data sasa{};
+    og_sur* sur = nullptr;                 // learned Born-radius surrogate (optional)
+    std::vector<double> mass_h;            // host copies kept for the surrogate set-up
+    std::vector<int> sys_tile_off_h;
     // dynamics
     bool dyn_ready = false;
     og_dynamics_params dyn{};
@@ -148,6 +152,11 @@ static void free_all(og_engine* e) {
                     e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    if (e->sur) {
+        og_sur_free(e->sur);
+        delete e->sur;
+        e->sur = nullptr;
+    }
     og_sasa_free(&e->sasa);
     for (auto ev : e->ev_pool) hipEventDestroy(ev);
     for (auto& p : e->ev_pending) {
@@ -429,6 +438,8 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->shake_fail, zi));
     HIPCHK(upload(&e->diverged, zi));
     HIPCHK(upload(&e->clamped, zi));
+    e->mass_h = mass;
+    e->sys_tile_off_h = sys_tile_off;
     int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg, sys_tile_off,
                            e->tile_atom0, e->tile_cnt, e->n_tiles);
     if (rc != OG_OK) return fail(rc, "SASA set-up failed: %s", og_sasa_error());
@@ -501,9 +512,24 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         og_launch_bonded(b, energy, e->stream);
     }
     og_pair_args a = pair_args(e);
+    // Surrogate radii: descriptors + MLP for the systems still in surrogate
+    // mode; their tile pairs skip the descreening pass.  Every check_every
+    // calls the plain pass runs for every system too and systems whose radii
+    // deviate by more than the tolerance go back to plain GB.
+    og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
+    bool check = false;
+    if (sur) {
+        check = sur->check_every > 0 && (sur->calls % sur->check_every) == 0;
+        sur->calls++;
+        ProfScope p(e, OG_K_SURROGATE);
+        if (og_sur_descriptors(sur, false) || og_sur_infer(sur) || (check && og_sur_check_begin(sur)))
+            return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
+    }
     {
         ProfScope p(e, OG_K_DESCREEN);
-        og_launch_pair_descreen(a, mode_a, e->stream);
+        og_pair_args ad = a;
+        ad.skip_sys = (sur && !check) ? sur->mode : nullptr;
+        og_launch_pair_descreen(ad, mode_a, e->stream);
     }
     {
         ProfScope p(e, OG_K_RADII);
@@ -511,7 +537,10 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         r.iacc = e->iacc; r.atom_sys = e->atom_sys; r.atom_slot = e->atom_slot; r.rho = e->rho;
         r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
         r.eacc = e->eacc; r.n_atoms = e->n_atoms;
+        if (sur) { r.sur_mode = sur->mode; r.r_sur = sur->r_sur; r.dev = sur->dev; r.check = check ? 1 : 0; }
         og_launch_radii(r, energy, e->stream);
+        if (sur && check && og_sur_check_apply(sur))
+            return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
     }
     {
         ProfScope p(e, OG_K_FORCE);
@@ -554,7 +583,8 @@ static int read_energies(og_engine* e, double* out) {
 extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
                          double sasa_gamma_scale, double* energies) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
-    if (flags & OG_SURROGATE) return fail(OG_ERR_ARG, "surrogate radii are not enabled in this build");
+    if ((flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
+        return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     int rc;
     if (xyz && (rc = og_set_positions(e, xyz))) return rc;
     const bool energy = (flags & OG_ENERGY) || energies;
@@ -613,7 +643,8 @@ extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, cons
     if (!e || !p || !temps) return fail(OG_ERR_ARG, "NULL argument");
     if (!(p->dt_fs > 0.0) || !(p->friction_ps_inv >= 0.0))
         return fail(OG_ERR_ARG, "dt must be > 0 and friction >= 0");
-    if (p->flags & OG_SURROGATE) return fail(OG_ERR_ARG, "surrogate radii are not enabled in this build");
+    if ((p->flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
+        return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     for (int s = 0; s < e->n_sys; ++s)
         if (!(temps[s] >= 0.0)) return fail(OG_ERR_ARG, "temperature of system %d must be >= 0", s);
     e->dyn = *p;
@@ -747,5 +778,144 @@ extern "C" int og_profile_read(og_engine* e, double* ms, int64_t* count) {
         if (ms) ms[k] = e->prof_ms[k];
         if (count) count[k] = e->prof_cnt[k];
     }
+    return OG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Learned Born-radius surrogate
+// ---------------------------------------------------------------------------
+#define SUR_CHECK(e)                                                                   \
+    do {                                                                               \
+        if (!(e)) return fail(OG_ERR_ARG, "engine is NULL");                           \
+        if (!(e)->sur) return fail(OG_ERR_STATE, "call og_surrogate_setup first");     \
+    } while (0)
+#define SUR_CALL(x)                                                                    \
+    do {                                                                               \
+        if (x) return fail(OG_ERR_HIP, "surrogate: %s", og_sur_error());               \
+    } while (0)
+
+extern "C" int og_surrogate_setup(og_engine* e, int64_t capacity_rows, uint64_t seed) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (capacity_rows < 0) return fail(OG_ERR_ARG, "capacity_rows must be >= 0");
+    if (e->sur) {
+        og_sur_free(e->sur);
+        delete e->sur;
+    }
+    e->sur = new og_sur();
+    if (og_sur_create(e->sur, e->n_atoms, e->n_sys, e->mass_h, e->sys_tile_off_h, e->pos, e->rho,
+                      e->tile_atom0, e->tile_cnt, e->n_tiles, capacity_rows, seed, e->stream)) {
+        og_sur_free(e->sur);
+        delete e->sur;
+        e->sur = nullptr;
+        return fail(OG_ERR_HIP, "surrogate set-up failed: %s", og_sur_error());
+    }
+    return OG_OK;
+}
+
+// plain Born radii (descreening pass + radii) for every system at the current positions
+static int plain_radii(og_engine* e, uint32_t flags) {
+    og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
+    og_pair_args a = pair_args(e);
+    og_launch_pair_descreen(a, (flags & OG_MODE_A) != 0, e->stream);
+    og_radii_args r{};
+    r.iacc = e->iacc; r.atom_sys = e->atom_sys; r.atom_slot = e->atom_slot; r.rho = e->rho;
+    r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
+    r.eacc = e->eacc; r.n_atoms = e->n_atoms;
+    og_launch_radii(r, false, e->stream);
+    HIPCHK(hipGetLastError());
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_collect(og_engine* e, uint32_t flags, int64_t* rows_total) {
+    SUR_CHECK(e);
+    int rc;
+    if ((rc = plain_radii(e, flags))) return rc;
+    SUR_CALL(og_sur_descriptors(e->sur, true));
+    SUR_CALL(og_sur_collect(e->sur, e->radii));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (rows_total) *rows_total = e->sur->n_train;
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_train(og_engine* e, int32_t n_steps, int32_t batch, double lr, uint64_t seed,
+                                  float* loss_per_step) {
+    SUR_CHECK(e);
+    if (n_steps < 0 || !(lr > 0.0)) return fail(OG_ERR_ARG, "n_steps must be >= 0 and lr > 0");
+    for (int t = 0; t < n_steps; ++t) {
+        const uint64_t sd = seed * 0x9E3779B97F4A7C15ull + (uint64_t)t;
+        SUR_CALL(og_sur_train_step(e->sur, batch, lr, sd, loss_per_step ? loss_per_step + t : nullptr));
+    }
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_enable(og_engine* e, int32_t check_every, double tolerance) {
+    SUR_CHECK(e);
+    if (check_every < 0 || !(tolerance >= 0.0)) return fail(OG_ERR_ARG, "check_every >= 0, tolerance >= 0");
+    e->sur->check_every = check_every;
+    e->sur->tol = tolerance;
+    e->sur->calls = 0;
+    e->sur->enabled = true;
+    std::vector<int> on(e->n_sys, 1);
+    HIPCHK(hipMemcpyAsync(e->sur->mode, on.data(), on.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_radii(og_engine* e, const double* xyz, double* radii) {
+    SUR_CHECK(e);
+    int rc;
+    if (xyz && (rc = og_set_positions(e, xyz))) return rc;
+    SUR_CALL(og_sur_descriptors(e->sur, true));
+    SUR_CALL(og_sur_infer(e->sur));
+    if (radii)
+        HIPCHK(hipMemcpyAsync(radii, e->sur->r_sur, (size_t)e->n_atoms * sizeof(double), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_descriptors(og_engine* e, const double* xyz, float* x) {
+    SUR_CHECK(e);
+    if (!x) return fail(OG_ERR_ARG, "x is NULL");
+    int rc;
+    if (xyz && (rc = og_set_positions(e, xyz))) return rc;
+    SUR_CALL(og_sur_descriptors(e->sur, true));
+    std::vector<unsigned short> h((size_t)e->n_atoms * OG_SUR_K1);
+    HIPCHK(hipMemcpyAsync(h.data(), e->sur->X, h.size() * 2, hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < e->n_atoms; ++i)
+        for (int f = 0; f < OG_SUR_F; ++f) {
+            const uint32_t u = (uint32_t)h[(size_t)i * OG_SUR_K1 + f] << 16;
+            float v;
+            std::memcpy(&v, &u, 4);
+            x[(size_t)i * OG_SUR_F + f] = v;
+        }
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_params(og_engine* e, float* get, const float* set) {
+    SUR_CHECK(e);
+    if (set) SUR_CALL(og_sur_set_params(e->sur, set));
+    if (get) SUR_CALL(og_sur_get_params(e->sur, get));
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_gradient(og_engine* e, const float* x, const double* r_target, int32_t n,
+                                     float* grad, double* loss) {
+    SUR_CHECK(e);
+    if (!x || !r_target || !grad) return fail(OG_ERR_ARG, "NULL argument");
+    std::vector<float> t(n > 0 ? n : 1);
+    for (int b = 0; b < n; ++b) t[b] = (float)r_target[b];
+    SUR_CALL(og_sur_gradient(e->sur, x, t.data(), n, grad, loss));
+    return OG_OK;
+}
+
+extern "C" int og_surrogate_status(og_engine* e, int32_t* surrogate_mode, float* deviation) {
+    SUR_CHECK(e);
+    if (surrogate_mode)
+        HIPCHK(hipMemcpyAsync(surrogate_mode, e->sur->mode, (size_t)e->n_sys * sizeof(int), hipMemcpyDeviceToHost, e->stream));
+    if (deviation)
+        HIPCHK(hipMemcpyAsync(deviation, e->sur->dev, (size_t)e->n_sys * sizeof(float), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
     return OG_OK;
 }

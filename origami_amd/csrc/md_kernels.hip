@@ -180,20 +180,32 @@ template <bool ENERGY>
 __global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n_atoms) return;
-    const double rho = a.rho[i];
-    const double rt = rho - OG_OBC_OFFSET;
-    const double integral = (double)a.iacc[i] * OG_FIX_INV;
-    const double psi = integral * rt;
-    const double arg = 1.0 * psi - 0.8 * psi * psi + 4.85 * psi * psi * psi;
-    const double inv = 1.0 / rt - tanh(arg) / rho;
-    const double floor_r = rt > 0.5 ? rt : 0.5;
-    double R;
-    if (inv <= 0.0 || !isfinite(inv)) {
-        atomicAdd(a.clamped + a.atom_sys[i], 1);
-        R = floor_r;
-    } else {
-        R = 1.0 / inv;
-        R = R > floor_r ? R : floor_r;
+    const int sys = a.atom_sys[i];
+    const bool sur = a.sur_mode && a.sur_mode[sys];
+    double R = 0.0;
+    if (!sur || a.check) {
+        const double rho = a.rho[i];
+        const double rt = rho - OG_OBC_OFFSET;
+        const double integral = (double)a.iacc[i] * OG_FIX_INV;
+        const double psi = integral * rt;
+        const double arg = 1.0 * psi - 0.8 * psi * psi + 4.85 * psi * psi * psi;
+        const double inv = 1.0 / rt - tanh(arg) / rho;
+        const double floor_r = rt > 0.5 ? rt : 0.5;
+        if (inv <= 0.0 || !isfinite(inv)) {
+            if (!sur) atomicAdd(a.clamped + sys, 1);
+            R = floor_r;
+        } else {
+            R = 1.0 / inv;
+            R = R > floor_r ? R : floor_r;
+        }
+    }
+    if (sur) {
+        const double Rs = a.r_sur[i];
+        if (a.check) {
+            const float dev = (float)(fabs(Rs - R) / R);
+            atomicMax(a.dev + sys, isfinite(dev) ? __float_as_uint(dev) : 0x7F800000u);
+        }
+        R = Rs;
     }
     a.radii[i] = R;
     float4 g = a.gb[a.atom_slot[i]];
@@ -204,7 +216,7 @@ __global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
         const double q = a.charge[i];
         if (q != 0.0) {
             const double pref = -0.5 * (1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
-            og_atomic_add_fix(a.eacc + a.atom_sys[i] * OG_NE + OG_E_GB_SELF, og_to_fix(pref * q * q / R));
+            og_atomic_add_fix(a.eacc + sys * OG_NE + OG_E_GB_SELF, og_to_fix(pref * q * q / R));
         }
     }
 }

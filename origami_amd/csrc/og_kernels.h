@@ -21,6 +21,7 @@ struct og_pair_args {
     long long* eacc;                 // [S*OG_NE] energy accumulators
     float rc2_nb, rc2_gb, rc2_ds;    // squared cutoffs (Mode A)
     float inv_rc3, kappa, crf;       // reaction-field constants (Mode A)
+    const int* skip_sys;             // [S] descreening skipped where nonzero (may be null)
 };
 
 // energy slots per system
@@ -52,6 +53,10 @@ struct og_radii_args {
     int* clamped;                    // [S]
     long long* eacc;
     int n_atoms;
+    const int* sur_mode;             // [S] nonzero: take R from r_sur (may be null)
+    const double* r_sur;             // [N] surrogate radii
+    unsigned* dev;                   // [S] max |R_sur - R_plain| / R_plain (float bits), when check
+    int check;                       // plain radii were computed for every system this call
 };
 
 struct og_cluster_args {

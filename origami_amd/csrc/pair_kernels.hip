@@ -105,6 +105,7 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
     const int n_waves = (gridDim.x * blockDim.x) >> 6;
     for (int w = wave; w < a.n_work; w += n_waves) {
         const og_work it = a.work[w];
+        if (a.skip_sys && a.skip_sys[it.sys]) continue;     // radii from the surrogate
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],

@@ -115,12 +115,23 @@ OG_HD int intersect_count(const Cap& c1, const Cap& c2) {
     return fabs(c_sq) <= 1e-9 ? 1 : 2;
 }
 
-OG_HD double angular_position(v3 pt, const Cap& c) {
+// In-plane basis of a circle (arrangement.rs:233-248): u from the helper
+// axis, v = axis x u.
+OG_HD inline void circle_basis(const Cap& c, v3& u, v3& v) {
     const v3 h = helper_axis(c.axis);
-    const v3 u = normalize(h - c.axis * dot(h, c.axis));
-    const v3 v = cross(c.axis, u);
+    u = normalize(h - c.axis * dot(h, c.axis));
+    v = cross(c.axis, u);
+}
+
+OG_HD inline double angular_position_b(v3 pt, const Cap& c, v3 u, v3 v) {
     const v3 proj = pt - c.axis * dot(pt, c.axis);
     return atan2(dot(proj, v), dot(proj, u));
+}
+
+OG_HD double angular_position(v3 pt, const Cap& c) {
+    v3 u, v;
+    circle_basis(c, u, v);
+    return angular_position_b(pt, c, u, v);
 }
 
 OG_HD v3 sample_circle_point(const Cap& c) {

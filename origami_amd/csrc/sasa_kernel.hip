@@ -312,12 +312,15 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             const int off = T + popc(b1 & lt) + popc(b2 & lt);
             if (first && off <= SA_TMAX) L.seg[kl] = (unsigned short)off;
             if (cnt >= 1 && off < SA_TMAX) {
-                L.ta[off] = probe ? 0.0 : angular_position(P, cap_at(L, k));
+                const Cap ck = cap_at(L, k);
+                v3 bu, bv;
+                circle_basis(ck, bu, bv);
+                L.ta[off] = probe ? 0.0 : angular_position_b(P, ck, bu, bv);
                 L.tx[off] = P.x; L.ty[off] = P.y; L.tz[off] = P.z;
                 L.tk[off] = (unsigned char)kl; L.tm[off] = (unsigned char)m;
                 L.tq[off] = (unsigned char)(probe ? 2 : 0);
                 if (cnt == 2 && off + 1 < SA_TMAX) {
-                    L.ta[off + 1] = angular_position(Q, cap_at(L, k));
+                    L.ta[off + 1] = angular_position_b(Q, ck, bu, bv);
                     L.tx[off + 1] = Q.x; L.ty[off + 1] = Q.y; L.tz[off + 1] = Q.z;
                     L.tk[off + 1] = (unsigned char)kl; L.tm[off + 1] = (unsigned char)m; L.tq[off + 1] = 1;
                 }

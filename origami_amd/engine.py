@@ -37,6 +37,9 @@ ENERGY_TERMS = ["bond", "angle", "dihedral", "improper", "lj", "coulomb", "gb_se
 KERNEL_CLASSES = ["cluster", "bonded", "descreen", "radii", "force", "sasa", "back", "surrogate",
                   "other"]
 
+SURROGATE_FEATURES = 161
+SURROGATE_PARAMS = 41345
+
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)
 
@@ -93,6 +96,16 @@ def lib():
         L.og_get_status.argtypes = [vp, _ip, _ip, _ip]
         L.og_profile_enable.argtypes = [vp, C.c_int32]
         L.og_profile_read.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
+        _fp = C.POINTER(C.c_float)
+        L.og_surrogate_setup.argtypes = [vp, C.c_int64, C.c_uint64]
+        L.og_surrogate_collect.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int64)]
+        L.og_surrogate_train.argtypes = [vp, C.c_int32, C.c_int32, C.c_double, C.c_uint64, _fp]
+        L.og_surrogate_enable.argtypes = [vp, C.c_int32, C.c_double]
+        L.og_surrogate_radii.argtypes = [vp, _dp, _dp]
+        L.og_surrogate_descriptors.argtypes = [vp, _dp, _fp]
+        L.og_surrogate_params.argtypes = [vp, _fp, _fp]
+        L.og_surrogate_gradient.argtypes = [vp, _fp, _dp, C.c_int32, _fp, _dp]
+        L.og_surrogate_status.argtypes = [vp, _ip, _fp]
         _lib = L
     return _lib
 
@@ -206,8 +219,9 @@ class Engine:
 
     # -- forces / energies ----------------------------------------------------
     def total_force(self, xyz=None, mode_a=False, include_sasa=False, sasa_gamma_scale=1.0,
-                    energies=False):
-        flags = (MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
+                    energies=False, surrogate=False):
+        flags = ((MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
+                 | (SURROGATE if surrogate else 0))
         f = np.zeros((self.n_atoms, 3))
         e = np.zeros((self.n_systems, len(ENERGY_TERMS))) if energies else None
         x = None
@@ -219,13 +233,13 @@ class Engine:
             return f, [dict(zip(ENERGY_TERMS, row)) for row in e]
         return f
 
-    def total_energy(self, xyz=None, mode_a=False):
+    def total_energy(self, xyz=None, mode_a=False, surrogate=False):
         e = np.zeros((self.n_systems, len(ENERGY_TERMS)))
         x = None
         if xyz is not None:
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
         _check(lib().og_energies(self._h, _d(x) if x is not None else None,
-                                 MODE_A if mode_a else 0, _d(e)))
+                                 (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0), _d(e)))
         out = []
         for row in e:
             dct = dict(zip(ENERGY_TERMS, row.tolist()))
@@ -244,13 +258,14 @@ class Engine:
     # -- dynamics ---------------------------------------------------------------
     def dynamics_setup(self, temperatures, dt_fs=1.0, friction_ps_inv=1.0, seed=0,
                        constrain_h_bonds=False, mode_a=False, include_sasa=False,
-                       sasa_gamma_scale=1.0):
+                       sasa_gamma_scale=1.0, surrogate=False):
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperatures, dtype=np.float64),
                                                  (self.n_systems,)))
         self.temperatures = t.copy()
         p = DynamicsParams(dt_fs, friction_ps_inv, seed & ((1 << 64) - 1),
                            1 if constrain_h_bonds else 0,
-                           (MODE_A if mode_a else 0) | (SASA if include_sasa else 0),
+                           (MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
+                           | (SURROGATE if surrogate else 0),
                            sasa_gamma_scale)
         self.dt_fs = dt_fs
         _check(lib().og_dynamics_setup(self._h, C.byref(p), _d(t)))
@@ -275,6 +290,64 @@ class Engine:
         div, fails, cl = (np.zeros(n, np.int32) for _ in range(3))
         _check(lib().og_get_status(self._h, _i(div), _i(fails), _i(cl)))
         return {"diverged": div, "shake_failures": fails, "clamped": cl}
+
+    # -- learned Born-radius surrogate -------------------------------------------
+    def surrogate_setup(self, capacity_rows, seed=0):
+        _check(lib().og_surrogate_setup(self._h, int(capacity_rows), int(seed) & ((1 << 64) - 1)))
+
+    def surrogate_collect(self, mode_a=False):
+        n = C.c_int64(0)
+        _check(lib().og_surrogate_collect(self._h, MODE_A if mode_a else 0, C.byref(n)))
+        return int(n.value)
+
+    def surrogate_train(self, n_steps, batch=4096, lr=1e-3, seed=0):
+        loss = np.zeros(max(int(n_steps), 1), np.float32)
+        _check(lib().og_surrogate_train(self._h, int(n_steps), int(batch), float(lr),
+                                        int(seed) & ((1 << 64) - 1),
+                                        loss.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss[:int(n_steps)]
+
+    def surrogate_enable(self, check_every=100, tolerance=0.02):
+        _check(lib().og_surrogate_enable(self._h, int(check_every), float(tolerance)))
+
+    def surrogate_radii(self, xyz=None):
+        r = np.zeros(self.n_atoms)
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_surrogate_radii(self._h, _d(x) if x is not None else None, _d(r)))
+        return r
+
+    def surrogate_descriptors(self, xyz=None):
+        out = np.zeros((self.n_atoms, SURROGATE_FEATURES), np.float32)
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
+        _check(lib().og_surrogate_descriptors(self._h, _d(x) if x is not None else None,
+                                              out.ctypes.data_as(C.POINTER(C.c_float))))
+        return out
+
+    def surrogate_params(self, new=None):
+        got = np.zeros(SURROGATE_PARAMS, np.float32)
+        st = None
+        if new is not None:
+            st = np.ascontiguousarray(new, dtype=np.float32)
+            assert st.shape == (SURROGATE_PARAMS,)
+        _check(lib().og_surrogate_params(self._h, got.ctypes.data_as(C.POINTER(C.c_float)),
+                                         st.ctypes.data_as(C.POINTER(C.c_float)) if st is not None else None))
+        return got
+
+    def surrogate_gradient(self, x, r_target):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(r_target, dtype=np.float64)
+        g = np.zeros(SURROGATE_PARAMS, np.float32)
+        loss = C.c_double(0.0)
+        _check(lib().og_surrogate_gradient(self._h, x.ctypes.data_as(C.POINTER(C.c_float)), _d(t),
+                                           int(len(t)), g.ctypes.data_as(C.POINTER(C.c_float)),
+                                           C.byref(loss)))
+        return g, float(loss.value)
+
+    def surrogate_status(self):
+        mode = np.zeros(self.n_systems, np.int32)
+        dev = np.zeros(self.n_systems, np.float32)
+        _check(lib().og_surrogate_status(self._h, _i(mode), dev.ctypes.data_as(C.POINTER(C.c_float))))
+        return mode, dev
 
     # -- profiling --------------------------------------------------------------
     def profile(self, on=True):
