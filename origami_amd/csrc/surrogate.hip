@@ -94,30 +94,48 @@ struct DescArgs {
 
 }  // namespace
 
-// Descriptor: one wave per 64-atom tile; lane = atom i; every tile J of the
-// same system whose bounding sphere comes within 12 Å contributes its atoms j
-// (j != i, r < 12 Å) to i's histogram row in LDS, bin = class_j * 32 +
-// floor(r / 0.375 Å) decided exactly on r^2 against the bin edges
-// (0.375 k)^2 = 0.140625 k^2, which are exact in FP64.
-__global__ void __launch_bounds__(64) og_k_sur_desc(DescArgs a) {
+// Descriptor: one 256-thread block per 64-atom tile I.  The block lists the
+// tiles J of the same system whose bounding spheres come within 12 Å of
+// tile I, then its four waves take those tiles in turn: lane = atom i, the
+// atoms j of tile J are broadcast one by one, and every j != i with r < 12 Å
+// adds one count to i's histogram row in LDS (integer counts, so the result
+// does not depend on the order).  The bin floor(r / 0.375 Å) is estimated
+// from a single-precision square root and then settled exactly on r^2
+// against the bin edges (0.375 k)^2 = 0.140625 k^2, which are exact in FP64.
+#define SUR_DESC_WAVES 4
+__global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a) {
     __shared__ unsigned hist[64 * OG_SUR_F];          // stride 161: conflict-free rows
+    __shared__ int near_list[1024];
+    __shared__ int n_near;
     const int t = blockIdx.x;
-    const int lane = threadIdx.x;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = tid >> 6;
     const int sys = a.tile_sys[t];
     if (a.mode && !a.mode[sys]) return;
-    unsigned* row = hist + lane * OG_SUR_F;
-    for (int f = 0; f < 160; ++f) row[f] = 0u;
+    for (int q = tid; q < 64 * OG_SUR_F; q += blockDim.x) hist[q] = 0u;
+    if (tid == 0) n_near = 0;
+    __syncthreads();
+    const double4 bi = a.tile_bound[t];
+    const int jb = a.sys_tile_b[sys], je = a.sys_tile_e[sys];
+    for (int J = jb + tid; J < je; J += blockDim.x) {
+        const double4 bj = a.tile_bound[J];
+        const double cx = bj.x - bi.x, cy = bj.y - bi.y, cz = bj.z - bi.z;
+        const double reach = bi.w + bj.w + 12.0;
+        if (cx * cx + cy * cy + cz * cz <= reach * reach) {
+            const int slot = atomicAdd(&n_near, 1);
+            if (slot < 1024) near_list[slot] = J;
+        }
+    }
+    __syncthreads();
+    const int nn = n_near < 1024 ? n_near : 1024;
     const int a0 = a.tile_atom0[t], cnt = a.tile_cnt[t];
     const bool vi = lane < cnt;
     const int i = a0 + lane;
     double xi = 0.0, yi = 0.0, zi = 0.0;
     if (vi) { xi = a.pos[3 * (size_t)i]; yi = a.pos[3 * (size_t)i + 1]; zi = a.pos[3 * (size_t)i + 2]; }
-    const double4 bi = a.tile_bound[t];
-    for (int J = a.sys_tile_b[sys]; J < a.sys_tile_e[sys]; ++J) {
-        const double4 bj = a.tile_bound[J];
-        const double cx = bj.x - bi.x, cy = bj.y - bi.y, cz = bj.z - bi.z;
-        const double reach = bi.w + bj.w + 12.0;
-        if (cx * cx + cy * cy + cz * cz > reach * reach) continue;
+    unsigned* row = hist + lane * OG_SUR_F;
+    for (int q = wv; q < nn; q += SUR_DESC_WAVES) {
+        const int J = near_list[q];
         const int b0 = a.tile_atom0[J], cj = a.tile_cnt[J];
         double xj = 0.0, yj = 0.0, zj = 0.0;
         int cl = 0;
@@ -131,19 +149,23 @@ __global__ void __launch_bounds__(64) og_k_sur_desc(DescArgs a) {
             const int c = __builtin_amdgcn_readlane(cl, jj);
             const double r2 = dx * dx + dy * dy + dz * dz;
             if (vi && b0 + jj != i && r2 < 144.0) {
-                int k = (int)sqrt(r2 / 0.140625);
-                k = k > 31 ? 31 : k;
+                int k = (int)(sqrtf((float)r2) * (1.0f / 0.375f));
+                k = k > 31 ? 31 : (k < 0 ? 0 : k);
                 while (k < 31 && 0.140625 * (double)((k + 1) * (k + 1)) <= r2) ++k;
                 while (k > 0 && 0.140625 * (double)(k * k) > r2) --k;
-                row[c * 32 + k] += 1u;
+                atomicAdd(row + c * 32 + k, 1u);
             }
         }
     }
-    if (!vi) return;
-    unsigned short* xo = a.X + (size_t)i * OG_SUR_K1;
-    for (int f = 0; f < 160; ++f) xo[f] = f2bf((float)row[f] * 0.125f);
-    xo[160] = f2bf((float)((a.rho[i] - 1.5) * 4.0));
-    for (int f = 161; f < OG_SUR_K1; ++f) xo[f] = 0;
+    __syncthreads();
+    // write the 64 rows: thread per (row, feature)
+    for (int q = tid; q < cnt * OG_SUR_K1; q += blockDim.x) {
+        const int rr = q / OG_SUR_K1, f = q % OG_SUR_K1;
+        unsigned short v = 0;
+        if (f < 160) v = f2bf((float)hist[rr * OG_SUR_F + f] * 0.125f);
+        else if (f == 160) v = f2bf((float)((a.rho[a0 + rr] - 1.5) * 4.0));
+        a.X[(size_t)(a0 + rr) * OG_SUR_K1 + f] = v;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -506,6 +528,7 @@ int og_sur_create(og_sur* s, int n_atoms, int n_sys, const std::vector<double>& 
     for (int q = 0; q < n_sys; ++q) {
         tb[q] = sys_tile_off[q];
         te[q] = sys_tile_off[q + 1];
+        if (te[q] - tb[q] > 1024) return fail("the descriptor kernel handles systems of up to 65536 atoms");
         for (int t = tb[q]; t < te[q]; ++t) tsys[t] = q;
     }
     if (!upload(&s->cls, cls) || !upload(&s->tile_sys, tsys) || !upload(&s->sys_tile_b, tb) ||
@@ -564,7 +587,7 @@ int og_sur_descriptors(og_sur* s, bool all) {
                        s->tile_cnt, s->n_tiles, s->tile_bound);
     DescArgs a{s->pos, s->rho, s->cls, s->tile_atom0, s->tile_cnt, s->tile_sys, s->sys_tile_b, s->sys_tile_e,
                s->tile_bound, all ? nullptr : s->mode, s->X};
-    hipLaunchKernelGGL(og_k_sur_desc, dim3(s->n_tiles), dim3(64), 0, s->stream, a);
+    hipLaunchKernelGGL(og_k_sur_desc, dim3(s->n_tiles), dim3(64 * SUR_DESC_WAVES), 0, s->stream, a);
     return hipGetLastError() == hipSuccess ? 0 : fail("kernel launch failed");
 }
 
