@@ -120,10 +120,11 @@ def surrogate_leg(eng, systems, temps, args, dt_fs, gamma_scale, include_sasa, n
                        constrain_h_bonds=True, include_sasa=include_sasa,
                        sasa_gamma_scale=gamma_scale, surrogate=True)
     eng.run(args.warmup)
-    eng.profile(True)
     t0 = time.perf_counter()
     eng.run(args.steps)
     wall = time.perf_counter() - t0
+    eng.profile(True)
+    eng.run(args.profile_steps)
     prof = eng.profile_read()
     eng.profile(False)
     mode, dev = eng.surrogate_status()
@@ -152,7 +153,7 @@ def surrogate_leg(eng, systems, temps, args, dt_fs, gamma_scale, include_sasa, n
         "ms_per_step": wall * 1e3 / max(args.steps, 1),
         "pair_interactions_per_s": pairs / ((k_ms.get("force", 0) + k_ms.get("surrogate", 0)
                                              + k_ms.get("descreen", 0)) * 1e-3),
-        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in prof.items() if v[1] > 0},
+        "kernel_ms_per_step": {k: v[0] / max(args.profile_steps, 1) for k, v in prof.items() if v[1] > 0},
         "training": {"rows": rows, "adam_steps": args.train_steps, "batch": 8192,
                      "collect_s": t_collect, "train_s": t_train,
                      "loss_first": float(loss[:20].mean()), "loss_last": float(loss[-20:].mean()),
@@ -183,6 +184,9 @@ def main():
     ap.add_argument("--no-surrogate", action="store_true",
                     help="skip the learned-radii leg (training + timed surrogate run)")
     ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--replicas", type=int, default=8, help="config 3: ladder size (8 or 64)")
+    ap.add_argument("--profile-steps", type=int, default=20,
+                    help="steps of the separate profiled run that gives the kernel breakdown")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -207,10 +211,11 @@ def main():
         workload = ("80-residue seeded chain, extended + L-BFGS; all-pairs LJ + Coulomb + GB + "
                     "PowerSasa (gamma scale 0.25); BAOAB dt 2 fs + SHAKE")
     else:
-        systems, temps = build_config3()
+        systems, temps = build_config3(n_rep=args.replicas)
         include_sasa = False
-        workload = ("REMD batch: 8 replicas of a seeded 10-residue chain, 280-395 K geometric "
-                    "ladder; all-pairs GB + LJ + Coulomb; BAOAB dt 2 fs + SHAKE")
+        workload = ("REMD batch: %d replicas of a seeded 10-residue chain, 280-395 K geometric "
+                    "ladder; all-pairs GB + LJ + Coulomb; BAOAB dt 2 fs + SHAKE; one captured "
+                    "graph per step for all replicas" % args.replicas)
     log("built %d systems in %.1f s" % (len(systems), time.time() - t0))
     eng = E.Engine(systems)
     info = eng.info()
@@ -227,11 +232,16 @@ def main():
     eng.dynamics_setup(temps, dt_fs=dt_fs, friction_ps_inv=2.0, seed=2024,
                        constrain_h_bonds=True, include_sasa=include_sasa,
                        sasa_gamma_scale=gamma_scale)
+    tune = eng.autotune(include_sasa=include_sasa)
+    log("autotune:", tune)
     eng.run(args.warmup)
-    eng.profile(True)
+    # timed run: captured-graph replay, no per-kernel events
     t0 = time.perf_counter()
     eng.run(args.steps)
     wall = time.perf_counter() - t0
+    # kernel breakdown from a separate run with HIP events around every launch
+    eng.profile(True)
+    eng.run(args.profile_steps)
     prof = eng.profile_read()
     eng.profile(False)
     st = eng.status()
@@ -252,7 +262,7 @@ def main():
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
             "traffic": None, "flops_per_pair": FLOPS_FORCE, "pairs_per_launch": pairs,
             "avg_ms": k_ms["force"]}
-    dominant = {"kernel_class": dom, "ms_per_step": totals[dom] / max(args.steps, 1),
+    dominant = {"kernel_class": dom, "ms_per_step": totals[dom] / max(args.profile_steps, 1),
                 "bound": "latency / divergence (FP64 PowerSasa arrangement)" if dom == "sasa" else None}
     # FP32 pair kernels beside it
     pk = {}
@@ -272,7 +282,10 @@ def main():
                    "dt_fs": dt_fs, "sasa_gamma_scale": gamma_scale if include_sasa else 0.0,
                    "surrogate": False},
         "pair_interactions_per_s": pair_ips,
-        "kernel_ms_per_step": {k: v[0] / max(args.steps, 1) for k, v in prof.items() if v[1] > 0},
+        "kernel_ms_per_step": {k: v[0] / max(args.profile_steps, 1) for k, v in prof.items() if v[1] > 0},
+        "kernel_timing": "HIP events on the engine stream over a separate %d-step run with direct "
+                         "launches (the timed run replays captured graphs)" % args.profile_steps,
+        "autotune": tune,
         "pair_kernels": pk,
         "roofline": roof,
         "dominant_kernel": dominant,

@@ -127,6 +127,20 @@ int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, const double* t
  * system diverged; diverged systems stop moving. */
 int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 
+/* og_run replays one captured hipGraph per BAOAB step (all systems, every
+ * launch of the step) when profiling is off and no per-step kinetic energy
+ * is requested; on = 0 forces direct launches.  No counterpart in origami
+ * (its step loop is langevin.rs:195-367, run per replica by remd.rs:286). */
+int og_use_graphs(og_engine* e, int32_t on);
+
+/* Autotuner (no counterpart in origami): times the candidate launch widths
+ * of the all-pairs kernels (2..32 blocks of 256 threads per CU) and of the
+ * SASA arrangement kernel (2..8 blocks of 2 waves per CU) at the current
+ * positions and keeps the fastest.  choice[2] (optional) receives the pair
+ * and SASA block counts, best_ms[2] their times.  Results do not change:
+ * accumulation is order-independent fixed point. */
+int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms);
+
 /* Set the bath temperature of one system (REMD ladder assignment). */
 int og_set_temperature(og_engine* e, int32_t system, double temperature_k);
 

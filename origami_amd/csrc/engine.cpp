@@ -79,6 +79,11 @@ struct og_engine {
     int *shake_fail = nullptr, *diverged = nullptr, *clamped = nullptr;
     og_sasa_data sasa{};
     og_sur* sur = nullptr;                 // learned Born-radius surrogate (optional)
+    uint32_t* step_dev = nullptr;          // BAOAB step counter (Philox counter word)
+    bool use_graphs = true;                // replay captured steps in og_run
+    bool graph_ok = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
+    int pair_grid = 0;                     // autotuned pair-kernel width (0: default)
     std::vector<double> mass_h;            // host copies kept for the surrogate set-up
     std::vector<int> sys_tile_off_h;
     // dynamics
@@ -152,6 +157,9 @@ static void free_all(og_engine* e) {
                     e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
     for (void* p : ptrs)
         if (p) hipFree(p);
+    for (auto& g : e->gexec)
+        if (g) (void)hipGraphExecDestroy(g);
+    if (e->step_dev) (void)hipFree(e->step_dev);
     if (e->sur) {
         og_sur_free(e->sur);
         delete e->sur;
@@ -437,6 +445,8 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<int> zi(n_sys, 0);
     HIPCHK(upload(&e->shake_fail, zi));
     HIPCHK(upload(&e->diverged, zi));
+    HIPCHK(hipMalloc((void**)&e->step_dev, sizeof(uint32_t)));
+    HIPCHK(hipMemset(e->step_dev, 0, sizeof(uint32_t)));
     HIPCHK(upload(&e->clamped, zi));
     e->mass_h = mass;
     e->sys_tile_off_h = sys_tile_off;
@@ -490,6 +500,7 @@ static og_pair_args pair_args(og_engine* e) {
     a.tile_atom0 = e->tile_atom0; a.tile_cnt = e->tile_cnt;
     a.work = e->work; a.n_work = e->n_work; a.masks = e->masks;
     a.facc = e->facc; a.iacc = e->iacc; a.eacc = e->eacc;
+    a.grid = e->pair_grid;
     const double rc = 10.0;
     a.rc2_nb = (float)(rc * rc);
     a.rc2_gb = (float)(rc * rc);
@@ -500,7 +511,16 @@ static og_pair_args pair_args(og_engine* e) {
     return a;
 }
 
-static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy) {
+static void invalidate_graphs(og_engine* e) {
+    for (auto& g : e->gexec) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    e->graph_ok = false;
+}
+
+// sur_check: -1 = decide from the call counter; 0 / 1 = forced (graph capture)
+static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1) {
     const bool mode_a = flags & OG_MODE_A;
     {
         ProfScope p(e, OG_K_BONDED);
@@ -519,8 +539,12 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
     og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
     bool check = false;
     if (sur) {
-        check = sur->check_every > 0 && (sur->calls % sur->check_every) == 0;
-        sur->calls++;
+        if (sur_check < 0) {
+            check = sur->check_every > 0 && (sur->calls % sur->check_every) == 0;
+            sur->calls++;
+        } else {
+            check = sur_check > 0;
+        }
         ProfScope p(e, OG_K_SURROGATE);
         if (og_sur_descriptors(sur, false) || og_sur_infer(sur) || (check && og_sur_check_begin(sur)))
             return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
@@ -661,6 +685,7 @@ extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, cons
     HIPCHK(hipMemsetAsync(e->diverged, 0, e->n_sys * sizeof(int), e->stream));
     e->step = 0;
     e->dyn_ready = true;
+    invalidate_graphs(e);
     // initial forces at the current positions
     return og_forces(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->sasa_gamma_scale, nullptr);
 }
@@ -670,6 +695,38 @@ extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
     e->temps[s] = t;
     HIPCHK(hipMemcpyAsync(e->sys_temp + s, &e->temps[s], sizeof(double), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+static int record_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check) {
+    const og_dynamics_params& p = e->dyn;
+    {
+        ProfScope ps(e, OG_K_CLUSTER);
+        og_launch_cluster_step(c, e->stream);
+    }
+    int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false, sur_check);
+    if (rc) return rc;
+    {
+        ProfScope ps(e, OG_K_BACK);
+        og_launch_back(b, e->stream);
+    }
+    return OG_OK;
+}
+
+static int capture_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check,
+                        hipGraphExec_t* out) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = record_step(e, c, b, sur_check);
+    const hipError_t er = hipStreamEndCapture(e->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (er != hipSuccess) return fail(OG_ERR_HIP, "graph capture failed: %s", hipGetErrorString(er));
+    const hipError_t ei = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) return fail(OG_ERR_HIP, "graph instantiation failed: %s", hipGetErrorString(ei));
     return OG_OK;
 }
 
@@ -693,32 +750,49 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     c.use_shake = p.constrain_h_bonds ? 1 : 0;
     c.key0 = (uint32_t)(p.seed & 0xffffffffu);
     c.key1 = (uint32_t)(p.seed >> 32);
+    c.step_dev = e->step_dev;
     og_back_args b{};
     b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
     b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = p.dt_fs; b.do_kick = 1;
     b.diverged = e->diverged;
-    for (int k = 0; k < n_steps; ++k) {
-        c.step = e->step;
-        {
-            ProfScope ps(e, OG_K_CLUSTER);
-            og_launch_cluster_step(c, e->stream);
+    b.step_dev = e->step_dev;
+    // the device step counter restarts from the host's count each call
+    og_launch_set_u32(e->step_dev, e->step, e->stream);
+    og_sur* sur = (p.flags & OG_SURROGATE) ? e->sur : nullptr;
+    // Captured steps: every launch of one BAOAB step (cluster B-A-SHAKE-O-A,
+    // forces, final kick) is recorded once into a hipGraph and replayed; the
+    // step counter lives in device memory so one graph serves every step.
+    const bool graphs = e->use_graphs && !e->prof && !ke_dev && n_steps >= 2;
+    if (graphs) {
+        if (!e->graph_ok) {
+            invalidate_graphs(e);
+            if (int rc = capture_step(e, c, b, 0, &e->gexec[0])) return rc;
+            if (sur && sur->check_every > 0)
+                if (int rc = capture_step(e, c, b, 1, &e->gexec[1])) return rc;
+            e->graph_ok = true;
         }
-        int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false);
-        if (rc) {
-            if (ke_dev) hipFreeAsync(ke_dev, e->stream);
-            return rc;
+        for (int k = 0; k < n_steps; ++k) {
+            int chk = 0;
+            if (sur) {
+                chk = (sur->check_every > 0 && (sur->calls % sur->check_every) == 0) ? 1 : 0;
+                sur->calls++;
+            }
+            HIPCHK(hipGraphLaunch(e->gexec[chk], e->stream));
         }
-        b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
-        {
-            ProfScope ps(e, OG_K_BACK);
-            og_launch_back(b, e->stream);
-        }
-        e->step++;
-        if (e->prof && (k & 63) == 63) {
-            HIPCHK(hipStreamSynchronize(e->stream));
-            prof_collect(e);
+    } else {
+        for (int k = 0; k < n_steps; ++k) {
+            b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
+            if (int rc = record_step(e, c, b, -1)) {
+                if (ke_dev) (void)hipFreeAsync(ke_dev, e->stream);
+                return rc;
+            }
+            if (e->prof && (k & 63) == 63) {
+                HIPCHK(hipStreamSynchronize(e->stream));
+                prof_collect(e);
+            }
         }
     }
+    e->step += (uint32_t)n_steps;
     HIPCHK(hipGetLastError());
     if (ke_dev) {
         std::vector<long long> h((size_t)n_steps * e->n_sys);
@@ -734,6 +808,73 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     HIPCHK(hipMemcpy(div.data(), e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
     for (int s = 0; s < e->n_sys; ++s)
         if (div[s]) return fail(OG_ERR_NONFINITE, "system %d diverged (non-finite velocities)", s);
+    return OG_OK;
+}
+
+extern "C" int og_use_graphs(og_engine* e, int32_t on) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    e->use_graphs = on != 0;
+    invalidate_graphs(e);
+    return OG_OK;
+}
+
+// Autotuner: times the candidate launch widths of the all-pairs kernels and
+// of the SASA arrangement kernel on the engine's current positions and keeps
+// the fastest.  Candidates: pair kernels 2..32 blocks of 256 threads per CU;
+// SASA 2..8 blocks of 2 waves per CU.
+extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    int dev = 0, n_cu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+        n_cu = prop.multiProcessorCount;
+    hipEvent_t t0, t1;
+    HIPCHK(hipEventCreate(&t0));
+    HIPCHK(hipEventCreate(&t1));
+    auto timed = [&](auto&& launch, float& ms) -> int {
+        launch();                                   // warm
+        HIPCHK(hipEventRecord(t0, e->stream));
+        for (int r = 0; r < 3; ++r) launch();
+        HIPCHK(hipEventRecord(t1, e->stream));
+        HIPCHK(hipEventSynchronize(t1));
+        HIPCHK(hipEventElapsedTime(&ms, t0, t1));
+        ms /= 3.0f;
+        return OG_OK;
+    };
+    const bool mode_a = flags & OG_MODE_A;
+    int best_grid = 0;
+    float best_pair = 1e30f;
+    for (int per_cu : {2, 4, 8, 16, 32}) {
+        const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
+        og_pair_args a = pair_args(e);
+        a.grid = g;
+        float ms = 0.0f;
+        if (int rc = timed([&] { og_launch_pair_force(a, mode_a, e->stream); }, ms)) return rc;
+        if (ms < best_pair) { best_pair = ms; best_grid = g; }
+        if (g == std::max((e->n_work + 3) / 4, 1)) break;
+    }
+    e->pair_grid = best_grid;
+    int best_sasa = og_sasa_max_blocks(&e->sasa);
+    float best_s = 0.0f;
+    if ((flags & OG_SASA) && e->n_sasa_work > 0) {
+        best_s = 1e30f;
+        for (int per_cu : {2, 4, 8}) {
+            const int g = std::min(per_cu * n_cu, og_sasa_max_blocks(&e->sasa));
+            og_sasa_set_blocks(&e->sasa, g);
+            float ms = 0.0f;
+            if (int rc = timed([&] { og_sasa_launch(&e->sasa, e->pos, e->facc, 1.0, e->stream); }, ms)) return rc;
+            if (ms < best_s) { best_s = ms; best_sasa = g; }
+        }
+        og_sasa_set_blocks(&e->sasa, best_sasa);
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
+    HIPCHK(hipStreamSynchronize(e->stream));
+    invalidate_graphs(e);
+    if (choice) { choice[0] = best_grid; choice[1] = best_sasa; }
+    if (best_ms) { best_ms[0] = best_pair; best_ms[1] = best_s; }
     return OG_OK;
 }
 
@@ -801,6 +942,7 @@ extern "C" int og_surrogate_setup(og_engine* e, int64_t capacity_rows, uint64_t 
         og_sur_free(e->sur);
         delete e->sur;
     }
+    invalidate_graphs(e);
     e->sur = new og_sur();
     if (og_sur_create(e->sur, e->n_atoms, e->n_sys, e->mass_h, e->sys_tile_off_h, e->pos, e->rho,
                       e->tile_atom0, e->tile_cnt, e->n_tiles, capacity_rows, seed, e->stream)) {
@@ -856,6 +998,7 @@ extern "C" int og_surrogate_enable(og_engine* e, int32_t check_every, double tol
     e->sur->tol = tolerance;
     e->sur->calls = 0;
     e->sur->enabled = true;
+    invalidate_graphs(e);
     std::vector<int> on(e->n_sys, 1);
     HIPCHK(hipMemcpyAsync(e->sur->mode, on.data(), on.size() * sizeof(int), hipMemcpyHostToDevice, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));

@@ -353,7 +353,8 @@ __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a) {
                 const int g = at[t];
                 const double sigma = sqrt(om * kbt * dofc * OG_ACCEL / a.mass[g]);
                 double g0, g1, g2;
-                noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], a.step, g0, g1, g2);
+                noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], a.step_dev ? *a.step_dev : a.step,
+                       g0, g1, g2);
                 vs[3 * t] = a.alpha * vs[3 * t] + sigma * g0;
                 vs[3 * t + 1] = a.alpha * vs[3 * t + 1] + sigma * g1;
                 vs[3 * t + 2] = a.alpha * vs[3 * t + 2] + sigma * g2;
@@ -402,6 +403,7 @@ void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream
 __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n_atoms) return;
+    if (i == 0 && a.do_kick && a.step_dev) *a.step_dev += 1u;
     double f[3];
     for (int d = 0; d < 3; ++d) {
         f[d] = (double)a.facc[3 * (size_t)i + d] * OG_FIX_INV;
@@ -422,6 +424,14 @@ __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
     }
     if (!finite) atomicMax(a.diverged + sys, 1);
     if (a.ke_acc) og_atomic_add_fix(a.ke_acc + sys, og_to_fix(0.5 * m * v2 / OG_ACCEL));
+}
+
+__global__ void og_k_set_u32(uint32_t* p, uint32_t v) {
+    if (threadIdx.x == 0) *p = v;
+}
+
+void og_launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(og_k_set_u32, dim3(1), dim3(64), 0, s, p, v);
 }
 
 void og_launch_back(const og_back_args& a, hipStream_t s) {

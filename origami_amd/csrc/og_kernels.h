@@ -22,6 +22,7 @@ struct og_pair_args {
     float rc2_nb, rc2_gb, rc2_ds;    // squared cutoffs (Mode A)
     float inv_rc3, kappa, crf;       // reaction-field constants (Mode A)
     const int* skip_sys;             // [S] descreening skipped where nonzero (may be null)
+    int grid;                        // blocks of 256 threads (0: default)
 };
 
 // energy slots per system
@@ -79,6 +80,7 @@ struct og_cluster_args {
     int use_shake;
     uint32_t key0, key1;
     uint32_t step;
+    const uint32_t* step_dev;        // if set, the step counter is read from device memory
 };
 
 struct og_back_args {
@@ -91,6 +93,7 @@ struct og_back_args {
     int do_kick;
     long long* ke_acc;               // [S] or null
     int* diverged;                   // [S]
+    uint32_t* step_dev;              // if set (and do_kick), advanced by one per step
 };
 
 void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s);
@@ -110,5 +113,6 @@ void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s);
 void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s);
 void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream_t s);
 void og_launch_back(const og_back_args& a, hipStream_t s);
+void og_launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 void og_launch_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
                     double scale_a, double scale_b, hipStream_t s);

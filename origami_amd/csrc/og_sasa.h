@@ -51,6 +51,9 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
 void og_sasa_free(og_sasa_data* d);
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
                    hipStream_t s);
+// Arrangement-kernel width (blocks of 2 waves); at most the staging slots allocated.
+int og_sasa_max_blocks(const og_sasa_data* d);
+void og_sasa_set_blocks(og_sasa_data* d, int blocks);
 // Reads and clears the overflow count.
 int og_sasa_overflow(og_sasa_data* d, int* count);
 const char* og_sasa_error();

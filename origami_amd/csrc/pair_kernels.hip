@@ -257,14 +257,14 @@ static int pair_grid(int n_work) {
 
 void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s) {
     if (a.n_work <= 0) return;
-    const int g = pair_grid(a.n_work);
+    const int g = a.grid > 0 ? a.grid : pair_grid(a.n_work);
     if (mode_a) hipLaunchKernelGGL(og_k_descreen_a, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_descreen_b, dim3(g), dim3(256), 0, s, a);
 }
 
 void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
     if (a.n_work <= 0) return;
-    const int g = pair_grid(a.n_work);
+    const int g = a.grid > 0 ? a.grid : pair_grid(a.n_work);
     if (mode_a) hipLaunchKernelGGL(og_k_force_a, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_force_b, dim3(g), dim3(256), 0, s, a);
 }

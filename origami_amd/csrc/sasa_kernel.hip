@@ -803,6 +803,13 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
     return 0;
 }
 
+int og_sasa_max_blocks(const og_sasa_data* d) { return d->stride / SA_WAVES; }
+
+void og_sasa_set_blocks(og_sasa_data* d, int blocks) {
+    const int mx = d->stride / SA_WAVES;
+    d->blocks = blocks < 1 ? 1 : (blocks > mx ? mx : blocks);
+}
+
 int og_sasa_overflow(og_sasa_data* d, int* count) {
     if (hipMemcpy(count, d->overflow, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess) return -2;
     if (*count != 0 && hipMemset(d->overflow, 0, sizeof(int)) != hipSuccess) return -2;

@@ -96,6 +96,8 @@ def lib():
         L.og_get_status.argtypes = [vp, _ip, _ip, _ip]
         L.og_profile_enable.argtypes = [vp, C.c_int32]
         L.og_profile_read.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
+        L.og_use_graphs.argtypes = [vp, C.c_int32]
+        L.og_autotune.argtypes = [vp, C.c_uint32, _ip, _dp]
         _fp = C.POINTER(C.c_float)
         L.og_surrogate_setup.argtypes = [vp, C.c_int64, C.c_uint64]
         L.og_surrogate_collect.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int64)]
@@ -277,6 +279,17 @@ class Engine:
             raise EngineError("diverged: " + lib().og_last_error().decode())
         _check(rc)
         return ke
+
+    def use_graphs(self, on=True):
+        _check(lib().og_use_graphs(self._h, 1 if on else 0))
+
+    def autotune(self, mode_a=False, include_sasa=False):
+        ch = np.zeros(2, np.int32)
+        ms = np.zeros(2)
+        _check(lib().og_autotune(self._h, (MODE_A if mode_a else 0) | (SASA if include_sasa else 0),
+                                 _i(ch), _d(ms)))
+        return {"pair_blocks": int(ch[0]), "sasa_blocks": int(ch[1]), "pair_ms": float(ms[0]),
+                "sasa_ms": float(ms[1])}
 
     def set_temperature(self, system, t):
         _check(lib().og_set_temperature(self._h, int(system), float(t)))

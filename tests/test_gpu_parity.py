@@ -248,3 +248,40 @@ def test_trp_cage_minimise_acceptance(E):
     print(res, e)
     assert res.final_energy < 1000.0
     assert e["bond"] < 200.0
+
+
+def test_graph_replay_matches_eager(E, c1_system, c3_system):
+    """Captured-step replay (og_run with graphs) against direct launches, with
+    eager work and synchronisations between replays, every replay checked."""
+    def make():
+        eng = E.Engine([c1_system, c3_system, c3_system])
+        v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(5 + k))
+                            for k, s in enumerate([c1_system, c3_system, c3_system])])
+        eng.set_velocities(v)
+        eng.dynamics_setup([300.0, 320.0, 340.0], dt_fs=1.0, friction_ps_inv=2.0, seed=77,
+                           constrain_h_bonds=True)
+        return eng
+    eager, graph = make(), make()
+    eager.use_graphs(False)
+    for rep in range(4):
+        eager.run(8)
+        graph.run(8)
+        assert np.array_equal(eager.positions(), graph.positions()), rep
+        assert np.array_equal(eager.velocities(), graph.velocities()), rep
+        fa, fb = eager.total_force(), graph.total_force()          # eager work between replays
+        assert np.array_equal(fa, fb)
+    graph.swap(1, 2)
+    eager.swap(1, 2)
+    eager.run(5)
+    graph.run(5)
+    assert np.array_equal(eager.positions(), graph.positions())
+
+
+def test_autotune_keeps_results(E, c2_system):
+    eng = E.Engine([c2_system])
+    f0 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    ch = eng.autotune(include_sasa=True)
+    print("autotune", ch)
+    f1 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    assert np.array_equal(f0, f1)
+    assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0
