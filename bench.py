@@ -172,6 +172,23 @@ def surrogate_leg(eng, systems, temps, args, dt_fs, gamma_scale, include_sasa, n
     }
 
 
+def pmc_probe(E):
+    """Workload for rocprofv3 --pmc runs (FETCH_SIZE / WRITE_SIZE per dispatch)."""
+    import torch
+    n = 128 * 1024 * 1024                       # 512 MiB of float32
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    systems, _ = build_config4()
+    eng = E.Engine(systems)
+    for _ in range(3):
+        eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    print(json.dumps({"pmc_probe": "copy 536870912 bytes read + written; 3 force evaluations",
+                      "atoms": eng.n_atoms}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +202,9 @@ def main():
                     help="skip the learned-radii leg (training + timed surrogate run)")
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--replicas", type=int, default=8, help="config 3: ladder size (8 or 64)")
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="short workload for hardware-counter runs: a 512 MiB device copy (the "
+                         "calibration kernel) and three force evaluations of the 64-chain batch")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps of the separate profiled run that gives the kernel breakdown")
     args = ap.parse_args()
@@ -196,6 +216,8 @@ def main():
 
     from origami_amd import engine as E
     from origami_amd.minimize import minimize
+    if args.pmc_probe:
+        return pmc_probe(E)
 
     gamma_scale = 0.25
     dt_fs = 2.0
