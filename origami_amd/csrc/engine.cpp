@@ -73,6 +73,8 @@ struct og_engine {
     int nb = 0, na = 0, nd = 0, ni = 0;
     int *bonds = nullptr, *angles = nullptr, *dih = nullptr, *imp = nullptr;
     double *bond_p = nullptr, *angle_p = nullptr, *dih_p = nullptr, *imp_p = nullptr;
+    double* bonded_tf = nullptr;                    // per-(term, atom) force slots
+    int *bonded_csr_start = nullptr, *bonded_csr_slot = nullptr;
     int *cl_start = nullptr, *cl_atoms = nullptr, *cn_start = nullptr, *cn_pair = nullptr;
     double* cn_len = nullptr;
     double *sys_temp = nullptr, *sys_dofc = nullptr;
@@ -153,6 +155,7 @@ static void free_all(og_engine* e) {
                     e->rmin14, e->atom_sys, e->atom_local,
                     e->atom_slot, e->par, e->gb, e->qs, e->tile_atom0, e->tile_cnt, e->work,
                     e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
+                    e->bonded_tf, e->bonded_csr_start, e->bonded_csr_slot,
                     e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start, e->cn_pair,
                     e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
     for (void* p : ptrs)
@@ -391,6 +394,25 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     e->na = (int)angle_p.size() / 2;
     e->nd = (int)dih_p.size() / 3;
     e->ni = (int)imp_p.size() / 2;
+    // per-atom lists of bonded force slots (ascending slot order)
+    {
+        std::vector<int> slot_atom;
+        slot_atom.reserve(2 * (size_t)e->nb + 3 * (size_t)e->na + 4 * (size_t)(e->nd + e->ni));
+        slot_atom.insert(slot_atom.end(), bonds.begin(), bonds.end());
+        slot_atom.insert(slot_atom.end(), angles.begin(), angles.end());
+        slot_atom.insert(slot_atom.end(), dih.begin(), dih.end());
+        slot_atom.insert(slot_atom.end(), imp.begin(), imp.end());
+        std::vector<int> start(N + 1, 0), fillp(N, 0), slots(slot_atom.size());
+        for (int a : slot_atom) start[a + 1]++;
+        for (int i = 0; i < N; ++i) start[i + 1] += start[i];
+        for (size_t q = 0; q < slot_atom.size(); ++q) {
+            const int a = slot_atom[q];
+            slots[start[a] + fillp[a]++] = (int)q;
+        }
+        HIPCHK(upload(&e->bonded_csr_start, start));
+        HIPCHK(upload(&e->bonded_csr_slot, slots));
+        HIPCHK(hipMalloc((void**)&e->bonded_tf, std::max<size_t>(slot_atom.size(), 1) * 3 * sizeof(double)));
+    }
     e->temps.assign(n_sys, 0.0);
     e->dofcorr.assign(n_sys, 1.0);
 
@@ -529,6 +551,8 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         b.nb = e->nb; b.na = e->na; b.nd = e->nd; b.ni = e->ni;
         b.bonds = e->bonds; b.bond_p = e->bond_p; b.angles = e->angles; b.angle_p = e->angle_p;
         b.dih = e->dih; b.dih_p = e->dih_p; b.imp = e->imp; b.imp_p = e->imp_p;
+        b.tf = e->bonded_tf; b.csr_start = e->bonded_csr_start; b.csr_slot = e->bonded_csr_slot;
+        b.n_atoms = e->n_atoms;
         og_launch_bonded(b, energy, e->stream);
     }
     og_pair_args a = pair_args(e);

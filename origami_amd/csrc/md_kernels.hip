@@ -27,10 +27,10 @@ __device__ __forceinline__ d3 cross(d3 a, d3 b) {
 }
 __device__ __forceinline__ double nrm(d3 a) { return sqrt(dot(a, a)); }
 
-__device__ __forceinline__ void addf(long long* facc, int atom, d3 f) {
-    og_atomic_add_fix(facc + 3 * (size_t)atom + 0, og_to_fix(f.x));
-    og_atomic_add_fix(facc + 3 * (size_t)atom + 1, og_to_fix(f.y));
-    og_atomic_add_fix(facc + 3 * (size_t)atom + 2, og_to_fix(f.z));
+__device__ __forceinline__ void put(double* tf, long long slot, d3 f) {
+    tf[3 * slot] = f.x;
+    tf[3 * slot + 1] = f.y;
+    tf[3 * slot + 2] = f.z;
 }
 
 // forces_bonded.rs:124-167
@@ -80,12 +80,13 @@ __device__ void bonded_body(const og_bonded_args& a) {
             d3 d = sub(ld3(a.pos + 3 * ia), ld3(a.pos + 3 * ib));
             d = neg(d);  // d = x_b - x_a
             const double r = nrm(d);
+            d3 f = {0.0, 0.0, 0.0};
             if (r >= 1e-9) {
                 const double mag = 2.0 * K * (r - r0) * OG_KCAL_TO_KJ;
-                const d3 f = mul(d, mag / r);
-                addf(a.facc, ia, f);
-                addf(a.facc, ib, neg(f));
+                f = mul(d, mag / r);
             }
+            put(a.tf, 2ll * k, f);
+            put(a.tf, 2ll * k + 1, neg(f));
             if (ENERGY) {
                 const double dr = r - r0;
                 og_atomic_add_fix(a.eacc + a.atom_sys[ia] * OG_NE + OG_E_BOND,
@@ -100,6 +101,8 @@ __device__ void bonded_body(const og_bonded_args& a) {
             const d3 pb = ld3(a.pos + 3 * ib);
             const d3 u = sub(ld3(a.pos + 3 * ia), pb), v = sub(ld3(a.pos + 3 * ic), pb);
             const double un = nrm(u), vn = nrm(v);
+            const long long s0 = 2ll * a.nb + 3ll * kk;
+            d3 fa = {0.0, 0.0, 0.0}, fc = {0.0, 0.0, 0.0};
             if (un >= 1e-9 && vn >= 1e-9) {
                 const d3 uh = dv(u, un), vh = dv(v, vn);
                 double ct = fmin(fmax(dot(uh, vh), -1.0), 1.0);
@@ -107,11 +110,8 @@ __device__ void bonded_body(const og_bonded_args& a) {
                 if (st >= 1e-9) {
                     const double theta = acos(ct);
                     const double coeff = 2.0 * K * (theta - th0) * OG_KCAL_TO_KJ / st;
-                    const d3 fa = mul(sub(vh, mul(uh, ct)), coeff / un);
-                    const d3 fc = mul(sub(uh, mul(vh, ct)), coeff / vn);
-                    addf(a.facc, ia, fa);
-                    addf(a.facc, ib, neg(add(fa, fc)));
-                    addf(a.facc, ic, fc);
+                    fa = mul(sub(vh, mul(uh, ct)), coeff / un);
+                    fc = mul(sub(uh, mul(vh, ct)), coeff / vn);
                 }
                 if (ENERGY) {
                     const double dt = acos(fmin(fmax(dot(uh, vh), -1.0), 1.0)) - th0;
@@ -119,6 +119,9 @@ __device__ void bonded_body(const og_bonded_args& a) {
                                       og_to_fix(K * dt * dt * OG_KCAL_TO_KJ));
                 }
             }
+            put(a.tf, s0, fa);
+            put(a.tf, s0 + 1, neg(add(fa, fc)));
+            put(a.tf, s0 + 2, fc);
             continue;
         }
         kk -= a.na;
@@ -129,6 +132,8 @@ __device__ void bonded_body(const og_bonded_args& a) {
                  pd = ld3(a.pos + 3 * q[3]);
         d3 da, db, dc, dd;
         double phi;
+        const long long s0 = 2ll * a.nb + 3ll * a.na + (is_dih ? 4ll * kk : 4ll * a.nd + 4ll * kk);
+        d3 f0 = {0.0, 0.0, 0.0}, f1 = f0, f2 = f0, f3 = f0;
         if (dihedral_gradient(pa, pb, pc, pd, da, db, dc, dd, phi)) {
             double dvdphi;
             if (is_dih) {
@@ -137,11 +142,15 @@ __device__ void bonded_body(const og_bonded_args& a) {
             } else {
                 dvdphi = 2.0 * a.imp_p[2 * kk] * wrap_pi(phi - a.imp_p[2 * kk + 1]) * OG_KCAL_TO_KJ;
             }
-            addf(a.facc, q[0], mul(da, -dvdphi));
-            addf(a.facc, q[1], mul(db, -dvdphi));
-            addf(a.facc, q[2], mul(dc, -dvdphi));
-            addf(a.facc, q[3], mul(dd, -dvdphi));
+            f0 = mul(da, -dvdphi);
+            f1 = mul(db, -dvdphi);
+            f2 = mul(dc, -dvdphi);
+            f3 = mul(dd, -dvdphi);
         }
+        put(a.tf, s0, f0);
+        put(a.tf, s0 + 1, f1);
+        put(a.tf, s0 + 2, f2);
+        put(a.tf, s0 + 3, f3);
         if (ENERGY) {
             const double chi = measure_dihedral(pa, pb, pc, pd);
             double e;
@@ -164,6 +173,24 @@ __device__ void bonded_body(const og_bonded_args& a) {
 __global__ void __launch_bounds__(256) og_k_bonded(og_bonded_args a) { bonded_body<false>(a); }
 __global__ void __launch_bounds__(256) og_k_bonded_e(og_bonded_args a) { bonded_body<true>(a); }
 
+// Per atom: sum its (term, atom) force slots in slot order (bonds, angles,
+// dihedrals, impropers, each in list order, as origami's term loops add
+// them) in FP64, then one fixed-point add per component.
+__global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_atoms) return;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int q = a.csr_start[i]; q < a.csr_start[i + 1]; ++q) {
+        const long long s = a.csr_slot[q];
+        fx += a.tf[3 * s];
+        fy += a.tf[3 * s + 1];
+        fz += a.tf[3 * s + 2];
+    }
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix(fx));
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix(fy));
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
+}
+
 void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
     const int total = a.nb + a.na + a.nd + a.ni;
     if (total <= 0) return;
@@ -171,6 +198,8 @@ void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
     if (g > 4096) g = 4096;
     if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_bonded, dim3(g), dim3(256), 0, s, a);
+    if (a.n_atoms > 0)
+        hipLaunchKernelGGL(og_k_bonded_gather, dim3((a.n_atoms + 255) / 256), dim3(256), 0, s, a);
 }
 
 // ---------------------------------------------------------------------------

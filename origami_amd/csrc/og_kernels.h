@@ -41,6 +41,12 @@ struct og_bonded_args {
     const int* angles; const double* angle_p;     // (K, θ0)
     const int* dih; const double* dih_p;          // (k, n, δ)
     const int* imp; const double* imp_p;          // (K, ψ0)
+    // per-(term, atom) force slots, gathered per atom in slot order:
+    // bonds 2 per term, then angles 3, dihedrals 4, impropers 4
+    double* tf;                                   // [slots][3]
+    const int* csr_start;                         // [N+1]
+    const int* csr_slot;                          // [slots]
+    int n_atoms;
 };
 
 struct og_radii_args {

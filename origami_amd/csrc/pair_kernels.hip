@@ -3,11 +3,15 @@
 //
 // Work decomposition: one wave64 per tile pair (I <= J) of one system,
 // grid-stride over a flattened work list that covers every system of the
-// batch.  Lane l owns i-atom I*64+l; the j loop is wave-uniform, j data is
-// broadcast with v_readlane (SGPR operands), and the j-side reaction
-// (Newton's third law) is reduced across the 64 lanes and parked in lane j's
-// accumulator.  Per tile pair every lane flushes one FP32 partial per
-// component into the int64 fixed-point accumulators.
+// batch.  Lane l owns i-atom I*64+l.  The j atoms travel around the wave:
+// lane l starts with j-atom l and, after every pair step, each lane passes
+// its j data and its running j-side reaction (Newton's third law) to the
+// neighbouring lane with one DPP wave rotate (v_mov_dpp wave_rol:1), so in
+// 64 steps every lane meets every j atom and the j-side sums arrive back
+// home.  A diagonal tile (I = J) runs steps 1..32 (step 32 on lanes 0..31)
+// and swaps the halves of the wave at the end.  Per tile pair every lane
+// flushes one FP32 partial per component into the int64 fixed-point
+// accumulators.  The FP64 energy kernel keeps v_readlane broadcasts.
 //
 // Physics follows origami's SoA kernels:
 //   descreening   energy/src/gb.rs:271-283 (pair), :347-369 (h)
@@ -55,6 +59,15 @@ __device__ __forceinline__ float rl(float v, int j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
 
+// one-lane rotation of a value around the wave (DPP wave_rol:1)
+// (mov_dpp: every lane is written, so no "old" value has to be materialised)
+__device__ __forceinline__ float rot(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ int roti(int v) { return __builtin_amdgcn_mov_dpp(v, 0x134, 0xF, 0xF, false); }
+// exchange with the lane 32 away (halves of the wave)
+__device__ __forceinline__ float swap32(float v) { return __shfl_xor(v, 32, 64); }
+
 // Descreening integral term h(r, rho~_i, S_j) of energy/src/gb.rs:347-369,
 // written with one reciprocal and one log:  1/L - 1/U = (U-L)/(LU),
 // 1/U^2 - 1/L^2 = -(1/L - 1/U)(U+L)/(LU),  ln(L/U) = ln(L^2/(LU)).
@@ -76,11 +89,15 @@ __device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, floa
 
 template <bool MODEA, bool DIAG>
 __device__ __forceinline__ void descreen_tile(const og_pair_args& a, const LaneData& di,
-                                              const LaneData& dj, int lane, int cnt_j,
-                                              float& acc_i, float& acc_j) {
-    for (int jj = 0; jj < cnt_j; ++jj) {
-        const float xj = rl(dj.x, jj), yj = rl(dj.y, jj), zj = rl(dj.z, jj);
-        const float rtj = rl(dj.gb.x, jj), Sj = rl(dj.gb.y, jj);
+                                              const LaneData& dj, int lane, float& acc_i, float& acc_j) {
+    float xj = dj.x, yj = dj.y, zj = dj.z, rtj = dj.gb.x, Sj = dj.gb.y;
+    float gj = 0.0f;
+    const int k0 = DIAG ? 1 : 0, k1 = DIAG ? 32 : 63;
+    if (DIAG) {
+        xj = rot(xj); yj = rot(yj); zj = rot(zj); rtj = rot(rtj); Sj = rot(Sj);
+    }
+#pragma unroll 2
+    for (int k = k0; k <= k1; ++k) {
         const float dx = xj - di.x, dy = yj - di.y, dz = zj - di.z;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float inv_r = __builtin_amdgcn_rsqf(r2);
@@ -89,13 +106,16 @@ __device__ __forceinline__ void descreen_tile(const og_pair_args& a, const LaneD
         float hji = descreen_h(r, inv_r, rtj, di.gb.y);
         bool ok = r2 >= 1e-18f;
         if (MODEA) ok = ok && (r2 <= a.rc2_ds);
-        if (DIAG) ok = ok && (jj > lane);
-        hij = ok ? hij : 0.0f;
-        hji = ok ? hji : 0.0f;
-        acc_i += hij;
-        const float red = og_wave_sum(hji);
-        if (lane == jj) acc_j += red;
+        if (DIAG) ok = ok && (k < 32 || lane < 32);
+        acc_i += ok ? hij : 0.0f;
+        gj += ok ? hji : 0.0f;
+        if (k < k1) {
+            xj = rot(xj); yj = rot(yj); zj = rot(zj); rtj = rot(rtj); Sj = rot(Sj);
+            gj = rot(gj);
+        }
     }
+    // home: one more step around the wave (64 in all), or half a turn on a diagonal tile
+    acc_j += DIAG ? swap32(gj) : rot(gj);
 }
 
 template <bool MODEA>
@@ -113,10 +133,10 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
         const LaneData di = load_lane(a, it.ti, a0i, ci, lane, ox, oy, oz);
         float acc_i = 0.0f, acc_j = 0.0f;
         if (it.ti == it.tj) {
-            descreen_tile<MODEA, true>(a, di, di, lane, cj, acc_i, acc_j);
+            descreen_tile<MODEA, true>(a, di, di, lane, acc_i, acc_j);
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
-            descreen_tile<MODEA, false>(a, di, dj, lane, cj, acc_i, acc_j);
+            descreen_tile<MODEA, false>(a, di, dj, lane, acc_i, acc_j);
         }
         if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix((double)acc_i));
         if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix((double)acc_j));
@@ -140,14 +160,23 @@ struct ForceAcc {
 
 template <bool MODEA, bool MASKED, bool DIAG>
 __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData& di,
-                                           const LaneData& dj, int lane, int cnt_j,
+                                           const LaneData& dj, int lane,
                                            unsigned long long xmask, unsigned long long m14,
                                            ForceAcc& acc) {
-    for (int jj = 0; jj < cnt_j; ++jj) {
-        const float xj = rl(dj.x, jj), yj = rl(dj.y, jj), zj = rl(dj.z, jj);
-        const float qj = rl(dj.q, jj);
-        const float sej = rl(dj.par.x, jj), rmj = rl(dj.par.y, jj);
-        const float Rj = rl(dj.gb.z, jj), iRj = rl(dj.gb.w, jj);
+    float xj = dj.x, yj = dj.y, zj = dj.z, qj = dj.q;
+    float sej = dj.par.x, rmj = dj.par.y, se14j = dj.par.z, rm14j = dj.par.w;
+    float Rj = dj.gb.z, iRj = dj.gb.w;
+    int jid = lane;                         // j index within tile J held by this lane
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const int k0 = DIAG ? 1 : 0, k1 = DIAG ? 32 : 63;
+    auto advance = [&]() {
+        xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj);
+        sej = rot(sej); rmj = rot(rmj); Rj = rot(Rj); iRj = rot(iRj);
+        if (MASKED) { se14j = rot(se14j); rm14j = rot(rm14j); jid = roti(jid); }
+    };
+    if (DIAG) advance();
+#pragma unroll 2
+    for (int k = k0; k <= k1; ++k) {
         const float dx = xj - di.x, dy = yj - di.y, dz = zj - di.z;
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float inv_r = __builtin_amdgcn_rsqf(r2);
@@ -155,9 +184,8 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
         float rmin, eij;
         bool excl = false;
         if (MASKED) {
-            const float se14j = rl(dj.par.z, jj), rm14j = rl(dj.par.w, jj);
-            excl = (xmask >> jj) & 1ull;
-            const bool one4 = (m14 >> jj) & 1ull;
+            excl = (xmask >> jid) & 1ull;
+            const bool one4 = (m14 >> jid) & 1ull;
             rmin = one4 ? di.par.w + rm14j : di.par.y + rmj;
             eij = one4 ? di.par.z * se14j : di.par.x * sej;
         } else {
@@ -184,21 +212,29 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
         if (MODEA) gbc = (r2 <= a.rc2_gb) ? gbc : 0.0f;
         float c = cnb + gbc;
         bool ok = r2 >= 1e-18f;
-        if (DIAG) ok = ok && (jj > lane);
+        if (DIAG) ok = ok && (k < 32 || lane < 32);
         c = ok ? c : 0.0f;
         const float fx = c * dx, fy = c * dy, fz = c * dz;
         acc.fx += fx;
         acc.fy += fy;
         acc.fz += fz;
-        const float sx = og_wave_sum(fx);
-        const float sy = og_wave_sum(fy);
-        const float sz = og_wave_sum(fz);
-        if (lane == jj) {
-            acc.gx -= sx;
-            acc.gy -= sy;
-            acc.gz -= sz;
+        gx -= fx;
+        gy -= fy;
+        gz -= fz;
+        if (k < k1) {
+            advance();
+            gx = rot(gx); gy = rot(gy); gz = rot(gz);
         }
     }
+    // home: one more step around the wave (64 in all), or half a turn on a diagonal tile
+    if (DIAG) {
+        gx = swap32(gx); gy = swap32(gy); gz = swap32(gz);
+    } else {
+        gx = rot(gx); gy = rot(gy); gz = rot(gz);
+    }
+    acc.gx += gx;
+    acc.gy += gy;
+    acc.gz += gz;
 }
 
 template <bool MODEA>
@@ -220,13 +256,13 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
             m14 = a.masks[(size_t)it.mask * 128 + 64 + lane];
         }
         if (it.ti == it.tj) {
-            force_tile<MODEA, true, true>(a, di, di, lane, cj, xm, m14, acc);
+            force_tile<MODEA, true, true>(a, di, di, lane, xm, m14, acc);
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
             if (it.mask >= 0)
-                force_tile<MODEA, true, false>(a, di, dj, lane, cj, xm, m14, acc);
+                force_tile<MODEA, true, false>(a, di, dj, lane, xm, m14, acc);
             else
-                force_tile<MODEA, false, false>(a, di, dj, lane, cj, xm, m14, acc);
+                force_tile<MODEA, false, false>(a, di, dj, lane, xm, m14, acc);
         }
         if (lane < ci) {
             long long* f = a.facc + 3 * (size_t)(a0i + lane);

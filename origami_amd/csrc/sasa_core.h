@@ -148,7 +148,9 @@ OG_HD v3 midpoint_on_circle(v3 pt, const Cap& c, double delta) {
     if (rn < 1e-12) return pt;
     const v3 u = radial / rn;
     const v3 v = cross(c.axis, u);
-    return axial + (u * cos(delta) + v * sin(delta)) * rn;
+    double sd, cd;
+    sincos(delta, &sd, &cd);
+    return axial + (u * cd + v * sd) * rn;
 }
 
 OG_HD inline double wrap_2pi_positive(double t) {

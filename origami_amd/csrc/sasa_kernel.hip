@@ -79,6 +79,8 @@ struct ArrLds {
     unsigned char tk[SA_TMAX], tm[SA_TMAX], tq[SA_TMAX];  // chunk cap, other cap, p/q/probe
     unsigned char perm[SA_TMAX];                     // sorted order within segments
     unsigned char rm[SA_TMAX];                       // sorted position removed by dedup
+    unsigned char est[SA_TMAX];                      // slot state: 0 no arc, 1 undecided, 2 arc
+    unsigned char ulist[SA_TMAX];                    // undecided slots
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -397,26 +399,88 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
         }
         wave_sync();
 
-        // one slot per kept sorted position: segment to the next kept
-        // position (cyclic), or the probe test for a probe entry
-        for (int base = 0; base < T; base += 64) {
-            const int y = base + lane;
-            bool emit = false, segm = false;
-            double theta = 0.0;
-            v3 st = mk(0, 0, 0), en = mk(0, 0, 0);
-            int kk = 0, oth = 0;
-            if (y < T && !L.rm[y]) {
+        // One slot per kept sorted position: the segment to the next kept
+        // position (cyclic), or the probe test of a probe entry.  The burial
+        // test is an any() over the other caps, so it is run in two rounds:
+        // (A) every slot tries the likeliest covers (the hint cap, the two
+        // caps bounding the segment, and the caps of the four transitions
+        // before it, whose entries may still be open); (B) the slots none of
+        // them covers scan all caps, lane-parallel over a compacted list.
+        // est[y]: 0 no arc, 1 undecided, 2 arc.
+        for (int y = lane; y < T; y += 64) {
+            unsigned char st8 = 0;
+            if (!L.rm[y]) {
                 const int x = L.perm[y];
                 const int kl = L.tk[x];
-                kk = L.act[a0 + kl];
-                const Cap ck = cap_at(L, kk);
+                const int kk = L.act[a0 + kl];
                 const int hint = L.hint[kk];
                 const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
                 if (L.tq[x] == 2) {
-                    // whole circle: a boundary loop unless its probe point is buried
-                    const bool buried = (hint != 0xFF && contains_strict(cap_at(L, hint), apt)) ||
-                                        any_contains(L, nc, kk, apt);
-                    emit = !buried;
+                    st8 = (hint != 0xFF && contains_strict(cap_at(L, hint), apt)) ? 0 : 1;
+                } else {
+                    const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                    int yn = y + 1 < s1 ? y + 1 : s0;
+                    while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
+                    const int bx = L.perm[yn];
+                    const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
+                    if (!(ccw < 1e-9)) {
+                        const v3 mid = midpoint_on_circle(apt, cap_at(L, kk), ccw / 2.0);
+                        bool bad = (hint != 0xFF && contains_strict(cap_at(L, hint), mid)) ||
+                                   contains_strict(cap_at(L, L.tm[x]), mid) ||
+                                   contains_strict(cap_at(L, L.tm[bx]), mid);
+                        const int len = s1 - s0;
+                        for (int t = 1; t <= 4 && !bad; ++t) {
+                            int yp = y - t;
+                            while (yp < s0) yp += len;
+                            bad = contains_strict(cap_at(L, L.tm[L.perm[yp]]), mid);
+                        }
+                        st8 = bad ? 0 : 1;
+                    }
+                }
+            }
+            L.est[y] = st8;
+        }
+        wave_sync();
+        int U = 0;
+        for (int base = 0; base < T; base += 64) {
+            const int y = base + lane;
+            const bool u = y < T && L.est[y] == 1;
+            const unsigned long long um = __ballot(u);
+            if (u) L.ulist[U + popc(um & lt)] = (unsigned char)y;
+            U += popc(um);
+        }
+        wave_sync();
+        for (int q = lane; q < U; q += 64) {
+            const int y = L.ulist[q];
+            const int x = L.perm[y];
+            const int kl = L.tk[x];
+            const int kk = L.act[a0 + kl];
+            const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
+            v3 probe = apt;
+            if (L.tq[x] != 2) {
+                const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                int yn = y + 1 < s1 ? y + 1 : s0;
+                while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
+                const double ccw = wrap_2pi_positive(L.ta[L.perm[yn]] - L.ta[x]);
+                probe = midpoint_on_circle(apt, cap_at(L, kk), ccw / 2.0);
+            }
+            L.est[y] = any_contains(L, nc, kk, probe) ? 0 : 2;
+        }
+        wave_sync();
+        // emission in slot order
+        for (int base = 0; base < T; base += 64) {
+            const int y = base + lane;
+            const bool emit = y < T && L.est[y] == 2;
+            bool segm = false;
+            double theta = 0.0;
+            v3 st = mk(0, 0, 0), en = mk(0, 0, 0);
+            int kk = 0, oth = 0;
+            if (emit) {
+                const int x = L.perm[y];
+                const int kl = L.tk[x];
+                kk = L.act[a0 + kl];
+                const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
+                if (L.tq[x] == 2) {
                     st = apt; en = apt; theta = -two_pi;
                 } else {
                     segm = true;
@@ -424,21 +488,10 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
                     int yn = y + 1 < s1 ? y + 1 : s0;
                     while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
                     const int bx = L.perm[yn];
-                    const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
-                    if (!(ccw < 1e-9)) {
-                        const v3 mid = midpoint_on_circle(apt, ck, ccw / 2.0);
-                        const int oa = L.tm[x], ob = L.tm[bx];
-                        const bool bad = (hint != 0xFF && contains_strict(cap_at(L, hint), mid)) ||
-                                         contains_strict(cap_at(L, oa), mid) ||
-                                         contains_strict(cap_at(L, ob), mid) || any_contains(L, nc, kk, mid);
-                        emit = !bad;
-                        if (emit) {
-                            st = mk(L.tx[bx], L.ty[bx], L.tz[bx]);
-                            en = apt;
-                            theta = -ccw;
-                            oth = oa;
-                        }
-                    }
+                    st = mk(L.tx[bx], L.ty[bx], L.tz[bx]);
+                    en = apt;
+                    theta = -wrap_2pi_positive(L.ta[bx] - L.ta[x]);
+                    oth = L.tm[x];
                 }
             }
             const unsigned long long em = __ballot(emit), es = __ballot(emit && segm);
