@@ -604,7 +604,8 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
     }
     if (flags & OG_SASA) {
         ProfScope p(e, OG_K_SASA);
-        int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream);
+        int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream,
+                                energy ? e->eacc : nullptr, e->atom_sys);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
     }
     HIPCHK(hipGetLastError());

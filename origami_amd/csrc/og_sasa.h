@@ -8,6 +8,8 @@
 // over a batch.  Protein atoms carry ~5-20; exceeding it is reported as an
 // error, never as silently missing forces.
 #define OG_SASA_POOL_ARCS 64
+// probe points of the accessible-component count (arrangement.rs:327-389)
+#define OG_SASA_PROBES 500
 
 struct og_sasa_data {
     int n_work = 0;               // atoms with gamma != 0
@@ -40,6 +42,12 @@ struct og_sasa_data {
     unsigned* vtx_code = nullptr;
     double* vtx_terms = nullptr;  // [9 per vertex]
     int* owners = nullptr;        // cap owners per atom
+    // accessible area (energy evaluations only)
+    double4* probes = nullptr;    // [OG_SASA_PROBES] golden-spiral points
+    int* padj_start = nullptr;    // [OG_SASA_PROBES + 1] probes within 2.5 spacings
+    int* padj = nullptr;
+    long long* eacc = nullptr;    // engine energy accumulators (set per launch)
+    const int* atom_sys = nullptr;
 };
 
 // sys_tile_off[s] .. sys_tile_off[s+1] are system s's tiles (64-atom, never
@@ -49,8 +57,10 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
                   const std::vector<double>& gamma, const std::vector<int>& sys_tile_off,
                   const int* tile_atom0_dev, const int* tile_cnt_dev, int n_tiles);
 void og_sasa_free(og_sasa_data* d);
+// eacc != null: also accumulate gamma_i * A_i (unscaled gamma) into each
+// system's SASA energy slot (powersasa_energy, mod.rs:55-111).
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
-                   hipStream_t s);
+                   hipStream_t s, long long* eacc = nullptr, const int* atom_sys = nullptr);
 // Arrangement-kernel width (blocks of 2 waves); at most the staging slots allocated.
 int og_sasa_max_blocks(const og_sasa_data* d);
 void og_sasa_set_blocks(og_sasa_data* d, int blocks);

@@ -43,6 +43,7 @@
 #include <string>
 
 #include "og_device.h"
+#include "og_kernels.h"
 #include "og_sasa.h"
 #include "sasa_core.h"
 
@@ -136,8 +137,140 @@ __device__ __forceinline__ bool bound_inside(v3 p, const Cap& ck, double ct, dou
     return rn >= 1e-3 && h * ct - rn * st > ca_m + 1e-11;
 }
 
+__device__ __forceinline__ void add_sasa_energy(const og_sasa_data& d, int ia, double area) {
+    // powersasa_energy (mod.rs:55-111): gamma * A, no gamma scale
+    og_atomic_add_fix(d.eacc + (size_t)d.atom_sys[ia] * OG_NE + OG_E_SASA,
+                      og_to_fix(d.gamma[ia] * area));
+}
+
+__device__ __forceinline__ double wave_min_d(double v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i(int v) {
+    for (int o = 32; o >= 1; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Accessible area of one atom from its boundary (area.rs:19-126):
+// A = R^2 (2 pi (2c - L) - sum_arcs cos(a) theta - sum_vertices eps), clamped
+// to [0, 4 pi R^2]; L and the vertex sum from the face walk
+// (walk_faces_summing_vertices, area.rs:140-222), c from the probe
+// components (count_accessible_components, arrangement.rs:327-389) capped
+// at L and at least 1.  The chunk arrays of ArrLds are free here and hold
+// the walk's visited flags, the probes' accessibility and their labels.
+__device__ void atom_area(ArrLds& L, const og_sasa_data& d, int ia, double ri, int nc, int na, int lane,
+                          const double* wd, const int* wi) {
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    unsigned char* visited = L.perm;                                  // [na] (na <= 256)
+    for (int a = lane; a < na; a += 64) visited[a] = 0;
+    wave_sync();
+    // ---- face walk: lanes over candidate arcs, the walk itself is uniform ----
+    int loops = 0;
+    double vsum = 0.0;
+    for (int start = 0; start < na; ++start) {
+        if (visited[start]) continue;
+        if (wi[WI_ARC + 2 * start + 1]) {          // full circle: a loop, no vertices
+            if (lane == 0) visited[start] = 1;
+            wave_sync();
+            ++loops;
+            continue;
+        }
+        int cur = start;
+        for (int guard = 0; guard <= na; ++guard) {   // origami's walk has no guard
+            if (lane == 0) visited[cur] = 1;
+            wave_sync();
+            const double* Ac = wd + WD_ARC + 7 * cur;
+            const v3 end_pt = mk(Ac[3], Ac[4], Ac[5]);
+            const v3 t_in = normalize(cross(end_pt, cap_at(L, wi[WI_ARC + 2 * cur]).axis));
+            int first = 0x7fffffff, best_i = 0x7fffffff;
+            double best_a = INFINITY;
+            for (int base = 0; base < na; base += 64) {
+                const int c = base + lane;
+                double ang = INFINITY;
+                if (c < na && !wi[WI_ARC + 2 * c + 1]) {
+                    const double* Cc = wd + WD_ARC + 7 * c;
+                    if (norm2(mk(Cc[0], Cc[1], Cc[2]) - end_pt) < 1e-10) {
+                        first = min(first, c);
+                        if (c != cur) {
+                            const v3 t_out = normalize(cross(end_pt, cap_at(L, wi[WI_ARC + 2 * c]).axis));
+                            const double dt = fmin(fmax(dot(t_in, t_out), -1.0), 1.0);
+                            ang = atan2(dot(end_pt, cross(t_in, t_out)), dt);
+                            if (ang <= 1e-9) ang += two_pi;
+                        }
+                    }
+                }
+                // first strictly smaller angle in candidate order = argmin, ties to the lower index
+                const double m = wave_min_d(ang);
+                const int who = wave_min_i(ang == m && m < best_a ? c : 0x7fffffff);
+                if (m < best_a) { best_a = m; best_i = who; }
+            }
+            first = wave_min_i(first);
+            if (first == 0x7fffffff) break;           // open path
+            const int best = best_i != 0x7fffffff ? best_i : first;
+            const v3 t_out = normalize(cross(end_pt, cap_at(L, wi[WI_ARC + 2 * best]).axis));
+            const double dt = fmin(fmax(dot(t_in, t_out), -1.0), 1.0);
+            vsum += atan2(dot(end_pt, cross(t_in, t_out)), dt);
+            if (best == start) break;
+            cur = best;
+        }
+        ++loops;
+    }
+    wave_sync();
+    // ---- accessible probe components ----
+    unsigned char* acc = L.tk;                                        // [OG_SASA_PROBES] (tk..tq, 768 bytes)
+    int* label = reinterpret_cast<int*>(L.ta);                        // [OG_SASA_PROBES] (ta, 2 KB)
+    bool any_acc = false;
+    for (int i = lane; i < OG_SASA_PROBES; i += 64) {
+        const double4 pr = d.probes[i];
+        bool buried = false;
+        for (int k = 0; k < nc && !buried; ++k) buried = contains_strict(cap_at(L, k), mk(pr.x, pr.y, pr.z));
+        acc[i] = buried ? 0 : 1;
+        label[i] = buried ? 0x7fffffff : i;
+        any_acc = any_acc || !buried;
+    }
+    int comps = 0;
+    if (__ballot(any_acc)) {
+        wave_sync();
+        for (int it = 0; it < OG_SASA_PROBES; ++it) {
+            bool changed = false;
+            for (int i = lane; i < OG_SASA_PROBES; i += 64) {
+                if (!acc[i]) continue;
+                int m = label[i];
+                for (int q = d.padj_start[i]; q < d.padj_start[i + 1]; ++q) {
+                    const int j = d.padj[q];
+                    if (acc[j]) m = min(m, label[j]);
+                }
+                if (m < label[i]) { label[i] = m; changed = true; }
+            }
+            wave_sync();
+            if (!__ballot(changed)) break;
+        }
+        for (int i = lane; i < OG_SASA_PROBES; i += 64) comps += (acc[i] && label[i] == i) ? 1 : 0;
+        for (int o = 32; o >= 1; o >>= 1) comps += __shfl_xor(comps, o, 64);
+    }
+    // ---- area ----
+    if (lane == 0) {
+        double arc_sum = 0.0;
+        for (int a = 0; a < na; ++a) arc_sum += cap_at(L, wi[WI_ARC + 2 * a]).ca * wd[WD_ARC + 7 * a + 6];
+        double area = 0.0;
+        if (loops > 0) {
+            int c = comps < loops ? comps : loops;
+            if (c < 1) c = 1;
+            const double r_sq = ri * ri;
+            area = r_sq * (two_pi * (double)(2 * c - loops) - arc_sum - vsum);
+            if (area < 0.0) area = 0.0;
+            if (area > 2.0 * two_pi * r_sq) area = 2.0 * two_pi * r_sq;
+        }
+        add_sasa_energy(d, ia, area);
+    }
+    wave_sync();
+}
+
 // Arrangement of one work atom; returns its pool record (empty when the atom
 // contributes nothing or exceeded capacity).
+template <bool AREA>
 __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos, int w, int lane,
                              unsigned long long lt, double* wd, int* wi) {
     const int4 empty = make_int4(0, 0, 0, 0);
@@ -194,6 +327,8 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
         if (lane == 0) atomicAdd(d.overflow, 1);
         return empty;
     }
+    if (AREA && !enclosed && nc == 0 && lane == 0)
+        add_sasa_energy(d, ia, 4.0 * 3.14159265358979323846 * ri * ri);   // fully exposed
     if (enclosed || nc == 0) return empty;
 
     // ---------------- per-cap transition counts and burial hints ----------
@@ -524,6 +659,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     }
     // FullyBuried and the other arc-less outcomes contribute no gradient
     if (na == 0) return empty;
+    if (AREA) atom_area(L, d, ia, ri, nc, na, lane, wd, wi);
 
     // ---------------- pool allocation ----------------
     int abase = 0, vbase = 0, obase = 0;
@@ -613,6 +749,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
 
 }  // namespace
 
+template <bool AREA>
 __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     __shared__ ArrLds lds_all[SA_WAVES];
     const int lane = threadIdx.x & 63;
@@ -624,7 +761,8 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_arrange(og_sasa_data 
     int* wi = d.iscratch + (size_t)slot * WI_TOTAL;
     const unsigned long long lt = lanemask_lt(lane);
     for (int w = slot; w < d.n_work; w += n_slots) {
-        const int4 rec = arrange_atom(L, d, pos, w, lane, lt, wd, wi);
+        const int4 rec = AREA ? arrange_atom<true>(L, d, pos, w, lane, lt, wd, wi)
+                              : arrange_atom<false>(L, d, pos, w, lane, lt, wd, wi);
         if (lane == 0) d.rec[w] = rec;
         wave_sync();
     }
@@ -813,6 +951,34 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
     if (!up(&d->work, work) || !up(&d->tile_begin, tb) || !up(&d->tile_end, te) ||
         !up(&d->radius, radius) || !up(&d->gamma, gamma))
         return fail("device allocation failed");
+    // area probes (count_accessible_components, arrangement.rs:327-389): golden-spiral
+    // points and, per probe, the probes within 2.5 spacings
+    {
+        std::vector<double4> pr(OG_SASA_PROBES);
+        const double golden = M_PI * (3.0 - std::sqrt(5.0));
+        for (int i = 0; i < OG_SASA_PROBES; ++i) {
+            const double z = 1.0 - (2.0 * i + 1.0) / (double)OG_SASA_PROBES;
+            const double t = 1.0 - z * z;
+            const double r = std::sqrt(t > 0.0 ? t : 0.0);
+            const double th = golden * i;
+            pr[i] = make_double4(r * std::cos(th), r * std::sin(th), z, 0.0);
+        }
+        const double spacing = std::sqrt(4.0 * M_PI / OG_SASA_PROBES);
+        double thresh = 2.5 * spacing;
+        if (thresh > M_PI) thresh = M_PI;
+        const double cos_thresh = std::cos(thresh);
+        std::vector<int> st{0}, adj;
+        for (int i = 0; i < OG_SASA_PROBES; ++i) {
+            for (int j = 0; j < OG_SASA_PROBES; ++j) {
+                if (j == i) continue;
+                const double dt = pr[i].x * pr[j].x + pr[i].y * pr[j].y + pr[i].z * pr[j].z;
+                if (dt > cos_thresh) adj.push_back(j);
+            }
+            st.push_back((int)adj.size());
+        }
+        if (!up(&d->probes, pr) || !up(&d->padj_start, st) || !up(&d->padj, adj))
+            return fail("device allocation failed");
+    }
     if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 4) ||
         !zalloc(&d->rec, (size_t)d->n_work))
         return fail("device allocation failed");
@@ -833,19 +999,24 @@ void og_sasa_free(og_sasa_data* d) {
     void* p[] = {d->work, d->tile_begin, d->tile_end, d->radius, d->gamma, d->tile_bound,
                  d->scratch, d->iscratch, d->overflow, d->counters, d->rec,
                  d->arc_atoms, d->arc_par, d->arc_code, d->arc_terms,
-                 d->vtx_atoms, d->vtx_sign, d->vtx_code, d->vtx_terms, d->owners};
+                 d->vtx_atoms, d->vtx_sign, d->vtx_code, d->vtx_terms, d->owners,
+                 (void*)d->probes, (void*)d->padj_start, (void*)d->padj};
     for (void* q : p)
         if (q) hipFree(q);
     *d = og_sasa_data{};
 }
 
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
-                   hipStream_t s) {
-    if (d->n_work == 0 || gamma_scale == 0.0) return 0;
+                   hipStream_t s, long long* eacc, const int* atom_sys) {
+    const bool area = eacc != nullptr;
+    if (d->n_work == 0 || (gamma_scale == 0.0 && !area)) return 0;
+    d->eacc = eacc;
+    d->atom_sys = atom_sys;
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles), dim3(64), 0, s, pos, d->tile_atom0,
                        d->tile_cnt, d->n_tiles, d->tile_bound);
     hipLaunchKernelGGL(og_k_sasa_reset, dim3(1), dim3(64), 0, s, d->counters);
-    hipLaunchKernelGGL(og_k_sasa_arrange, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    if (area) hipLaunchKernelGGL(og_k_sasa_arrange<true>, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    else hipLaunchKernelGGL(og_k_sasa_arrange<false>, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
     hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
     hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
                        0, s, *d, facc, gamma_scale);

@@ -235,13 +235,14 @@ class Engine:
             return f, [dict(zip(ENERGY_TERMS, row)) for row in e]
         return f
 
-    def total_energy(self, xyz=None, mode_a=False, surrogate=False):
+    def total_energy(self, xyz=None, mode_a=False, surrogate=False, include_sasa=False):
         e = np.zeros((self.n_systems, len(ENERGY_TERMS)))
         x = None
         if xyz is not None:
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
         _check(lib().og_energies(self._h, _d(x) if x is not None else None,
-                                 (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0), _d(e)))
+                                 (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0)
+                                 | (SASA if include_sasa else 0), _d(e)))
         out = []
         for row in e:
             dct = dict(zip(ENERGY_TERMS, row.tolist()))

@@ -285,3 +285,23 @@ def test_autotune_keeps_results(E, c2_system):
     f1 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
     assert np.array_equal(f0, f1)
     assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0
+
+
+def test_sasa_energy_matches_oracle(E, c1_system, c2_system):
+    """PowerSasa energy sum_i gamma_i A_i (powersasa_energy, no gamma scale) on
+    the GPU against the oracle's per-atom accessible areas, generic geometry,
+    uneven batch of two chains."""
+    systems = [c1_system, c2_system]
+    pos = [jittered(s, 5) for s in systems]
+    eng = E.Engine(systems)
+    e = eng.total_energy(np.concatenate(pos), include_sasa=True)
+    for k, s in enumerate(systems):
+        total, per_atom = O.Oracle(s).sasa_area(pos[k])
+        ref = float((s.sasa_gamma * per_atom).sum())
+        print("system %d SASA energy gpu %.10f oracle %.10f kJ/mol (area %.3f A^2)"
+              % (k, e[k]["sasa"], ref, total))
+        assert abs(e[k]["sasa"] - ref) <= 1e-9 * max(1.0, abs(ref))
+    # forces are unchanged by the energy request
+    f0 = eng.total_force(np.concatenate(pos), include_sasa=True, sasa_gamma_scale=0.25)
+    f1, _ = eng.total_force(np.concatenate(pos), include_sasa=True, sasa_gamma_scale=0.25, energies=True)
+    assert np.array_equal(f0, f1)
