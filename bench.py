@@ -284,6 +284,18 @@ def main():
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
             "traffic": None, "flops_per_pair": FLOPS_FORCE, "pairs_per_launch": pairs,
             "avg_ms": k_ms["force"]}
+    # HBM bytes per launch of the force kernel, from the rocprofv3 --pmc runs
+    # summarised in profiles/ (same kernel, same 157,072-atom batch)
+    tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                      "hbm_traffic_64chains_157k_atoms.json")
+    if os.path.exists(tf):
+        tr = json.load(open(tf))
+        if tr.get("atoms") == info["atoms"] and "og_k_force_b" in tr["kernels"]:
+            kb = tr["kernels"]["og_k_force_b"]
+            roof["traffic"] = kb["fetch_bytes"] + kb["write_bytes"]
+            roof["traffic_note"] = ("HBM bytes per launch (FETCH_SIZE + WRITE_SIZE, calibrated) from "
+                                    "profiles/hbm_traffic_64chains_157k_atoms.md; algorithmic bytes "
+                                    "13.2 MB, the rest is int64 fixed-point atomics")
     dominant = {"kernel_class": dom, "ms_per_step": totals[dom] / max(args.profile_steps, 1),
                 "bound": "latency / divergence (FP64 PowerSasa arrangement)" if dom == "sasa" else None}
     # FP32 pair kernels beside it
