@@ -115,6 +115,26 @@ OG_HD int intersect_count(const Cap& c1, const Cap& c2) {
     return fabs(c_sq) <= 1e-9 ? 1 : 2;
 }
 
+// intersect_count's answer, decided without its two divisions where it is
+// clear.  intersect_circles forms c^2 = 1 - a^2 - b^2 - 2ab*sigma, which
+// equals g / (1 - sigma^2) with g = (1 - ca1^2)(1 - ca2^2) - (sigma - ca1 ca2)^2.
+// With 1 - sigma^2 >= 1e-2 and c^2 near its thresholds (|a|, |b| <= 15
+// there), its rounding stays below 1e-10; the class is read from g when
+// g / (1 - sigma^2) is at least 5e-10 away from both thresholds (+-1e-9),
+// and by the exact arithmetic otherwise.
+OG_HD int intersect_count_fast(const Cap& c1, const Cap& c2) {
+    const double sigma = dot(c1.axis, c2.axis);
+    const double denom = 1.0 - sigma * sigma;
+    if (denom >= 1e-2) {
+        const double u = sigma - c1.ca * c2.ca;
+        const double g = (1.0 - c1.ca * c1.ca) * (1.0 - c2.ca * c2.ca) - u * u;
+        if (g < -1.5e-9 * denom) return 0;
+        if (g > 1.5e-9 * denom) return 2;
+        if (g > -0.5e-9 * denom && g < 0.5e-9 * denom) return 1;
+    }
+    return intersect_count(c1, c2);
+}
+
 // In-plane basis of a circle (arrangement.rs:233-248): u from the helper
 // axis, v = axis x u.
 OG_HD inline void circle_basis(const Cap& c, v3& u, v3& v) {

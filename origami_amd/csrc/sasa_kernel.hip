@@ -75,7 +75,7 @@ struct ArrLds {
     unsigned char hl[MAXC];                          // caps with a hint, ascending
     unsigned short cnt[MAXC];                        // transitions of cap k
     unsigned short seg[MAXC + 1];                    // chunk: segment start per chunk cap
-    double ta[SA_TMAX];                              // chunk: transition angles (push order)
+    alignas(16) double ta[SA_TMAX];                  // chunk: transition angles (push order)
     double tx[SA_TMAX], ty[SA_TMAX], tz[SA_TMAX];    // chunk: transition points
     unsigned char tk[SA_TMAX], tm[SA_TMAX], tq[SA_TMAX];  // chunk cap, other cap, p/q/probe
     unsigned char perm[SA_TMAX];                     // sorted order within segments
@@ -109,25 +109,6 @@ __device__ __forceinline__ Cap cap_at(const ArrLds& L, int k) {
 
 __device__ __forceinline__ Cap as_cap(double4 c) { return {mk(c.x, c.y, c.z), c.w}; }
 
-// any(o != skip_k && contains_strict(cap o, p)) over the nc caps, four caps
-// per step so the LDS loads overlap.  Written with a loop-carried flag: the
-// same loop with early returns from inside the unrolled body gave wrong
-// results (hipcc, ROCm 7.2, gfx950) when inlined into lane-divergent code.
-__device__ __forceinline__ bool any_contains(const ArrLds& L, int nc, int skip_k, v3 p) {
-    int o = 0;
-    bool r = false;
-    for (; o + 4 <= nc && !r; o += 4) {
-        const double4 c0 = L.cap4[o], c1 = L.cap4[o + 1], c2 = L.cap4[o + 2], c3 = L.cap4[o + 3];
-        const bool b0 = o != skip_k && contains_strict(as_cap(c0), p);
-        const bool b1 = o + 1 != skip_k && contains_strict(as_cap(c1), p);
-        const bool b2 = o + 2 != skip_k && contains_strict(as_cap(c2), p);
-        const bool b3 = o + 3 != skip_k && contains_strict(as_cap(c3), p);
-        r = b0 | b1 | b2 | b3;
-    }
-    for (; o < nc && !r; ++o) r = (o != skip_k) && contains_strict(cap_at(L, o), p);
-    return r;
-}
-
 // Lower bound of dot(x, axis_m) over the rotations x of point p about
 // axis_k (ct, st: cosine and sine between the axes) clears cap m's
 // containment test with margin; points near the axis are not trusted.
@@ -135,6 +116,28 @@ __device__ __forceinline__ bool bound_inside(v3 p, const Cap& ck, double ct, dou
     const double h = dot(p, ck.axis);
     const double rn = norm(p - ck.axis * h);
     return rn >= 1e-3 && h * ct - rn * st > ca_m + 1e-11;
+}
+
+// Segment midpoints for the first burial round.  The midpoint is the
+// rotation of point a by ccw/2 about axis n, n*h + r*cos + (n x r)*sin with
+// h = a.n and r = a - n*h, the same point midpoint_on_circle forms; here
+// sine and cosine come from sincosf (a few FP32 ulp, plus the FP32 rounding
+// of the angle), which keeps it within 1e-6 of the exact midpoint.  A cap
+// that contains it with margin 1e-4 therefore also passes the exact test
+// dot(mid, axis) > cos(a) + 1e-12: such a slot is buried with certainty.
+// Slots it does not settle take the exact route.
+__device__ __forceinline__ v3 approx_midpoint(v3 a, double4 n4, double ccw) {
+    const v3 n = mk(n4.x, n4.y, n4.z);
+    const double h = dot(a, n);
+    const v3 r = a - n * h;
+    const v3 t = cross(n, r);
+    float sd, cd;
+    sincosf((float)(0.5 * ccw), &sd, &cd);
+    return n * h + r * (double)cd + t * (double)sd;
+}
+
+__device__ __forceinline__ bool contains_margin(double4 c, v3 m) {
+    return c.x * m.x + c.y * m.y + c.z * m.z > c.w + 1e-4;
 }
 
 __device__ __forceinline__ void add_sasa_energy(const og_sasa_data& d, int ia, double area) {
@@ -282,45 +285,56 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     // ---------------- caps (build_atom_cache + build_caps) ----------------
     // Candidates in ascending atom order; an enclosing neighbour ends the
     // scan (no accessible surface) unless the cap count overflowed first.
+    // Tiles are culled 64 at a time (lane per tile), then the near ones are
+    // scanned in ascending order.
     int nc = 0;
     bool enclosed = false, overflow = false;
-    for (int t = d.tile_begin[w]; t < d.tile_end[w]; ++t) {
-        const double4 tb = d.tile_bound[t];
-        const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
-        const double reach = ri + d.max_radius + tb.w;
-        if (dxc * dxc + dyc * dyc + dzc * dzc > reach * reach) continue;
-        const int j = d.tile_atom0[t] + lane;
-        const bool valid = lane < d.tile_cnt[t] && j != ia;
-        bool is_cap = false, enc = false;
-        v3 axis = mk(0, 0, 0);
-        double ca = 0.0;
-        if (valid) {
-            const v3 pj = ldp(pos, j);
-            const double rj = d.radius[j];
-            const double dd = norm(pj - pi);
-            if (dd <= ri + rj) {
-                if (dd + ri <= rj) {
-                    enc = true;
-                } else if (!(dd + rj <= ri || dd >= ri + rj)) {
-                    ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
-                    if (ca >= -1.0 && ca <= 1.0) {
-                        is_cap = true;
-                        axis = (pj - pi) / dd;
+    const int t_end = d.tile_end[w];
+    for (int tb0 = d.tile_begin[w]; tb0 < t_end && !enclosed && !overflow; tb0 += 64) {
+        bool near = false;
+        if (tb0 + lane < t_end) {
+            const double4 tb = d.tile_bound[tb0 + lane];
+            const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
+            const double reach = ri + d.max_radius + tb.w;
+            near = !(dxc * dxc + dyc * dyc + dzc * dzc > reach * reach);
+        }
+        unsigned long long near_m = __ballot(near);
+        while (near_m) {
+            const int t = tb0 + __builtin_ctzll(near_m);
+            near_m &= near_m - 1;
+            const int j = d.tile_atom0[t] + lane;
+            const bool valid = lane < d.tile_cnt[t] && j != ia;
+            bool is_cap = false, enc = false;
+            v3 axis = mk(0, 0, 0);
+            double ca = 0.0;
+            if (valid) {
+                const v3 pj = ldp(pos, j);
+                const double rj = d.radius[j];
+                const double dd = norm(pj - pi);
+                if (dd <= ri + rj) {
+                    if (dd + ri <= rj) {
+                        enc = true;
+                    } else if (!(dd + rj <= ri || dd >= ri + rj)) {
+                        ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
+                        if (ca >= -1.0 && ca <= 1.0) {
+                            is_cap = true;
+                            axis = (pj - pi) / dd;
+                        }
                     }
                 }
             }
+            unsigned long long m = __ballot(is_cap);
+            const unsigned long long em = __ballot(enc);
+            if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
+            if (nc + popc(m) > MAXC) { overflow = true; break; }
+            if ((m >> lane) & 1ull) {
+                const int p = nc + popc(m & lt);
+                L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
+                L.own[p] = j;
+            }
+            nc += popc(m);
+            if (em) { enclosed = true; break; }
         }
-        unsigned long long m = __ballot(is_cap);
-        const unsigned long long em = __ballot(enc);
-        if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
-        if (nc + popc(m) > MAXC) { overflow = true; break; }
-        if ((m >> lane) & 1ull) {
-            const int p = nc + popc(m & lt);
-            L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
-            L.own[p] = j;
-        }
-        nc += popc(m);
-        if (em) { enclosed = true; break; }
     }
     wave_sync();
     if (overflow) {
@@ -335,24 +349,44 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     // hint[k]: the first cap m that (up to rounding) contains circle k
     // entirely.  It is tested first in the burial tests below (that changes
     // only the order in which an any() is evaluated), and it drives the exact
-    // skip test that follows.
+    // skip test that follows.  Two caps m per step, so that the two
+    // dependent FP64 chains overlap.
     for (int k = lane; k < nc; k += 64) {
         const Cap ck = cap_at(L, k);
-        const double sk = sqrt(fmax(1.0 - ck.ca * ck.ca, 0.0));
+        const double sk2 = fmax(1.0 - ck.ca * ck.ca, 0.0);
         int cnt = 0, hint = 0xFF;
-        for (int m = 0; m < nc; ++m) {
-            if (m == k) continue;
-            const Cap cm = cap_at(L, m);
-            cnt += intersect_count(ck, cm);
+        bool near_par = false;
+        for (int m0 = 0; m0 < nc; m0 += 2) {
+            const int m1 = m0 + 1 < nc ? m0 + 1 : m0;
+            const Cap c0 = cap_at(L, m0), c1 = cap_at(L, m1);
+            const bool v0 = m0 != k, v1 = m0 + 1 < nc && m1 != k;
+            int n0 = intersect_count_fast(ck, c0), n1 = intersect_count_fast(ck, c1);
+            n0 = v0 ? n0 : 0;
+            n1 = v1 ? n1 : 0;
+            const double s0 = dot(ck.axis, c0.axis), s1 = dot(ck.axis, c1.axis);
+            const double d0 = 1.0 - s0 * s0, d1 = 1.0 - s1 * s1;
+            cnt += n0 + n1;
+            near_par = near_par || (n0 > 0 && d0 < 1e-2) || (n1 > 0 && d1 < 1e-2);
             if (hint == 0xFF) {
-                const double ct = dot(ck.axis, cm.axis);
-                const double st = norm(cross(ck.axis, cm.axis));
-                if (ct * ck.ca - st * sk > cm.ca) hint = m;
+                // circle k inside cap m (up to rounding):
+                // cos(axes) cos(a_k) - sin(axes) sin(a_k) > cos(a_m)
+                const double l0 = s0 * ck.ca - c0.ca, l1 = s1 * ck.ca - c1.ca;
+                if (v0 && l0 > 0.0 && l0 * l0 > d0 * sk2) hint = m0;
+                else if (v1 && l1 > 0.0 && l1 * l1 > d1 * sk2) hint = m1;
             }
+        }
+        // skip[k]: 0 keep, 1 test every point below, 2 proven here (see below)
+        int sk8 = 0;
+        if (hint != 0xFF) {
+            const Cap ch = cap_at(L, hint);
+            const double sk = sqrt(sk2);
+            const double ct = dot(ck.axis, ch.axis);
+            const double st = norm(cross(ck.axis, ch.axis));
+            sk8 = (!near_par && sk >= 0.05 && ct * ck.ca - st * sk - ch.ca > 1e-6) ? 2 : 1;
         }
         L.cnt[k] = (unsigned short)cnt;
         L.hint[k] = (unsigned char)hint;
-        L.skip[k] = hint != 0xFF ? 1 : 0;
+        L.skip[k] = (unsigned char)sk8;
     }
     wave_sync();
 
@@ -364,10 +398,18 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     // between the axes).  When that bound clears cap m's containment test by
     // 1e-11 (>> rounding) for every transition point, or for the probe,
     // circle k contributes no arc and is dropped.
+    // skip[k] = 2 settles this without forming the points: the whole circle
+    // (h = cos a_k, rn = sin a_k >= 0.05) clears the test by 1e-6, and every
+    // circle m that meets circle k is at least 5.7 degrees from parallel
+    // (1 - cos^2 >= 1e-2), where intersect_circles keeps |a|, |b| <= 15 and
+    // places its points within 1e-12 of circle k in h and rn (a tangent
+    // point, |c^2| <= 1e-9, within 1e-8 in rn); the probe point is exact to
+    // rounding.  The per-point bound then holds for every point with a
+    // hundredfold margin.  skip[k] = 1 tests every point.
     int n_h = 0;
     for (int base = 0; base < nc; base += 64) {
         const int k = base + lane;
-        const bool hh = k < nc && L.hint[k] != 0xFF;
+        const bool hh = k < nc && L.skip[k] == 1;
         const unsigned long long hm = __ballot(hh);
         if (hh) L.hl[n_h + popc(hm & lt)] = (unsigned char)k;
         n_h += popc(hm);
@@ -475,26 +517,60 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             if (__ballot(empty_seg)) { overflow = true; break; }
         }
 
-        // stable angular sort inside each cap's segment: rank on (angle, push index)
-        for (int x = lane; x < T; x += 64) {
-            const int kl = L.tk[x];
-            const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
-            const double ax = L.ta[x];
-            int r0 = 0, r1 = 0, r2 = 0, r3 = 0;
-            int y = s0;
-            for (; y + 4 <= s1; y += 4) {
-                const double y0 = L.ta[y], y1 = L.ta[y + 1], y2 = L.ta[y + 2], y3 = L.ta[y + 3];
-                r0 += (y0 < ax || (y0 == ax && y < x)) ? 1 : 0;
-                r1 += (y1 < ax || (y1 == ax && y + 1 < x)) ? 1 : 0;
-                r2 += (y2 < ax || (y2 == ax && y + 2 < x)) ? 1 : 0;
-                r3 += (y3 < ax || (y3 == ax && y + 3 < x)) ? 1 : 0;
+        // stable angular sort inside each cap's segment: rank on (angle, push
+        // index).  The fast pass counts smaller angles only; equal angles
+        // then land on one position, which the check below detects, and the
+        // chunk is ranked again with the push index as tie-break.
+        int rk[SA_TMAX / 64];
+#pragma unroll
+        for (int i = 0; i < SA_TMAX / 64; ++i) {
+            const int x = lane + 64 * i;
+            rk[i] = 0;
+            if (x < T) {
+                const int kl = L.tk[x];
+                const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                const double ax = L.ta[x];
+                int n0 = 0, n1 = 0;
+                int y = s0;
+                if ((y & 1) && y < s1) {
+                    n0 += L.ta[y] < ax ? 1 : 0;
+                    ++y;
+                }
+                const double2* t2 = reinterpret_cast<const double2*>(L.ta);
+                for (; y + 8 <= s1; y += 8) {
+                    const double2 v0 = t2[y / 2], v1 = t2[y / 2 + 1], v2 = t2[y / 2 + 2], v3 = t2[y / 2 + 3];
+                    n0 += (v0.x < ax ? 1 : 0) + (v0.y < ax ? 1 : 0) + (v1.x < ax ? 1 : 0) + (v1.y < ax ? 1 : 0);
+                    n1 += (v2.x < ax ? 1 : 0) + (v2.y < ax ? 1 : 0) + (v3.x < ax ? 1 : 0) + (v3.y < ax ? 1 : 0);
+                }
+                for (; y + 2 <= s1; y += 2) {
+                    const double2 v = t2[y / 2];
+                    n0 += (v.x < ax ? 1 : 0) + (v.y < ax ? 1 : 0);
+                }
+                if (y < s1) n1 += L.ta[y] < ax ? 1 : 0;
+                rk[i] = s0 + n0 + n1;
+                L.perm[rk[i]] = (unsigned char)x;
             }
-            for (; y < s1; ++y) {
-                const double ay = L.ta[y];
-                r0 += (ay < ax || (ay == ax && y < x)) ? 1 : 0;
+        }
+        wave_sync();
+        bool tied = false;
+#pragma unroll
+        for (int i = 0; i < SA_TMAX / 64; ++i) {
+            const int x = lane + 64 * i;
+            if (x < T) tied = tied || L.perm[rk[i]] != x;
+        }
+        if (__ballot(tied)) {
+            wave_sync();
+            for (int x = lane; x < T; x += 64) {
+                const int kl = L.tk[x];
+                const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                const double ax = L.ta[x];
+                int r = 0;
+                for (int y = s0; y < s1; ++y) {
+                    const double ay = L.ta[y];
+                    r += (ay < ax || (ay == ax && y < x)) ? 1 : 0;
+                }
+                L.perm[s0 + r] = (unsigned char)x;
             }
-            const int r = r0 + r1 + r2 + r3;
-            if (s0 + r < s1) L.perm[s0 + r] = (unsigned char)x;
         }
         wave_sync();
 
@@ -539,8 +615,10 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
         // test is an any() over the other caps, so it is run in two rounds:
         // (A) every slot tries the likeliest covers (the hint cap, the two
         // caps bounding the segment, and the caps of the four transitions
-        // before it, whose entries may still be open); (B) the slots none of
-        // them covers scan all caps, lane-parallel over a compacted list.
+        // before it, whose entries may still be open) on an approximate
+        // midpoint, with a margin that implies the exact test;
+        // (B) the slots none of them covers form the exact midpoint and scan
+        // all caps, lane-parallel over a compacted list.
         // est[y]: 0 no arc, 1 undecided, 2 arc.
         for (int y = lane; y < T; y += 64) {
             unsigned char st8 = 0;
@@ -559,16 +637,22 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
                     const int bx = L.perm[yn];
                     const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
                     if (!(ccw < 1e-9)) {
-                        const v3 mid = midpoint_on_circle(apt, cap_at(L, kk), ccw / 2.0);
-                        bool bad = (hint != 0xFF && contains_strict(cap_at(L, hint), mid)) ||
-                                   contains_strict(cap_at(L, L.tm[x]), mid) ||
-                                   contains_strict(cap_at(L, L.tm[bx]), mid);
+                        // all seven candidates at once (independent LDS chains)
+                        const v3 mid = approx_midpoint(apt, L.cap4[kk], ccw);
                         const int len = s1 - s0;
-                        for (int t = 1; t <= 4 && !bad; ++t) {
+                        int cand[7];
+                        cand[0] = L.tm[x];
+                        cand[1] = L.tm[bx];
+                        cand[2] = hint != 0xFF ? hint : cand[0];
+#pragma unroll
+                        for (int t = 1; t <= 4; ++t) {
                             int yp = y - t;
                             while (yp < s0) yp += len;
-                            bad = contains_strict(cap_at(L, L.tm[L.perm[yp]]), mid);
+                            cand[2 + t] = L.tm[L.perm[yp]];
                         }
+                        bool bad = false;
+#pragma unroll
+                        for (int c = 0; c < 7; ++c) bad = bad | contains_margin(L.cap4[cand[c]], mid);
                         st8 = bad ? 0 : 1;
                     }
                 }
@@ -585,21 +669,40 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             U += popc(um);
         }
         wave_sync();
-        for (int q = lane; q < U; q += 64) {
-            const int y = L.ulist[q];
-            const int x = L.perm[y];
-            const int kl = L.tk[x];
-            const int kk = L.act[a0 + kl];
-            const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
-            v3 probe = apt;
-            if (L.tq[x] != 2) {
-                const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
-                int yn = y + 1 < s1 ? y + 1 : s0;
-                while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
-                const double ccw = wrap_2pi_positive(L.ta[L.perm[yn]] - L.ta[x]);
-                probe = midpoint_on_circle(apt, cap_at(L, kk), ccw / 2.0);
+        // Round B: exact midpoints lane-parallel, then one slot at a time with
+        // the caps on the lanes (any() over the caps = one ballot).
+        if (U > 0) {
+            const Cap cl0 = cap_at(L, lane < nc ? lane : 0);
+            const Cap cl1 = cap_at(L, lane + 64 < nc ? lane + 64 : 0);
+            for (int qb = 0; qb < U; qb += 64) {
+                const int q = qb + lane;
+                v3 probe = mk(0, 0, 0);
+                int kk = -1;
+                if (q < U) {
+                    const int y = L.ulist[q];
+                    const int x = L.perm[y];
+                    const int kl = L.tk[x];
+                    kk = L.act[a0 + kl];
+                    const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
+                    probe = apt;
+                    if (L.tq[x] != 2) {
+                        const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
+                        int yn = y + 1 < s1 ? y + 1 : s0;
+                        while (L.rm[yn]) yn = yn + 1 < s1 ? yn + 1 : s0;
+                        const double ccw = wrap_2pi_positive(L.ta[L.perm[yn]] - L.ta[x]);
+                        probe = midpoint_on_circle(apt, cap_at(L, kk), ccw / 2.0);
+                    }
+                }
+                const int nq = U - qb < 64 ? U - qb : 64;
+                for (int j = 0; j < nq; ++j) {
+                    const v3 pj = mk(rld(probe.x, j), rld(probe.y, j), rld(probe.z, j));
+                    const int kj = __builtin_amdgcn_readlane(kk, j);
+                    const bool b = (lane < nc && lane != kj && contains_strict(cl0, pj)) ||
+                                   (lane + 64 < nc && lane + 64 != kj && contains_strict(cl1, pj));
+                    const bool buried = __ballot(b) != 0;
+                    if (lane == j) L.est[L.ulist[q]] = buried ? 0 : 2;
+                }
             }
-            L.est[y] = any_contains(L, nc, kk, probe) ? 0 : 2;
         }
         wave_sync();
         // emission in slot order
@@ -750,7 +853,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
 }  // namespace
 
 template <bool AREA>
-__global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_arrange(og_sasa_data d, const double* pos) {
+__global__ void __launch_bounds__(64 * SA_WAVES) __attribute__((amdgpu_waves_per_eu(3))) og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     __shared__ ArrLds lds_all[SA_WAVES];
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);

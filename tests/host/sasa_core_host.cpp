@@ -1,0 +1,48 @@
+// Host build of the PowerSasa device helpers (sasa_core.h), for CPU tests of
+// the shortcuts the arrangement kernel takes: each must give the same
+// decision as the exact arithmetic it stands in for.
+#include "../../origami_amd/csrc/sasa_core.h"
+
+using namespace og_sasa;
+
+extern "C" {
+
+// counts[i] = intersect_count, fast[i] = intersect_count_fast for cap pairs
+// (axis1, ca1, axis2, ca2) packed as 8 doubles each.
+void host_intersect_counts(int n, const double* caps, int* counts, int* fast) {
+    for (int i = 0; i < n; ++i) {
+        const double* c = caps + 8 * i;
+        const Cap a = {mk(c[0], c[1], c[2]), c[3]};
+        const Cap b = {mk(c[4], c[5], c[6]), c[7]};
+        counts[i] = intersect_count(a, b);
+        fast[i] = intersect_count_fast(a, b);
+    }
+}
+
+// Points intersect_circles returns for each pair (6 doubles, NaN when absent)
+void host_intersect_points(int n, const double* caps, double* pts) {
+    for (int i = 0; i < n; ++i) {
+        const double* c = caps + 8 * i;
+        const Cap a = {mk(c[0], c[1], c[2]), c[3]};
+        const Cap b = {mk(c[4], c[5], c[6]), c[7]};
+        v3 p = mk(NAN, NAN, NAN), q = mk(NAN, NAN, NAN);
+        const int r = intersect_circles(a, b, p, q);
+        if (r == 1) q = mk(NAN, NAN, NAN);
+        if (r == 2) p = q = mk(NAN, NAN, NAN);
+        double* o = pts + 6 * i;
+        o[0] = p.x; o[1] = p.y; o[2] = p.z; o[3] = q.x; o[4] = q.y; o[5] = q.z;
+    }
+}
+
+// Exact midpoints (midpoint_on_circle) of point a rotated by ccw/2 about the
+// cap axis; input rows (ax, ay, az, nx, ny, nz, ca, ccw).
+void host_midpoints(int n, const double* in, double* out) {
+    for (int i = 0; i < n; ++i) {
+        const double* r = in + 8 * i;
+        const Cap c = {mk(r[3], r[4], r[5]), r[6]};
+        const v3 m = midpoint_on_circle(mk(r[0], r[1], r[2]), c, r[7] / 2.0);
+        out[3 * i] = m.x; out[3 * i + 1] = m.y; out[3 * i + 2] = m.z;
+    }
+}
+
+}

@@ -1,0 +1,141 @@
+"""CPU checks of the shortcuts the SASA arrangement kernel takes in place of
+origami's exact FP64 arithmetic (sasa_core.h, built here for the host).
+
+Each shortcut must reach the same decision as the arithmetic it replaces
+(arrangement.rs:100-228, geometry.rs:67-100); these tests drive them at and
+around the tolerance thresholds where a mistake would show.
+"""
+import ctypes as C
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def host():
+    d = tempfile.mkdtemp(prefix="og_sasa_host_")
+    so = os.path.join(d, "libsasa_host.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off",
+                    "-o", so, os.path.join(HERE, "host", "sasa_core_host.cpp")], check=True)
+    L = C.CDLL(so)
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.host_intersect_counts.argtypes = [C.c_int, dp, ip, ip]
+    L.host_intersect_points.argtypes = [C.c_int, dp, dp]
+    L.host_midpoints.argtypes = [C.c_int, dp, dp]
+    return L
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def _unit(rng, n):
+    v = rng.normal(size=(n, 3))
+    return v / np.linalg.norm(v, axis=1)[:, None]
+
+
+def _pairs_near(rng, n, targets, sigma_range=(-0.999, 0.999)):
+    """Cap pairs whose exact c^2 = (s1^2 s2^2 - (sigma - c1 c2)^2) / (1 - sigma^2)
+    equals one of the targets."""
+    n1 = _unit(rng, n)
+    sig = rng.uniform(*sigma_range, size=n)
+    w = _unit(rng, n)
+    w -= n1 * np.sum(w * n1, axis=1)[:, None]
+    w /= np.linalg.norm(w, axis=1)[:, None]
+    n2 = n1 * sig[:, None] + w * np.sqrt(1 - sig ** 2)[:, None]
+    c1 = rng.uniform(-0.95, 0.95, size=n)
+    t = rng.choice(targets, size=n)
+    D = 1 - sig ** 2
+    disc = np.maximum(D * ((1 - c1 ** 2) - t), 0)
+    c2 = sig * c1 + rng.choice([-1.0, 1.0], size=n) * np.sqrt(disc)
+    ok = np.abs(c2) < 1
+    return np.concatenate([n1, c1[:, None], n2, c2[:, None]], axis=1)[ok]
+
+
+def test_fast_count_matches_exact(host):
+    rng = np.random.default_rng(0)
+    th = 1e-9
+    targets = np.concatenate([
+        th * np.array([-1.6, -1.5, -1.4, -1.1, -1.0, -0.9, -0.6, -0.5, -0.4, 0.0,
+                       0.4, 0.5, 0.6, 0.9, 1.0, 1.1, 1.4, 1.5, 1.6]),
+        [-1e-3, 1e-3, -0.5, 0.3]])
+    caps = np.concatenate([
+        _pairs_near(rng, 400000, targets),
+        _pairs_near(rng, 100000, targets, sigma_range=(0.99, 0.9999)),   # near-parallel
+        np.concatenate([_unit(rng, 100000), rng.uniform(-1, 1, (100000, 1)),
+                        _unit(rng, 100000), rng.uniform(-1, 1, (100000, 1))], axis=1),
+    ])
+    caps = np.ascontiguousarray(caps)
+    n = len(caps)
+    a = np.zeros(n, np.int32)
+    b = np.zeros(n, np.int32)
+    host.host_intersect_counts(n, _ptr(caps, C.c_double), _ptr(a, C.c_int), _ptr(b, C.c_int))
+    assert np.array_equal(a, b), np.nonzero(a != b)[0][:10]
+    # the thresholds were actually exercised on both sides
+    assert set(np.unique(a)) == {0, 1, 2}
+
+
+def test_points_of_well_conditioned_pairs_stay_on_circle(host):
+    # the fast skip test relies on intersect_circles placing its points on
+    # circle 1 to 1e-12 in height and 1e-8 in radius (sin a >= 0.05) when
+    # 1 - sigma^2 >= 1e-2, tangent points included
+    rng = np.random.default_rng(1)
+    targets = np.array([-0.9e-9, -0.5e-9, 0.0, 0.5e-9, 0.9e-9, 1.1e-9, 1e-6, 1e-3, 0.1, 0.5])
+    caps = _pairs_near(rng, 300000, targets, sigma_range=(-0.99499, 0.99499))
+    caps = caps[np.sqrt(1 - caps[:, 3] ** 2) >= 0.05]
+    caps = np.ascontiguousarray(caps)
+    n = len(caps)
+    pts = np.zeros((n, 6))
+    host.host_intersect_points(n, _ptr(caps, C.c_double), _ptr(pts, C.c_double))
+    n1, c1 = caps[:, 0:3], caps[:, 3]
+    s1 = np.sqrt(1 - c1 ** 2)
+    checked = 0
+    for P in (pts[:, 0:3], pts[:, 3:6]):
+        have = ~np.isnan(P[:, 0])
+        h = np.sum(P[have] * n1[have], axis=1)
+        rn = np.linalg.norm(P[have] - n1[have] * h[:, None], axis=1)
+        assert np.abs(h - c1[have]).max() < 1e-12
+        assert np.abs(rn - s1[have]).max() < 1e-8
+        checked += have.sum()
+    assert checked > 100000
+
+
+def _approx_midpoint(a, n, ccw):
+    # the kernel's approx_midpoint: FP64 rotation with FP32 sine and cosine
+    h = np.sum(a * n, axis=1)
+    r = a - n * h[:, None]
+    t = np.cross(n, r)
+    d = (0.5 * ccw).astype(np.float32)
+    sd = np.sin(d).astype(np.float64)
+    cd = np.cos(d).astype(np.float64)
+    return n * h[:, None] + r * cd[:, None] + t * sd[:, None]
+
+
+def test_approximate_midpoint_within_margin(host):
+    # a cap containing the approximate midpoint with margin 1e-4 must contain
+    # the exact one: the two differ by far less than the margin
+    rng = np.random.default_rng(2)
+    n = 200000
+    nv = _unit(rng, n)
+    ca = rng.uniform(-0.999, 0.999, n)
+    u = _unit(rng, n)
+    u -= nv * np.sum(u * nv, axis=1)[:, None]
+    u /= np.linalg.norm(u, axis=1)[:, None]
+    a = nv * ca[:, None] + u * np.sqrt(1 - ca ** 2)[:, None]
+    ccw = rng.uniform(1e-9, 2 * np.pi, n)
+    ccw[:1000] = 2 * np.pi
+    rows = np.ascontiguousarray(np.concatenate([a, nv, ca[:, None], ccw[:, None]], axis=1))
+    exact = np.zeros((n, 3))
+    host.host_midpoints(n, _ptr(rows, C.c_double), _ptr(exact, C.c_double))
+    err = np.linalg.norm(_approx_midpoint(a, nv, ccw) - exact, axis=1).max()
+    assert err < 1e-6
+    # and so does every cap with that margin: check the implication directly
+    o = _unit(rng, n)
+    co = np.sum(o * _approx_midpoint(a, nv, ccw), axis=1) - rng.uniform(0, 2e-4, n)
+    sure = np.sum(o * _approx_midpoint(a, nv, ccw), axis=1) > co + 1e-4
+    assert np.all(np.sum(o[sure] * exact[sure], axis=1) > co[sure] + 1e-12)
