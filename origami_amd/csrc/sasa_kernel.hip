@@ -49,6 +49,7 @@
 
 #define SA_WAVES 2                    // waves per block (arrangement, scatter)
 #define SA_TMAX 256                   // transitions per chunk (>= 2 * (MAXC - 1))
+#define SA_NC_MAIN 64                 // caps per atom in the main arrangement pass
 
 // per-wave global staging (arcs and vertices of the atom in progress)
 #define WD_ARC 0                      // MAXA x (sx, sy, sz, ex, ey, ez, theta)
@@ -66,15 +67,19 @@ using namespace og_sasa;
 
 namespace {
 
+// Per-wave LDS of the arrangement, for atoms with at most NC caps.  The
+// main pass uses NC = SA_NC_MAIN (2 waves per block, 6 blocks per CU); the
+// atoms with more caps are deferred to a second pass with NC = MAXC.
+template <int NC>
 struct ArrLds {
-    double4 cap4[MAXC];                              // caps: axis, cos(alpha)
-    int own[MAXC];                                   // cap owner (global atom)
-    unsigned char hint[MAXC];                        // a cap that likely buries circle k
-    unsigned char skip[MAXC];                        // circle k provably contributes nothing
-    unsigned char act[MAXC];                         // caps still to process, ascending
-    unsigned char hl[MAXC];                          // caps with a hint, ascending
-    unsigned short cnt[MAXC];                        // transitions of cap k
-    unsigned short seg[MAXC + 1];                    // chunk: segment start per chunk cap
+    double4 cap4[NC];                                // caps: axis, cos(alpha)
+    int own[NC];                                     // cap owner (global atom)
+    unsigned char hint[NC];                          // a cap that likely buries circle k
+    unsigned char skip[NC];                          // circle k provably contributes nothing
+    unsigned char act[NC];                           // caps still to process, ascending
+    unsigned char hl[NC];                            // caps with a hint, ascending
+    unsigned char cnt[NC];                           // transitions of cap k (<= 2 (NC - 1))
+    unsigned short seg[NC + 1];                      // chunk: segment start per chunk cap
     alignas(16) double ta[SA_TMAX];                  // chunk: transition angles (push order)
     double tx[SA_TMAX], ty[SA_TMAX], tz[SA_TMAX];    // chunk: transition points
     unsigned char tk[SA_TMAX], tm[SA_TMAX], tq[SA_TMAX];  // chunk cap, other cap, p/q/probe
@@ -102,7 +107,8 @@ __device__ __forceinline__ double rld(double v, int l) {
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-__device__ __forceinline__ Cap cap_at(const ArrLds& L, int k) {
+template <class LT>
+__device__ __forceinline__ Cap cap_at(const LT& L, int k) {
     const double4 c = L.cap4[k];
     return {mk(c.x, c.y, c.z), c.w};
 }
@@ -163,7 +169,8 @@ __device__ __forceinline__ int wave_min_i(int v) {
 // components (count_accessible_components, arrangement.rs:327-389) capped
 // at L and at least 1.  The chunk arrays of ArrLds are free here and hold
 // the walk's visited flags, the probes' accessibility and their labels.
-__device__ void atom_area(ArrLds& L, const og_sasa_data& d, int ia, double ri, int nc, int na, int lane,
+template <class LT>
+__device__ void atom_area(LT& L, const og_sasa_data& d, int ia, double ri, int nc, int na, int lane,
                           const double* wd, const int* wi) {
     const double two_pi = 2.0 * 3.14159265358979323846;
     unsigned char* visited = L.perm;                                  // [na] (na <= 256)
@@ -273,8 +280,8 @@ __device__ void atom_area(ArrLds& L, const og_sasa_data& d, int ia, double ri, i
 
 // Arrangement of one work atom; returns its pool record (empty when the atom
 // contributes nothing or exceeded capacity).
-template <bool AREA>
-__device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos, int w, int lane,
+template <bool AREA, int NC>
+__device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double* pos, int w, int lane,
                              unsigned long long lt, double* wd, int* wi) {
     const int4 empty = make_int4(0, 0, 0, 0);
     const double two_pi = 2.0 * 3.14159265358979323846;
@@ -326,7 +333,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             unsigned long long m = __ballot(is_cap);
             const unsigned long long em = __ballot(enc);
             if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
-            if (nc + popc(m) > MAXC) { overflow = true; break; }
+            if (nc + popc(m) > NC) { overflow = true; break; }
             if ((m >> lane) & 1ull) {
                 const int p = nc + popc(m & lt);
                 L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
@@ -338,7 +345,11 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
     }
     wave_sync();
     if (overflow) {
-        if (lane == 0) atomicAdd(d.overflow, 1);
+        // more caps than this pass holds: the MAXC pass takes the atom
+        if (lane == 0) {
+            if (NC < MAXC) d.deferred[atomicAdd(d.counters + 3, 1)] = w;
+            else atomicAdd(d.overflow, 1);
+        }
         return empty;
     }
     if (AREA && !enclosed && nc == 0 && lane == 0)
@@ -463,7 +474,7 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
             tb += e;
             ++a1;
         }
-        if (a1 == a0) { overflow = true; break; }   // cannot happen: cnt <= 2 (MAXC - 1)
+        if (a1 == a0) { overflow = true; break; }   // cannot happen: cnt <= 2 (NC - 1)
         const int nck = a1 - a0;
 
         // transitions in push order (cap, then m ascending, p before q); a cap
@@ -852,20 +863,25 @@ __device__ int4 arrange_atom(ArrLds& L, const og_sasa_data& d, const double* pos
 
 }  // namespace
 
-template <bool AREA>
-__global__ void __launch_bounds__(64 * SA_WAVES) __attribute__((amdgpu_waves_per_eu(3))) og_k_sasa_arrange(og_sasa_data d, const double* pos) {
-    __shared__ ArrLds lds_all[SA_WAVES];
+// Main pass (NC = SA_NC_MAIN): every SASA atom; atoms with more caps are
+// listed for the second pass (NC = MAXC), which runs over that list.
+template <bool AREA, int NC>
+__global__ void __launch_bounds__(64 * SA_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
+og_k_sasa_arrange(og_sasa_data d, const double* pos) {
+    __shared__ ArrLds<NC> lds_all[SA_WAVES];
     const int lane = threadIdx.x & 63;
     const int wib = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    ArrLds& L = lds_all[wib];
+    ArrLds<NC>& L = lds_all[wib];
     const int slot = blockIdx.x * SA_WAVES + wib;
     const int n_slots = gridDim.x * SA_WAVES;
     double* wd = d.scratch + (size_t)slot * WD_TOTAL;
     int* wi = d.iscratch + (size_t)slot * WI_TOTAL;
     const unsigned long long lt = lanemask_lt(lane);
-    for (int w = slot; w < d.n_work; w += n_slots) {
-        const int4 rec = AREA ? arrange_atom<true>(L, d, pos, w, lane, lt, wd, wi)
-                              : arrange_atom<false>(L, d, pos, w, lane, lt, wd, wi);
+    const bool second = NC == MAXC;
+    const int n_items = second ? __builtin_amdgcn_readfirstlane(d.counters[3]) : d.n_work;
+    for (int it = slot; it < n_items; it += n_slots) {
+        const int w = second ? d.deferred[it] : it;
+        const int4 rec = arrange_atom<AREA, NC>(L, d, pos, w, lane, lt, wd, wi);
         if (lane == 0) d.rec[w] = rec;
         wave_sync();
     }
@@ -1083,7 +1099,7 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
             return fail("device allocation failed");
     }
     if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 4) ||
-        !zalloc(&d->rec, (size_t)d->n_work))
+        !zalloc(&d->rec, (size_t)d->n_work) || !zalloc(&d->deferred, (size_t)d->n_work))
         return fail("device allocation failed");
     if (d->n_work > 0) {
         if (!zalloc(&d->scratch, (size_t)WD_TOTAL * d->stride) ||
@@ -1100,7 +1116,7 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
 
 void og_sasa_free(og_sasa_data* d) {
     void* p[] = {d->work, d->tile_begin, d->tile_end, d->radius, d->gamma, d->tile_bound,
-                 d->scratch, d->iscratch, d->overflow, d->counters, d->rec,
+                 d->scratch, d->iscratch, d->overflow, d->counters, d->rec, d->deferred,
                  d->arc_atoms, d->arc_par, d->arc_code, d->arc_terms,
                  d->vtx_atoms, d->vtx_sign, d->vtx_code, d->vtx_terms, d->owners,
                  (void*)d->probes, (void*)d->padj_start, (void*)d->padj};
@@ -1118,8 +1134,13 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles), dim3(64), 0, s, pos, d->tile_atom0,
                        d->tile_cnt, d->n_tiles, d->tile_bound);
     hipLaunchKernelGGL(og_k_sasa_reset, dim3(1), dim3(64), 0, s, d->counters);
-    if (area) hipLaunchKernelGGL(og_k_sasa_arrange<true>, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
-    else hipLaunchKernelGGL(og_k_sasa_arrange<false>, dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    if (area) {
+        hipLaunchKernelGGL((og_k_sasa_arrange<true, SA_NC_MAIN>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+        hipLaunchKernelGGL((og_k_sasa_arrange<true, MAXC>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    } else {
+        hipLaunchKernelGGL((og_k_sasa_arrange<false, SA_NC_MAIN>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+        hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    }
     hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
     hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
                        0, s, *d, facc, gamma_scale);

@@ -97,6 +97,46 @@ def test_sasa_forces_c2_generic_geometry(E, c2_system):
     assert rel_rms(f1, fref) < FORCE_RMS_TOL
 
 
+def dense_packing(n, seed, density=0.1, min_sep=1.2):
+    """n points at protein-like density (atoms per A^3) in a ball: interior
+    atoms overlap 65-90 neighbours, more caps than the main arrangement pass
+    holds, so they go through the second (MAXC) pass."""
+    rng = np.random.default_rng(seed)
+    R = (3 * n / (4 * math.pi * density)) ** (1 / 3)
+    pts = []
+    while len(pts) < n:
+        p = rng.uniform(-R, R, 3)
+        if np.linalg.norm(p) > R:
+            continue
+        if pts and np.min(np.linalg.norm(np.asarray(pts) - p, axis=1)) < min_sep:
+            continue
+        pts.append(p)
+    return np.asarray(pts)
+
+
+def test_sasa_dense_packing_second_pass(E, c1_system):
+    s = c1_system
+    pos = dense_packing(s.n_atoms, 7)
+    rad = s.sasa_radius
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    over = ((d < rad[:, None] + rad[None]) & (d > 0))[s.sasa_gamma != 0].sum(1)
+    assert (over > 64).sum() >= 10
+    orc = O.Oracle(s)
+    fs, stats = orc.sasa_forces(pos, 0.25)
+    eng = E.Engine([s])
+    f0 = eng.total_force(pos)
+    f1 = eng.total_force(pos, include_sasa=True, sasa_gamma_scale=0.25)
+    err = np.abs((f1 - f0) - fs).max()
+    print("dense SASA max abs diff %.3e (max |F| %.3e), %d atoms over 64 neighbours, stats %s"
+          % (err, np.abs(fs).max(), (over > 64).sum(), stats))
+    assert stats["atoms_with_arcs"] > 20
+    assert err < 1e-8
+    e_gpu = eng.total_energy(pos, include_sasa=True)[0]["sasa"]
+    total, per_atom = orc.sasa_area(pos)
+    e_ref = float(np.sum(s.sasa_gamma * per_atom))
+    assert abs(e_gpu - e_ref) < 1e-9 * max(1.0, abs(e_ref))
+
+
 def test_sasa_total_force_c2_built_geometry(E, c2_system):
     # The freshly built chain is exactly symmetric, so a handful of cap
     # triple points sit on origami's 1e-12 topology tolerances and single
