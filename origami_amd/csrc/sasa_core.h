@@ -135,6 +135,37 @@ OG_HD int intersect_count_fast(const Cap& c1, const Cap& c2) {
     return intersect_count(c1, c2);
 }
 
+// Union cover test (no counterpart in origami; it only decides which
+// circles can be left out, see the arrangement kernel): the arc of circle k
+// (orthonormal in-plane basis u, v) on which cap m's containment value
+// dot(X, n_m) - cos a_m is at least SA_COV_DELTA, as start angle a and width
+// w2 (radians, FP32).  w2 < 0: no usable arc; w2 = SA_COV_FULL: the whole
+// circle.  FP64 up to the final angles; acosf / atan2f only see arguments
+// where they are well conditioned (|cos| <= 1 - 1e-3, amplitude >= 1e-3),
+// nearly full arcs are cut to width 2 (pi - 0.05) and tiny ones dropped, so
+// both ends lie within 2e-6 rad of the exact arc or inside it.
+#define SA_COV_DELTA 1e-6
+#define SA_COV_FULL 100.0f
+OG_HD inline void cover_arc(const Cap& ck, const Cap& cm, v3 u, v3 v, float& a, float& w2) {
+    const double sk = sqrt(fmax(1.0 - ck.ca * ck.ca, 0.0));
+    const double ct = dot(ck.axis, cm.axis);
+    const double x = dot(cm.axis, u), y = dot(cm.axis, v);
+    const double amp = sk * sqrt(x * x + y * y);
+    const double B = ck.ca * ct - cm.ca;
+    a = 0.0f;
+    w2 = -1.0f;
+    if (B - amp >= SA_COV_DELTA) {
+        w2 = SA_COV_FULL;
+    } else if (amp >= 1e-3) {
+        const double c = (SA_COV_DELTA - B) / amp;
+        if (c < 1.0 - 1e-3) {
+            const float w = c <= -1.0 + 1e-3 ? (float)(3.14159265358979323846 - 0.05) : acosf((float)c);
+            a = atan2f((float)y, (float)x) - w;
+            w2 = 2.0f * w;
+        }
+    }
+}
+
 // In-plane basis of a circle (arrangement.rs:233-248): u from the helper
 // axis, v = axis x u.
 OG_HD inline void circle_basis(const Cap& c, v3& u, v3& v) {

@@ -40,6 +40,7 @@
 // The pool placement depends on which wave allocates first, but every
 // value depends only on the atom's own arrangement, so results are
 // bit-reproducible.
+#include <cstddef>
 #include <string>
 
 #include "og_device.h"
@@ -50,6 +51,9 @@
 #define SA_WAVES 2                    // waves per block (arrangement, scatter)
 #define SA_TMAX 256                   // transitions per chunk (>= 2 * (MAXC - 1))
 #define SA_NC_MAIN 64                 // caps per atom in the main arrangement pass
+#define SA_COV_CAPS (SA_TMAX / 6)     // union cover test: circles per batch (bases in ta)
+#define SA_COV_PAIRS (3 * SA_TMAX)    // union cover test: arcs per batch (in tx..tz)
+#define SA_COV_EPS 1e-5f              // cover margin on angles (rad)
 
 // per-wave global staging (arcs and vertices of the atom in progress)
 #define WD_ARC 0                      // MAXA x (sx, sy, sz, ex, ey, ez, theta)
@@ -88,6 +92,12 @@ struct ArrLds {
     unsigned char est[SA_TMAX];                      // slot state: 0 no arc, 1 undecided, 2 arc
     unsigned char ulist[SA_TMAX];                    // undecided slots
 };
+
+// the union cover test stores its arcs across tx, ty and tz
+static_assert(offsetof(ArrLds<SA_NC_MAIN>, tz) == offsetof(ArrLds<SA_NC_MAIN>, tx) + 2 * SA_TMAX * sizeof(double),
+              "tx, ty, tz must be contiguous");
+static_assert(offsetof(ArrLds<MAXC>, tz) == offsetof(ArrLds<MAXC>, tx) + 2 * SA_TMAX * sizeof(double),
+              "tx, ty, tz must be contiguous");
 
 __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
@@ -144,6 +154,21 @@ __device__ __forceinline__ v3 approx_midpoint(v3 a, double4 n4, double ccw) {
 
 __device__ __forceinline__ bool contains_margin(double4 c, v3 m) {
     return c.x * m.x + c.y * m.y + c.z * m.z > c.w + 1e-4;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Union cover walk: how far past angle p the arc (start e.x, width e.y) reaches
+// when it contains p with SA_COV_EPS to spare, else 0.
+__device__ __forceinline__ float cover_ext(float p, float2 e) {
+    if (!(e.y > 0.0f)) return 0.0f;
+    const float two_pi = 6.28318530717958647692f;
+    float d = p - e.x;
+    d -= two_pi * floorf(d / two_pi);
+    return (d > SA_COV_EPS && d < e.y - SA_COV_EPS) ? e.y - d : 0.0f;
 }
 
 __device__ __forceinline__ void add_sasa_energy(const og_sasa_data& d, int ia, double area) {
@@ -386,14 +411,15 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                 else if (v1 && l1 > 0.0 && l1 * l1 > d1 * sk2) hint = m1;
             }
         }
-        // skip[k]: 0 keep, 1 test every point below, 2 proven here (see below)
-        int sk8 = 0;
+        // skip[k] low bits: 0 keep, 1 test every point below, 2 proven (see
+        // below); bit 2: a circle meeting circle k is nearly parallel to it
+        int sk8 = near_par ? 4 : 0;
         if (hint != 0xFF) {
             const Cap ch = cap_at(L, hint);
             const double sk = sqrt(sk2);
             const double ct = dot(ck.axis, ch.axis);
             const double st = norm(cross(ck.axis, ch.axis));
-            sk8 = (!near_par && sk >= 0.05 && ct * ck.ca - st * sk - ch.ca > 1e-6) ? 2 : 1;
+            sk8 |= (!near_par && sk >= 0.05 && ct * ck.ca - st * sk - ch.ca > 1e-6) ? 2 : 1;
         }
         L.cnt[k] = (unsigned short)cnt;
         L.hint[k] = (unsigned char)hint;
@@ -420,7 +446,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     int n_h = 0;
     for (int base = 0; base < nc; base += 64) {
         const int k = base + lane;
-        const bool hh = k < nc && L.skip[k] == 1;
+        const bool hh = k < nc && (L.skip[k] & 3) == 1;
         const unsigned long long hm = __ballot(hh);
         if (hh) L.hl[n_h + popc(hm & lt)] = (unsigned char)k;
         n_h += popc(hm);
@@ -432,7 +458,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         if (p < n_h * nper) {
             const int k = L.hl[p / nper], mi = p % nper;
             const int h = L.hint[k];
-            if (L.skip[k]) {
+            if (L.skip[k] & 3) {
                 const Cap ck = cap_at(L, k), chint = cap_at(L, h);
                 const double ct = dot(ck.axis, chint.axis);
                 const double st = norm(cross(ck.axis, chint.axis));
@@ -446,16 +472,98 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                     if (r <= 1) ok = bound_inside(P, ck, ct, st, chint.ca);
                     if (r == 0) ok = ok && bound_inside(Q, ck, ct, st, chint.ca);
                 }
-                if (!ok) L.skip[k] = 0;
+                if (!ok) L.skip[k] &= 4;
             }
         }
     }
     wave_sync();
+    // ---------------- union cover test ----------------
+    // A circle k that lies, with margin, inside the union of the other caps
+    // contributes no arc either: every point origami tests on it is then
+    // inside some cap m != k.  On circle k (point X(phi) = cos a_k n_k +
+    // sin a_k (cos(phi) u + sin(phi) v)), cap m's test value is
+    // f_m(phi) = B + sin a_k sin(t) cos(phi - phi_m) with B = cos a_k cos(t)
+    // - cos a_m (t: angle between the axes), so f_m >= SA_COV_DELTA on the
+    // arc |phi - phi_m| <= w_m.  Those arcs are formed in FP64 up to the
+    // angles, which come from atan2f/acosf on well-conditioned arguments
+    // (|cos w_m| <= 1 - 1e-3, sin a_k sin t >= 1e-3; nearly full arcs are
+    // cut to pi - 0.05, tiny ones dropped): each angle is within 2e-6 rad.
+    // A greedy walk from the end of the widest arc then steps, each time, to
+    // the end of the arc that contains the current angle with SA_COV_EPS
+    // (1e-5 rad) to spare and reaches furthest; reaching back into the
+    // widest arc proves the circle covered, every angle then lying inside
+    // some exact arc.  As for the skip test, the points origami forms lie
+    // within 1e-8 of circle k (no nearly parallel partner, sin a_k >= 0.05),
+    // which moves f by less than 2e-8 << SA_COV_DELTA = 1e-6.
+    int n_cand = 0;
+    for (int base = 0; base < nc; base += 64) {
+        const int k = base + lane;
+        bool c = false;
+        if (k < nc && L.skip[k] == 0) {
+            const double ca = L.cap4[k].w;
+            c = 1.0 - ca * ca >= 0.0025;
+        }
+        const unsigned long long cm = __ballot(c);
+        if (c) L.hl[n_cand + popc(cm & lt)] = (unsigned char)k;
+        n_cand += popc(cm);
+    }
+    wave_sync();
+    if (n_cand > 0 && nc > 1) {
+        double* bas = L.ta;                                      // [SA_COV_CAPS] x (u, v)
+        float2* iv = reinterpret_cast<float2*>(L.tx);            // [SA_COV_PAIRS] (start, width)
+        const int per_batch = SA_COV_CAPS < SA_COV_PAIRS / nper ? SA_COV_CAPS : SA_COV_PAIRS / nper;
+        for (int c0 = 0; c0 < n_cand; c0 += per_batch) {
+            const int nb = n_cand - c0 < per_batch ? n_cand - c0 : per_batch;
+            for (int ci = lane; ci < nb; ci += 64) {
+                v3 u, v;
+                circle_basis(cap_at(L, L.hl[c0 + ci]), u, v);
+                double* o = bas + 6 * ci;
+                o[0] = u.x; o[1] = u.y; o[2] = u.z; o[3] = v.x; o[4] = v.y; o[5] = v.z;
+            }
+            wave_sync();
+            for (int p = lane; p < nb * nper; p += 64) {
+                const int ci = p / nper, mi = p % nper;
+                const int k = L.hl[c0 + ci];
+                const int m = mi < k ? mi : mi + 1;
+                const double* o = bas + 6 * ci;
+                float a, wd2;
+                cover_arc(cap_at(L, k), cap_at(L, m), mk(o[0], o[1], o[2]), mk(o[3], o[4], o[5]), a, wd2);
+                iv[p] = make_float2(a, wd2);
+            }
+            wave_sync();
+            for (int ci = 0; ci < nb; ++ci) {
+                const int k = L.hl[c0 + ci];
+                const float2 e0 = lane < nper ? iv[ci * nper + lane] : make_float2(0.0f, -1.0f);
+                const float2 e1 = lane + 64 < nper ? iv[ci * nper + lane + 64] : make_float2(0.0f, -1.0f);
+                bool covered = __ballot(e0.y >= SA_COV_FULL || e1.y >= SA_COV_FULL) != 0;
+                if (!covered) {
+                    const float wmax = wave_max_f(fmaxf(e0.y, e1.y));
+                    if (wmax > 0.0f) {
+                        const unsigned long long b0 = __ballot(e0.y == wmax), b1 = __ballot(e1.y == wmax);
+                        const int l1 = b0 ? __builtin_ctzll(b0) : __builtin_ctzll(b1);
+                        const float a1 = __shfl(b0 ? e0.x : e1.x, l1, 64);
+                        const float start = a1 + wmax, len = 2.0f * 3.14159265f - wmax;
+                        float t = 0.0f;
+                        for (int step = 0; step <= nper + 1; ++step) {
+                            const float p = start + t;
+                            const float best = wave_max_f(fmaxf(cover_ext(p, e0), cover_ext(p, e1)));
+                            if (!(best > 0.0f)) break;
+                            t += best;
+                            if (t >= len + SA_COV_EPS) { covered = true; break; }
+                        }
+                    }
+                }
+                if (covered && lane == 0) L.skip[k] = 2;
+            }
+            wave_sync();
+        }
+    }
+
     // active caps, ascending
     int n_act = 0;
     for (int base = 0; base < nc; base += 64) {
         const int k = base + lane;
-        const bool a = k < nc && !L.skip[k];
+        const bool a = k < nc && !(L.skip[k] & 3);
         const unsigned long long am = __ballot(a);
         if (a) L.act[n_act + popc(am & lt)] = (unsigned char)k;
         n_act += popc(am);

@@ -45,4 +45,23 @@ void host_midpoints(int n, const double* in, double* out) {
     }
 }
 
+// Union cover arcs of circle k by cap m for pairs (axis_k, ca_k, axis_m,
+// ca_m) packed as 8 doubles; out: (start, width) per pair and the basis
+// (u, v) of circle k used.
+void host_cover_arcs(int n, const double* caps, float* arcs, double* basis) {
+    for (int i = 0; i < n; ++i) {
+        const double* c = caps + 8 * i;
+        const Cap k = {mk(c[0], c[1], c[2]), c[3]};
+        const Cap m = {mk(c[4], c[5], c[6]), c[7]};
+        v3 u, v;
+        circle_basis(k, u, v);
+        float a, w2;
+        cover_arc(k, m, u, v, a, w2);
+        arcs[2 * i] = a;
+        arcs[2 * i + 1] = w2;
+        double* b = basis + 6 * i;
+        b[0] = u.x; b[1] = u.y; b[2] = u.z; b[3] = v.x; b[4] = v.y; b[5] = v.z;
+    }
+}
+
 }

@@ -27,6 +27,7 @@ def host():
     L.host_intersect_counts.argtypes = [C.c_int, dp, ip, ip]
     L.host_intersect_points.argtypes = [C.c_int, dp, dp]
     L.host_midpoints.argtypes = [C.c_int, dp, dp]
+    L.host_cover_arcs.argtypes = [C.c_int, dp, C.POINTER(C.c_float), dp]
     return L
 
 
@@ -139,3 +140,109 @@ def test_approximate_midpoint_within_margin(host):
     co = np.sum(o * _approx_midpoint(a, nv, ccw), axis=1) - rng.uniform(0, 2e-4, n)
     sure = np.sum(o * _approx_midpoint(a, nv, ccw), axis=1) > co + 1e-4
     assert np.all(np.sum(o[sure] * exact[sure], axis=1) > co[sure] + 1e-12)
+
+
+def _cover_arcs(host, pairs):
+    pairs = np.ascontiguousarray(pairs)
+    n = len(pairs)
+    arcs = np.zeros((n, 2), np.float32)
+    bas = np.zeros((n, 6))
+    host.host_cover_arcs(n, _ptr(pairs, C.c_double), _ptr(arcs, C.c_float), _ptr(bas, C.c_double))
+    return arcs, bas
+
+
+def test_cover_arcs_inside_exact_arcs(host):
+    # every angle the cover arc claims (SA_COV_EPS inside its ends) has
+    # containment value >= 1e-6 on the exact circle
+    rng = np.random.default_rng(4)
+    n = 200000
+    n1, n2 = _unit(rng, n), _unit(rng, n)
+    c1 = rng.uniform(-0.99, 0.99, n)
+    c2 = rng.uniform(-0.99, 0.99, n)
+    pairs = np.concatenate([n1, c1[:, None], n2, c2[:, None]], axis=1)
+    arcs, bas = _cover_arcs(host, pairs)
+    full = arcs[:, 1] >= 100.0
+    part = (arcs[:, 1] > 0) & ~full
+    assert part.sum() > 50000 and full.sum() > 1000
+    s1 = np.sqrt(1 - c1 ** 2)
+    for sel, frac in ((part, np.linspace(0, 1, 9)), (full, np.linspace(0, 1, 9))):
+        idx = np.nonzero(sel)[0]
+        for f in frac:
+            if sel is full:
+                phi = 2 * np.pi * f * np.ones(len(idx))
+            else:
+                a = arcs[idx, 0].astype(np.float64)
+                w2 = arcs[idx, 1].astype(np.float64)
+                phi = a + 1e-5 + f * (w2 - 2e-5)
+            X = (n1[idx] * c1[idx, None] + s1[idx, None]
+                 * (np.cos(phi)[:, None] * bas[idx, 0:3] + np.sin(phi)[:, None] * bas[idx, 3:6]))
+            val = np.sum(X * n2[idx], axis=1) - c2[idx]
+            assert val.min() > 1e-6 - 1e-9, (f, val.min())
+
+
+def _cover_walk(arcs):
+    """The kernel's greedy cover walk (FP32) over one circle's arcs."""
+    eps = np.float32(1e-5)
+    two_pi = np.float32(2 * np.pi)
+    if np.any(arcs[:, 1] >= 100.0):
+        return True
+    w = arcs[:, 1]
+    if not np.any(w > 0):
+        return False
+    i1 = int(np.argmax(w))
+    start = np.float32(arcs[i1, 0] + w[i1])
+    length = np.float32(two_pi - w[i1])
+    t = np.float32(0)
+    for _ in range(len(arcs) + 2):
+        p = np.float32(start + t)
+        d = (p - arcs[:, 0]).astype(np.float32)
+        d = d - two_pi * np.floor(d / two_pi)
+        ext = np.where((w > 0) & (d > eps) & (d < w - eps), w - d, 0).astype(np.float32)
+        best = ext.max()
+        if not best > 0:
+            return False
+        t = np.float32(t + best)
+        if t >= length + eps:
+            return True
+    return False
+
+
+def test_cover_walk_on_chain_atoms(host):
+    # atoms of a jittered chain: every circle the walk declares covered has
+    # each of 4001 sampled points inside another cap with the 1e-6 margin
+    import math
+    from origami_amd import system as S
+    s = S.build_system(["Lys", "Trp", "Ala", "Leu", "Phe", "Glu"])
+    rng = np.random.default_rng(6)
+    pos = s.pos + rng.normal(scale=0.05, size=s.pos.shape)
+    R = s.sasa_radius
+    covered_n = checked = 0
+    for i in np.nonzero(s.sasa_gamma != 0)[0]:
+        d = np.linalg.norm(pos - pos[i], axis=1)
+        caps = []
+        for j in range(len(pos)):
+            if j == i or d[j] > R[i] + R[j] or d[j] + R[j] <= R[i] or d[j] + R[i] <= R[j]:
+                continue
+            ca = (d[j] ** 2 + R[i] ** 2 - R[j] ** 2) / (2 * d[j] * R[i])
+            caps.append(np.concatenate([(pos[j] - pos[i]) / d[j], [ca]]))
+        caps = np.array(caps)
+        nc = len(caps)
+        for k in range(nc):
+            if 1 - caps[k, 3] ** 2 < 0.0025:
+                continue
+            others = np.array([m for m in range(nc) if m != k])
+            if len(others) == 0:
+                continue
+            pairs = np.concatenate([np.repeat(caps[k:k + 1], len(others), 0), caps[others]], axis=1)
+            arcs, bas = _cover_arcs(host, pairs)
+            if not _cover_walk(arcs):
+                continue
+            covered_n += 1
+            u, v = bas[0, 0:3], bas[0, 3:6]
+            sk = math.sqrt(1 - caps[k, 3] ** 2)
+            phi = np.linspace(0, 2 * np.pi, 4001)
+            X = caps[k, :3] * caps[k, 3] + sk * (np.cos(phi)[:, None] * u + np.sin(phi)[:, None] * v)
+            val = (X @ caps[others, :3].T - caps[others, 3]).max(axis=1)
+            assert val.min() > 1e-6 - 1e-9
+            checked += 1
+    assert checked > 200
