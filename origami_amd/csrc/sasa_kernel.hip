@@ -156,10 +156,9 @@ __device__ __forceinline__ bool contains_margin(double4 c, v3 m) {
     return c.x * m.x + c.y * m.y + c.z * m.z > c.w + 1e-4;
 }
 
-__device__ __forceinline__ float wave_max_f(float v) {
-    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
-}
+// wave-wide maximum (device library reduction over DPP)
+extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
+__device__ __forceinline__ float wave_max_f(float v) { return __ockl_wfred_max_f32(v); }
 
 // Union cover walk: how far past angle p the arc (start e.x, width e.y) reaches
 // when it contains p with SA_COV_EPS to spare, else 0.
@@ -330,18 +329,36 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             const double reach = ri + d.max_radius + tb.w;
             near = !(dxc * dxc + dyc * dyc + dzc * dzc > reach * reach);
         }
+        // near tiles in ascending order, the next tile's atoms loaded while
+        // the current one is processed
         unsigned long long near_m = __ballot(near);
-        while (near_m) {
-            const int t = tb0 + __builtin_ctzll(near_m);
+        int t = -1, j = 0;
+        bool valid = false;
+        v3 pj = mk(0, 0, 0);
+        double rj = 0.0;
+        if (near_m) {
+            t = tb0 + __builtin_ctzll(near_m);
             near_m &= near_m - 1;
-            const int j = d.tile_atom0[t] + lane;
-            const bool valid = lane < d.tile_cnt[t] && j != ia;
+            j = d.tile_atom0[t] + lane;
+            valid = lane < d.tile_cnt[t] && j != ia;
+            if (valid) { pj = ldp(pos, j); rj = d.radius[j]; }
+        }
+        while (t >= 0) {
+            int tn = -1, jn = 0;
+            bool validn = false;
+            v3 pjn = mk(0, 0, 0);
+            double rjn = 0.0;
+            if (near_m) {
+                tn = tb0 + __builtin_ctzll(near_m);
+                near_m &= near_m - 1;
+                jn = d.tile_atom0[tn] + lane;
+                validn = lane < d.tile_cnt[tn] && jn != ia;
+                if (validn) { pjn = ldp(pos, jn); rjn = d.radius[jn]; }
+            }
             bool is_cap = false, enc = false;
             v3 axis = mk(0, 0, 0);
             double ca = 0.0;
             if (valid) {
-                const v3 pj = ldp(pos, j);
-                const double rj = d.radius[j];
                 const double dd = norm(pj - pi);
                 if (dd <= ri + rj) {
                     if (dd + ri <= rj) {
@@ -366,6 +383,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             }
             nc += popc(m);
             if (em) { enclosed = true; break; }
+            t = tn; j = jn; valid = validn; pj = pjn; rj = rjn;
         }
     }
     wave_sync();
@@ -542,7 +560,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                         const unsigned long long b0 = __ballot(e0.y == wmax), b1 = __ballot(e1.y == wmax);
                         const int l1 = b0 ? __builtin_ctzll(b0) : __builtin_ctzll(b1);
                         const float a1 = __shfl(b0 ? e0.x : e1.x, l1, 64);
-                        const float start = a1 + wmax, len = 2.0f * 3.14159265f - wmax;
+                        const float start = a1 + wmax, len = 6.28318530717958647692f - wmax;
                         float t = 0.0f;
                         for (int step = 0; step <= nper + 1; ++step) {
                             const float p = start + t;
