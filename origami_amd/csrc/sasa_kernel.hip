@@ -58,8 +58,7 @@
 // per-wave global staging (arcs and vertices of the atom in progress)
 #define WD_ARC 0                      // MAXA x (sx, sy, sz, ex, ey, ez, theta)
 #define WD_VTX (WD_ARC + 7 * MAXA)    // MAXV x (px, py, pz)
-#define WD_VSG (WD_VTX + 3 * MAXV)    // MAXV root signs
-#define WD_TOTAL (WD_VSG + MAXV)
+#define WD_TOTAL (WD_VTX + 3 * MAXV)
 #define WI_ARC 0                      // MAXA x (cap, full)
 #define WI_VTX (WI_ARC + 2 * MAXA)    // MAXV x (cap in, cap out)
 #define WI_TOTAL (WI_VTX + 2 * MAXV)
@@ -899,25 +898,22 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     }
     // FullyBuried and the other arc-less outcomes contribute no gradient
     if (na == 0) return empty;
-    if (AREA) atom_area(L, d, ia, ri, nc, na, lane, wd, wi);
 
     // ---------------- pool allocation ----------------
+    // issued here; the offsets are first needed after the vertex signs
     int abase = 0, vbase = 0, obase = 0;
     if (lane == 0) {
         abase = atomicAdd(d.counters + 0, na);
         vbase = atomicAdd(d.counters + 1, nv);
         obase = atomicAdd(d.counters + 2, nc);
     }
-    abase = __shfl(abase, 0, 64);
-    vbase = __shfl(vbase, 0, 64);
-    obase = __shfl(obase, 0, 64);
-    if (abase + na > d.arc_cap || vbase + nv > d.vtx_cap || obase + nc > d.own_cap) {
-        if (lane == 0) atomicAdd(d.overflow, 1);
-        return empty;
-    }
-    for (int c = lane; c < nc; c += 64) d.owners[obase + c] = L.own[c];
+    if (AREA) atom_area(L, d, ia, ri, nc, na, lane, wd, wi);
 
     // ---------------- vertex root signs ----------------
+    // signs and the vertex caps stay in LDS (free chunk arrays) for the links
+    double* vsg = L.ta;                                               // [nv] (MAXV = SA_TMAX)
+    unsigned char* vin = L.tk;
+    unsigned char* vout = L.tm;
     for (int V = lane; V < nv; V += 64) {
         const int cin = wi[WI_VTX + 2 * V], cout = wi[WI_VTX + 2 * V + 1];
         const double* Vd = wd + WD_VTX + 3 * V;
@@ -927,9 +923,22 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         if (vertex_point(cap_at(L, cin), cap_at(L, cout), 1.0, vp) &&
             vertex_point(cap_at(L, cin), cap_at(L, cout), -1.0, vm))
             sg = norm2(vp - ref) <= norm2(vm - ref) ? 1.0 : -1.0;
-        wd[WD_VSG + V] = sg;
+        vsg[V] = sg;
+        vin[V] = (unsigned char)cin;
+        vout[V] = (unsigned char)cout;
+    }
+    abase = __shfl(abase, 0, 64);
+    vbase = __shfl(vbase, 0, 64);
+    obase = __shfl(obase, 0, 64);
+    if (abase + na > d.arc_cap || vbase + nv > d.vtx_cap || obase + nc > d.own_cap) {
+        if (lane == 0) atomicAdd(d.overflow, 1);
+        return empty;
+    }
+    for (int c = lane; c < nc; c += 64) d.owners[obase + c] = L.own[c];
+    for (int V = lane; V < nv; V += 64) {
+        const int cin = vin[V], cout = vout[V];
         d.vtx_atoms[vbase + V] = make_int4(ia, L.own[cin], L.own[cout], 0);
-        d.vtx_sign[vbase + V] = sg;
+        d.vtx_sign[vbase + V] = vsg[V];
         d.vtx_code[vbase + V] = 0xFF0000u | ((unsigned)(1 + cout) << 8) | (unsigned)(1 + cin);
     }
     wave_sync();
@@ -968,9 +977,9 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             double ssg = 1.0, esg = 1.0;
             unsigned code = 0xFFFF00u | (unsigned)(1 + kc);
             if (!full && sv >= 0 && ev >= 0) {
-                const int s_in = wi[WI_VTX + 2 * sv], s_out = wi[WI_VTX + 2 * sv + 1];
-                const int e_in = wi[WI_VTX + 2 * ev], e_out = wi[WI_VTX + 2 * ev + 1];
-                const double svs = wd[WD_VSG + sv], evs = wd[WD_VSG + ev];
+                const int s_in = vin[sv], s_out = vout[sv];
+                const int e_in = vin[ev], e_out = vout[ev];
+                const double svs = vsg[sv], evs = vsg[ev];
                 int lc, mc;
                 if (s_in == kc) { lc = s_out; ssg = svs; } else { lc = s_in; ssg = -svs; }
                 if (e_in == kc) { mc = e_out; esg = evs; } else { mc = e_in; esg = -evs; }
