@@ -50,7 +50,7 @@
 
 #define SA_WAVES 2                    // waves per block (arrangement, scatter)
 #define SA_TMAX 256                   // transitions per chunk (>= 2 * (MAXC - 1))
-#define SA_NC_MAIN 64                 // caps per atom in the main arrangement pass
+#define SA_NC_MAIN 80                 // caps per atom in the main arrangement pass
 #define SA_COV_CAPS (SA_TMAX / 6)     // union cover test: circles per batch (bases in ta)
 #define SA_COV_PAIRS (3 * SA_TMAX)    // union cover test: arcs per batch (in tx..tz)
 #define SA_COV_EPS 1e-5f              // cover margin on angles (rad)
@@ -398,28 +398,28 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         add_sasa_energy(d, ia, 4.0 * 3.14159265358979323846 * ri * ri);   // fully exposed
     if (enclosed || nc == 0) return empty;
 
-    // ---------------- per-cap transition counts and burial hints ----------
+    // ---------------- burial hints ----------------
     // hint[k]: the first cap m that (up to rounding) contains circle k
     // entirely.  It is tested first in the burial tests below (that changes
     // only the order in which an any() is evaluated), and it drives the exact
     // skip test that follows.  Two caps m per step, so that the two
-    // dependent FP64 chains overlap.
+    // dependent FP64 chains overlap.  Transition counts are formed later,
+    // for the caps that stay active.
     for (int k = lane; k < nc; k += 64) {
         const Cap ck = cap_at(L, k);
         const double sk2 = fmax(1.0 - ck.ca * ck.ca, 0.0);
-        int cnt = 0, hint = 0xFF;
+        int hint = 0xFF;
         bool near_par = false;
         for (int m0 = 0; m0 < nc; m0 += 2) {
             const int m1 = m0 + 1 < nc ? m0 + 1 : m0;
             const Cap c0 = cap_at(L, m0), c1 = cap_at(L, m1);
             const bool v0 = m0 != k, v1 = m0 + 1 < nc && m1 != k;
-            int n0 = intersect_count_fast(ck, c0), n1 = intersect_count_fast(ck, c1);
-            n0 = v0 ? n0 : 0;
-            n1 = v1 ? n1 : 0;
             const double s0 = dot(ck.axis, c0.axis), s1 = dot(ck.axis, c1.axis);
             const double d0 = 1.0 - s0 * s0, d1 = 1.0 - s1 * s1;
-            cnt += n0 + n1;
-            near_par = near_par || (n0 > 0 && d0 < 1e-2) || (n1 > 0 && d1 < 1e-2);
+            // a nearly parallel circle matters only where it meets circle k
+            if ((v0 && d0 < 1e-2) || (v1 && d1 < 1e-2))
+                near_par = near_par || (v0 && d0 < 1e-2 && intersect_count_fast(ck, c0) > 0) ||
+                           (v1 && d1 < 1e-2 && intersect_count_fast(ck, c1) > 0);
             if (hint == 0xFF) {
                 // circle k inside cap m (up to rounding):
                 // cos(axes) cos(a_k) - sin(axes) sin(a_k) > cos(a_m)
@@ -438,7 +438,6 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             const double st = norm(cross(ck.axis, ch.axis));
             sk8 |= (!near_par && sk >= 0.05 && ct * ck.ca - st * sk - ch.ca > 1e-6) ? 2 : 1;
         }
-        L.cnt[k] = (unsigned short)cnt;
         L.hint[k] = (unsigned char)hint;
         L.skip[k] = (unsigned char)sk8;
     }
@@ -468,30 +467,39 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         if (hh) L.hl[n_h + popc(hm & lt)] = (unsigned char)k;
         n_h += popc(hm);
     }
+    // has[i]: circle hl[i] meets another circle (else its probe point is tested)
+    unsigned char* has = L.act;                       // free until the active list
+    for (int i = lane; i < n_h; i += 64) has[i] = 0;
     wave_sync();
     const int nper = nc > 1 ? nc - 1 : 1;
     for (int base = 0; base < n_h * nper; base += 64) {
         const int p = base + lane;
-        if (p < n_h * nper) {
-            const int k = L.hl[p / nper], mi = p % nper;
-            const int h = L.hint[k];
-            if (L.skip[k] & 3) {
-                const Cap ck = cap_at(L, k), chint = cap_at(L, h);
-                const double ct = dot(ck.axis, chint.axis);
-                const double st = norm(cross(ck.axis, chint.axis));
-                bool ok = true;
-                if (L.cnt[k] == 0) {
-                    if (mi == 0) ok = bound_inside(sample_circle_point(ck), ck, ct, st, chint.ca);
-                } else if (nc > 1) {
-                    const int m = mi < k ? mi : mi + 1;
-                    v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
-                    const int r = intersect_circles(ck, cap_at(L, m), P, Q);
-                    if (r <= 1) ok = bound_inside(P, ck, ct, st, chint.ca);
-                    if (r == 0) ok = ok && bound_inside(Q, ck, ct, st, chint.ca);
-                }
-                if (!ok) L.skip[k] &= 4;
+        if (p < n_h * nper && nc > 1) {
+            const int i = p / nper, mi = p % nper;
+            const int k = L.hl[i];
+            const Cap ck = cap_at(L, k), chint = cap_at(L, L.hint[k]);
+            const double ct = dot(ck.axis, chint.axis);
+            const double st = norm(cross(ck.axis, chint.axis));
+            const int m = mi < k ? mi : mi + 1;
+            v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
+            const int r = intersect_circles(ck, cap_at(L, m), P, Q);
+            bool ok = true;
+            if (r <= 1) {
+                has[i] = 1;
+                ok = bound_inside(P, ck, ct, st, chint.ca);
             }
+            if (r == 0) ok = ok && bound_inside(Q, ck, ct, st, chint.ca);
+            if (!ok) L.skip[k] &= 4;
         }
+    }
+    wave_sync();
+    for (int i = lane; i < n_h; i += 64) {
+        if (has[i]) continue;
+        const int k = L.hl[i];
+        const Cap ck = cap_at(L, k), chint = cap_at(L, L.hint[k]);
+        const double ct = dot(ck.axis, chint.axis);
+        const double st = norm(cross(ck.axis, chint.axis));
+        if (!bound_inside(sample_circle_point(ck), ck, ct, st, chint.ca)) L.skip[k] &= 4;
     }
     wave_sync();
     // ---------------- union cover test ----------------
@@ -586,6 +594,21 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         n_act += popc(am);
     }
     wave_sync();
+    // transition counts of the active caps, lanes over (active cap, other cap)
+    {
+        unsigned* c32 = reinterpret_cast<unsigned*>(L.ta);      // [n_act], free until the first chunk
+        for (int i = lane; i < n_act; i += 64) c32[i] = 0;
+        wave_sync();
+        for (int p = lane; p < n_act * nper && nc > 1; p += 64) {
+            const int i = p / nper, mi = p % nper;
+            const int k = L.act[i];
+            const int n = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1));
+            if (n) atomicAdd(c32 + i, (unsigned)n);
+        }
+        wave_sync();
+        for (int i = lane; i < n_act; i += 64) L.cnt[L.act[i]] = (unsigned char)c32[i];
+        wave_sync();
+    }
 
     // ---------------- boundary (find_boundary), chunk by chunk ------------
     int na = 0, nv = 0;
