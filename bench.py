@@ -67,33 +67,75 @@ def build_config3(seed=3, n_rep=8):
     return [s] * n_rep, E.geometric_ladder(280.0, 395.0, n_rep)
 
 
-def cpu_baseline(systems, temps, dt_fs, gamma_scale, budget_s=12.0):
-    """Time the FP64 oracle (single core) on a bounded sample: full force
+def _oracle_step(O, s, gamma_scale):
+    orc = O.Oracle(s)
+    f = orc.total_force(s.pos)
+    f += orc.sasa_forces(s.pos, gamma_scale)[0]
+    return f
+
+
+def cpu_baseline(systems, temps, dt_fs, gamma_scale, budget_s=12.0, budget_1core_s=6.0):
+    """Time the FP64 oracle on a bounded sample of the batch: full force
     evaluations (bonded + LJ/Coulomb + GB + SASA, all pairs) of chains taken
-    in a fixed shuffled order until the time budget is used, then scale by
-    pair count to one step of the whole batch."""
+    in a fixed shuffled order until the time budget is used, scaled by pair
+    count to one step of the whole batch.  Once on every host core this
+    process may use (one chain per thread; the oracle's C calls release the
+    interpreter lock) and once on a single core."""
+    import concurrent.futures as cf
+    import threading
     from oracle import oracle as O
-    order = np.random.default_rng(0).permutation(len(systems))
+    order = [int(k) for k in np.random.default_rng(0).permutation(len(systems))]
     total_pairs = sum(s.n_atoms * (s.n_atoms - 1) // 2 for s in systems)
-    done_pairs, done_t, done = 0, 0.0, []
+    pairs = lambda k: systems[k].n_atoms * (systems[k].n_atoms - 1) // 2
+
+    def ns_day(step_s):
+        return len(systems) * dt_fs * 1e-6 / step_s * 86400.0
+
+    # single core
+    done1, pairs1, t1 = [], 0, 0.0
     for k in order:
-        s = systems[k]
-        orc = O.Oracle(s)
         t0 = time.perf_counter()
-        f = orc.total_force(s.pos)
-        f += orc.sasa_forces(s.pos, gamma_scale)[0]
-        dt = time.perf_counter() - t0
-        done_pairs += s.n_atoms * (s.n_atoms - 1) // 2
-        done_t += dt
-        done.append(int(k))
-        if done_t >= budget_s:
+        _oracle_step(O, systems[k], gamma_scale)
+        t1 += time.perf_counter() - t0
+        pairs1 += pairs(k)
+        done1.append(k)
+        if t1 >= budget_1core_s:
             break
-    step_s = done_t * total_pairs / max(done_pairs, 1)
-    ns_day = len(systems) * dt_fs * 1e-6 / step_s * 86400.0
-    return {"value": ns_day, "unit": "ns/day", "cores": 1, "kind": "port",
-            "sample": "FP64 oracle force evaluation (bonded + LJ/Coulomb + GB + SASA, all pairs) "
-                      "of %d of %d chains (%d pairs, %.1f s), scaled by pair count to one step of "
-                      "the batch (%.2f s/step)" % (len(done), len(systems), done_pairs, done_t, step_s)}
+    step1 = t1 * total_pairs / max(pairs1, 1)
+    # all cores: workers take chains from the shuffled order until the budget is spent
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, int(os.environ.get("OMP_NUM_THREADS", cores))))
+    lock = threading.Lock()
+    state = {"next": 0, "pairs": 0, "done": 0}
+    t_start = time.perf_counter()
+
+    def worker():
+        while True:
+            with lock:
+                if state["next"] >= len(order) or time.perf_counter() - t_start >= budget_s:
+                    return
+                k = order[state["next"]]
+                state["next"] += 1
+            _oracle_step(O, systems[k], gamma_scale)
+            with lock:
+                state["pairs"] += pairs(k)
+                state["done"] += 1
+
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        for fut in [ex.submit(worker) for _ in range(cores)]:
+            fut.result()
+    wall = time.perf_counter() - t_start
+    step_n = wall * total_pairs / max(state["pairs"], 1)
+    return {"value": ns_day(step_n), "unit": "ns/day", "cores": cores, "kind": "port",
+            "value_1core": ns_day(step1),
+            "sample": "FP64 oracle force evaluation (bonded + LJ/Coulomb + GB + SASA, all pairs) of "
+                      "%d of %d chains on %d threads (%d pairs, %.1f s wall), scaled by pair count to one "
+                      "step of the batch (%.2f s/step); single core: %d chains, %.1f s (%.2f s/step)"
+                      % (state["done"], len(systems), cores, state["pairs"], wall, step_n,
+                         len(done1), t1, step1)}
 
 
 def surrogate_leg(eng, systems, temps, args, dt_fs, gamma_scale, include_sasa, ns_plain, pairs):

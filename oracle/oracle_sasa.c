@@ -133,23 +133,25 @@ static int every_point_buried(const cap_t *caps, int n) {
     return 1;
 }
 
-/* count_accessible_components, arrangement.rs:327-389 */
+/* count_accessible_components, arrangement.rs:327-389.  The golden-spiral
+ * probes are filled once when the library loads (read-only afterwards, so
+ * concurrent callers are safe). */
+enum { PC = 500 };
+static v3 probes[PC];
+
+__attribute__((constructor)) static void init_probes(void) {
+    double golden = M_PI * (3.0 - sqrt(5.0));
+    for (int i = 0; i < PC; ++i) {
+        double z = 1.0 - (2.0 * i + 1.0) / (double)PC;
+        double t = 1.0 - z * z;
+        double r = sqrt(t > 0.0 ? t : 0.0);
+        double th = golden * i;
+        probes[i] = mk(r * cos(th), r * sin(th), z);
+    }
+}
+
 static int count_accessible_components(const cap_t *caps, int n) {
     if (n == 0) return 1;
-    enum { PC = 500 };
-    static v3 probes[PC];
-    static int probes_ready = 0;
-    if (!probes_ready) {
-        double golden = M_PI * (3.0 - sqrt(5.0));
-        for (int i = 0; i < PC; ++i) {
-            double z = 1.0 - (2.0 * i + 1.0) / (double)PC;
-            double t = 1.0 - z * z;
-            double r = sqrt(t > 0.0 ? t : 0.0);
-            double th = golden * i;
-            probes[i] = mk(r * cos(th), r * sin(th), z);
-        }
-        probes_ready = 1;
-    }
     int acc[PC], parent[PC];
     int any_acc = 0;
     for (int i = 0; i < PC; ++i) {

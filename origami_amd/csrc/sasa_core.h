@@ -1,6 +1,9 @@
-// PowerSasa per-atom force routine shared by the GPU kernel (sasa_kernel.hip)
-// and its host build (used only to test the kernel logic on the CPU).
-// See sasa_kernel.hip for the algorithm description and origami citations.
+// PowerSasa geometry and derivative primitives (FP64), shared by the GPU
+// kernels (sasa_kernel.hip) and a host build that tests them on the CPU
+// (tests/host).  See sasa_kernel.hip for the algorithm and origami citations.
+
+// MAXC caps per atom (deeper-buried atoms are reported as overflow); MAXA
+// arcs and MAXV vertices per atom.
 #pragma once
 #include <math.h>
 #include <stddef.h>
@@ -19,24 +22,8 @@
 #endif
 
 #define MAXC 128
-#define MAXT (2 * MAXC)
 #define MAXA 256
 #define MAXV 256
-
-// FP64 scratch layout (element offsets)
-#define S_CAP 0                              // MAXC x (ax, ay, az, cos)
-#define S_TR (S_CAP + 4 * MAXC)              // MAXT x (ang, px, py, pz)
-#define S_ARC (S_TR + 4 * MAXT)              // MAXA x (sx, sy, sz, ex, ey, ez, theta)
-#define S_VTX (S_ARC + 7 * MAXA)             // MAXV x (px, py, pz, sign)
-#define S_GA (S_VTX + 4 * MAXV)              // (MAXC+1) x 3 arc-term sums
-#define S_GV (S_GA + 3 * (MAXC + 1))         // (MAXC+1) x 3 vertex-term sums
-#define S_TOTAL (S_GV + 3 * (MAXC + 1))
-// int scratch layout
-#define I_OWN 0                              // MAXC cap owners (global atom)
-#define I_TRO (I_OWN + MAXC)                 // MAXT transition "other cap"
-#define I_ARC (I_TRO + MAXT)                 // MAXA x (cap, full, sv, ev)
-#define I_VTX (I_ARC + 4 * MAXA)             // MAXV x (in, out)
-#define I_TOTAL (I_VTX + 2 * MAXV)
 
 namespace og_sasa {
 
@@ -55,23 +42,6 @@ OG_HD inline double norm(v3 a) { return sqrt(dot(a, a)); }
 OG_HD inline v3 normalize(v3 a) { return a / norm(a); }
 
 struct Cap { v3 axis; double ca; };
-
-struct Scr {
-    double* d;
-    int* i;
-    int stride, t;
-    OG_HD double& D(int k) const { return d[(size_t)k * stride + t]; }
-    OG_HD int& I(int k) const { return i[(size_t)k * stride + t]; }
-    OG_HD Cap cap(int k) const {
-        return {mk(D(S_CAP + 4 * k), D(S_CAP + 4 * k + 1), D(S_CAP + 4 * k + 2)), D(S_CAP + 4 * k + 3)};
-    }
-    OG_HD void set_cap(int k, Cap c) const {
-        D(S_CAP + 4 * k) = c.axis.x; D(S_CAP + 4 * k + 1) = c.axis.y; D(S_CAP + 4 * k + 2) = c.axis.z;
-        D(S_CAP + 4 * k + 3) = c.ca;
-    }
-    OG_HD v3 v(int off) const { return mk(D(off), D(off + 1), D(off + 2)); }
-    OG_HD void setv(int off, v3 p) const { D(off) = p.x; D(off + 1) = p.y; D(off + 2) = p.z; }
-};
 
 OG_HD inline Cap small_circle_new(v3 axis, double ca) { return {normalize(axis), ca}; }
 OG_HD inline bool contains_strict(const Cap& c, v3 p) { return dot(p, c.axis) > c.ca + 1e-12; }
@@ -401,242 +371,3 @@ OG_HD inline v3 unit(int axis) { return mk(axis == 0, axis == 1, axis == 2); }
 
 }  // namespace og_sasa
 
-
-namespace og_sasa {
-
-// Analytical SASA force contribution of one work atom.  emit(atom, axis, f)
-// receives F_atom[axis] += f for every affected atom.  *overflow_flag is set
-// (and nothing is emitted) when the arrangement exceeds MAXC/MAXA/MAXV.
-template <class Emit>
-OG_HD void atom_forces(const Scr& S, int work_atom, const double* pos, const double* radius,
-                       int sys_begin, int sys_end, double gam, int* overflow_flag, Emit emit) {
-
-    const int ia = work_atom;
-    const v3 pi = ldp(pos, ia);
-    const double ri = radius[ia];
-    if (gam == 0.0) return;
-
-    // ---- caps from overlapping neighbours (build_atom_cache + build_caps) ----
-    int nc = 0;
-    for (int j = sys_begin; j < sys_end; ++j) {
-        if (j == ia) continue;
-        const v3 pj = ldp(pos, j);
-        const double rj = radius[j];
-        const double dd = norm(pj - pi);
-        if (!(dd <= ri + rj)) continue;
-        if (dd + ri <= rj) return;               // enclosed: no accessible surface
-        if (dd + rj <= ri || dd >= ri + rj) continue;
-        const double ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
-        if (!(ca >= -1.0 && ca <= 1.0)) continue;
-        if (nc >= MAXC) { *overflow_flag = 1; return; }
-        S.set_cap(nc, {(pj - pi) / dd, ca});
-        S.I(I_OWN + nc) = j;
-        ++nc;
-    }
-    if (nc == 0) return;                         // fully exposed: constant area
-
-    // ---- boundary arcs and vertices (find_boundary) ----
-    int n_arcs = 0, n_verts = 0;
-    for (int k = 0; k < nc; ++k) {
-        const Cap ck = S.cap(k);
-        int m = 0;
-        for (int o = 0; o < nc; ++o) {
-            if (o == k) continue;
-            v3 p, q;
-            const int r = intersect_circles(ck, S.cap(o), p, q);
-            if (r == 2) continue;
-            S.D(S_TR + 4 * m) = angular_position(p, ck);
-            S.setv(S_TR + 4 * m + 1, p);
-            S.I(I_TRO + m) = o;
-            ++m;
-            if (r == 0) {
-                S.D(S_TR + 4 * m) = angular_position(q, ck);
-                S.setv(S_TR + 4 * m + 1, q);
-                S.I(I_TRO + m) = o;
-                ++m;
-            }
-        }
-        // stable insertion sort by angle
-        for (int x = 1; x < m; ++x) {
-            const double ang = S.D(S_TR + 4 * x);
-            const v3 pt = S.v(S_TR + 4 * x + 1);
-            const int oth = S.I(I_TRO + x);
-            int y = x - 1;
-            while (y >= 0 && S.D(S_TR + 4 * y) > ang) {
-                S.D(S_TR + 4 * (y + 1)) = S.D(S_TR + 4 * y);
-                S.setv(S_TR + 4 * (y + 1) + 1, S.v(S_TR + 4 * y + 1));
-                S.I(I_TRO + y + 1) = S.I(I_TRO + y);
-                --y;
-            }
-            S.D(S_TR + 4 * (y + 1)) = ang;
-            S.setv(S_TR + 4 * (y + 1) + 1, pt);
-            S.I(I_TRO + y + 1) = oth;
-        }
-        // dedup consecutive coincident points (keep the first)
-        int w = 0;
-        for (int x = 0; x < m; ++x) {
-            const v3 pt = S.v(S_TR + 4 * x + 1);
-            if (w > 0 && norm2(pt - S.v(S_TR + 4 * (w - 1) + 1)) < 1e-12) continue;
-            if (w != x) {
-                S.D(S_TR + 4 * w) = S.D(S_TR + 4 * x);
-                S.setv(S_TR + 4 * w + 1, pt);
-                S.I(I_TRO + w) = S.I(I_TRO + x);
-            }
-            ++w;
-        }
-        m = w;
-        if (m == 0) {
-            const v3 probe = sample_circle_point(ck);
-            bool buried = false;
-            for (int o = 0; o < nc && !buried; ++o) buried = (o != k) && contains_strict(S.cap(o), probe);
-            if (!buried) {
-                if (n_arcs >= MAXA) { *overflow_flag = 1; return; }
-                const int A = n_arcs++;
-                S.setv(S_ARC + 7 * A, probe);
-                S.setv(S_ARC + 7 * A + 3, probe);
-                S.D(S_ARC + 7 * A + 6) = -2.0 * 3.14159265358979323846;
-                S.I(I_ARC + 4 * A) = k;
-                S.I(I_ARC + 4 * A + 1) = 1;
-            }
-            continue;
-        }
-        for (int x = 0; x < m; ++x) {
-            const int y = (x + 1) % m;
-            const double ccw = wrap_2pi_positive(S.D(S_TR + 4 * y) - S.D(S_TR + 4 * x));
-            if (ccw < 1e-9) continue;
-            const v3 apt = S.v(S_TR + 4 * x + 1);
-            const v3 mid = midpoint_on_circle(apt, ck, ccw / 2.0);
-            bool bad = false;
-            for (int o = 0; o < nc && !bad; ++o) bad = (o != k) && contains_strict(S.cap(o), mid);
-            if (bad) continue;
-            if (n_arcs >= MAXA || n_verts >= MAXV) { *overflow_flag = 1; return; }
-            const int A = n_arcs++;
-            S.setv(S_ARC + 7 * A, S.v(S_TR + 4 * y + 1));
-            S.setv(S_ARC + 7 * A + 3, apt);
-            S.D(S_ARC + 7 * A + 6) = -ccw;
-            S.I(I_ARC + 4 * A) = k;
-            S.I(I_ARC + 4 * A + 1) = 0;
-            const int V = n_verts++;
-            S.setv(S_VTX + 4 * V, apt);
-            S.I(I_VTX + 2 * V) = k;
-            S.I(I_VTX + 2 * V + 1) = S.I(I_TRO + x);
-        }
-    }
-    // FullyBuried (no arcs and every probe buried) and the remaining
-    // no-arc case both contribute nothing to the gradient.
-    if (n_arcs == 0) return;
-
-    // ---- cache: vertex root signs, arc -> vertex links ----
-    for (int V = 0; V < n_verts; ++V) {
-        const Cap cin = S.cap(S.I(I_VTX + 2 * V)), cout = S.cap(S.I(I_VTX + 2 * V + 1));
-        const v3 ref = S.v(S_VTX + 4 * V);
-        v3 vp, vm;
-        double sg = 1.0;
-        if (vertex_point(cin, cout, 1.0, vp) && vertex_point(cin, cout, -1.0, vm))
-            sg = norm2(vp - ref) <= norm2(vm - ref) ? 1.0 : -1.0;
-        S.D(S_VTX + 4 * V + 3) = sg;
-    }
-    for (int A = 0; A < n_arcs; ++A) {
-        int sv = -1, ev = -1;
-        if (!S.I(I_ARC + 4 * A + 1)) {
-            const v3 st = S.v(S_ARC + 7 * A), en = S.v(S_ARC + 7 * A + 3);
-            for (int V = 0; V < n_verts && sv < 0; ++V)
-                if (norm2(S.v(S_VTX + 4 * V) - st) < 1e-10) sv = V;
-            for (int V = 0; V < n_verts && ev < 0; ++V)
-                if (norm2(S.v(S_VTX + 4 * V) - en) < 1e-10) ev = V;
-        }
-        S.I(I_ARC + 4 * A + 2) = sv;
-        S.I(I_ARC + 4 * A + 3) = ev;
-    }
-
-    // ---- gradient: per (affected atom, axis) arc and vertex sums ----
-    // affected local index: 0 = atom i, 1 + cap index = that cap's owner
-    for (int k = 0; k < 3 * (nc + 1); ++k) {
-        S.D(S_GA + k) = 0.0;
-        S.D(S_GV + k) = 0.0;
-    }
-    const v3 zero = mk(0, 0, 0);
-    for (int A = 0; A < n_arcs; ++A) {
-        const int kc = S.I(I_ARC + 4 * A);
-        const int ka = S.I(I_OWN + kc);
-        const v3 pk = ldp(pos, ka);
-        const double rk = radius[ka];
-        const double theta = S.D(S_ARC + 7 * A + 6);
-        const double dik = norm(pk - pi);
-        const double cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
-        const int sv = S.I(I_ARC + 4 * A + 2), ev = S.I(I_ARC + 4 * A + 3);
-        const bool simple = S.I(I_ARC + 4 * A + 1) || sv < 0 || ev < 0;
-        int lc = -1, mc = -1;
-        double ssg = 1.0, esg = 1.0;
-        if (!simple) {
-            const int vin = S.I(I_VTX + 2 * sv), vout = S.I(I_VTX + 2 * sv + 1);
-            if (vin == kc) { lc = vout; ssg = S.D(S_VTX + 4 * sv + 3); }
-            else { lc = vin; ssg = -S.D(S_VTX + 4 * sv + 3); }
-            const int win = S.I(I_VTX + 2 * ev), wout = S.I(I_VTX + 2 * ev + 1);
-            if (win == kc) { mc = wout; esg = S.D(S_VTX + 4 * ev + 3); }
-            else { mc = win; esg = -S.D(S_VTX + 4 * ev + 3); }
-        }
-        // distinct atoms touched by this arc: i (local 0), k, l, m (local 1 + cap)
-        int roles[4] = {0, 1 + kc, lc >= 0 ? 1 + lc : -1, mc >= 0 ? 1 + mc : -1};
-        const int la = lc >= 0 ? S.I(I_OWN + lc) : -1, ma = mc >= 0 ? S.I(I_OWN + mc) : -1;
-        const v3 pl = la >= 0 ? ldp(pos, la) : zero, pm = ma >= 0 ? ldp(pos, ma) : zero;
-        const double rl = la >= 0 ? radius[la] : 0.0, rm = ma >= 0 ? radius[ma] : 0.0;
-        for (int r = 0; r < 4; ++r) {
-            const int x = roles[r];
-            if (x < 0) continue;
-            bool seen = false;
-            for (int q = 0; q < r; ++q) seen = seen || (roles[q] == x);
-            if (seen) continue;
-            for (int axis = 0; axis < 3; ++axis) {
-                const v3 b = unit(axis);
-                const v3 dri = x == 0 ? b : zero;
-                const v3 drk = x == 1 + kc ? b : zero;
-                const double dcos = cos_alpha_directional(pi, pk, ri, rk, dri, drk);
-                double term;
-                if (simple) {
-                    term = theta * dcos;
-                } else {
-                    const v3 drl = x == 1 + lc ? b : zero;
-                    const v3 drm = x == 1 + mc ? b : zero;
-                    double dth = 0.0;
-                    if (!arc_theta_jvp(pi, pk, pl, pm, ri, rk, rl, rm, ssg, esg, dri, drk, drl, drm, dth))
-                        dth = 0.0;
-                    term = theta * dcos + cak * dth;
-                }
-                S.D(S_GA + 3 * x + axis) += term;
-            }
-        }
-    }
-    for (int V = 0; V < n_verts; ++V) {
-        const int kcap = S.I(I_VTX + 2 * V), lcap = S.I(I_VTX + 2 * V + 1);
-        const int ka = S.I(I_OWN + kcap), la = S.I(I_OWN + lcap);
-        const v3 pk = ldp(pos, ka), pl = ldp(pos, la);
-        const double rk = radius[ka], rl = radius[la];
-        const double sg = S.D(S_VTX + 4 * V + 3);
-        const int roles[3] = {0, 1 + kcap, 1 + lcap};
-        for (int r = 0; r < 3; ++r) {
-            const int x = roles[r];
-            for (int axis = 0; axis < 3; ++axis) {
-                const v3 b = unit(axis);
-                const v3 dri = x == 0 ? b : zero;
-                const v3 drk = x == 1 + kcap ? b : zero;
-                const v3 drl = x == 1 + lcap ? b : zero;
-                double de = 0.0;
-                if (!vertex_epsilon_jvp(pi, pk, pl, ri, rk, rl, sg, dri, drk, drl, de)) de = 0.0;
-                S.D(S_GV + 3 * x + axis) += de;
-            }
-        }
-    }
-    // ---- scatter F_x -= gamma_i * R_i^2 * (-sum_arc - sum_vert) ----
-    const double r_sq = ri * ri;
-    for (int x = 0; x <= nc; ++x) {
-        const int atom = x == 0 ? ia : S.I(I_OWN + x - 1);
-        for (int axis = 0; axis < 3; ++axis) {
-            const double g = (-S.D(S_GA + 3 * x + axis) - S.D(S_GV + 3 * x + axis)) * r_sq;
-            const double f = -(g * gam);
-            if (f != 0.0) emit(atom, axis, f);
-        }
-    }
-}
-
-}  // namespace og_sasa
