@@ -51,7 +51,7 @@ static hipError_t upload(T** dst, const std::vector<T>& src) {
 }
 
 struct og_engine {
-    int n_sys = 0, n_atoms = 0, n_tiles = 0, n_work = 0, n_masked = 0, n_clusters = 0;
+    int n_sys = 0, n_atoms = 0, n_tiles = 0, n_work = 0, n_masked = 0, n_clusters = 0, n_clusters_small = 0;
     long long n_pairs = 0;
     int n_sasa_work = 0;
     std::vector<int> atom_off, sys_n;
@@ -75,7 +75,7 @@ struct og_engine {
     double *bond_p = nullptr, *angle_p = nullptr, *dih_p = nullptr, *imp_p = nullptr;
     double* bonded_tf = nullptr;                    // per-(term, atom) force slots
     int *bonded_csr_start = nullptr, *bonded_csr_slot = nullptr;
-    int *cl_start = nullptr, *cl_atoms = nullptr, *cn_start = nullptr, *cn_pair = nullptr;
+    int *cl_start = nullptr, *cl_atoms = nullptr, *cn_start = nullptr;
     double* cn_len = nullptr;
     double *sys_temp = nullptr, *sys_dofc = nullptr;
     int *shake_fail = nullptr, *diverged = nullptr, *clamped = nullptr;
@@ -156,7 +156,7 @@ static void free_all(og_engine* e) {
                     e->atom_slot, e->par, e->gb, e->qs, e->tile_atom0, e->tile_cnt, e->work,
                     e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
                     e->bonded_tf, e->bonded_csr_start, e->bonded_csr_slot,
-                    e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start, e->cn_pair,
+                    e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start,
                     e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
     for (void* p : ptrs)
         if (p) hipFree(p);
@@ -266,7 +266,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<float> qs;
     std::vector<int> bonds, angles, dih, imp;
     std::vector<double> bond_p, angle_p, dih_p, imp_p;
-    std::vector<int> cl_start{0}, cl_atoms, cn_start{0}, cn_pair;
+    std::vector<int> cl_start{0}, cl_atoms, cn_start{0};
     std::vector<double> cn_len;
     e->n_cons_sys.resize(n_sys);
 
@@ -349,8 +349,9 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
             imp_p.push_back(d.improper_params[2 * k]); imp_p.push_back(d.improper_params[2 * k + 1]);
         }
         // SHAKE clusters: each constraint joins its heavy atom's cluster; every
-        // other atom is a singleton.  Constraint order within a cluster follows
-        // the caller's (origami's sorted bond) order.
+        // other atom is a singleton.  A cluster lists its heavy atom first,
+        // then the hydrogen of each of its constraints, the constraints in the
+        // caller's (origami's sorted bond) order.
         std::vector<int> owner(n, -1);  // heavy atom of an H, or itself
         std::vector<std::vector<int>> cons_of(n);
         for (int k = 0; k < d.n_constraints; ++k) {
@@ -376,13 +377,15 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
             if (members[k].empty()) continue;
             if ((int)members[k].size() > 8)
                 return fail(OG_ERR_ARG, "system %d: atom %d has more than 7 constrained partners", s, k);
-            for (int m : members[k]) cl_atoms.push_back(off + m);
-            cl_start.push_back((int)cl_atoms.size());
+            cl_atoms.push_back(off + k);
             for (int c : cons_of[k]) {
-                cn_pair.push_back(off + d.constraints[2 * c]);
-                cn_pair.push_back(off + d.constraints[2 * c + 1]);
+                const int a0 = d.constraints[2 * c], b0 = d.constraints[2 * c + 1];
+                cl_atoms.push_back(off + (a0 == k ? b0 : a0));
                 cn_len.push_back(d.constraint_len[c]);
             }
+            if ((int)cl_atoms.size() - cl_start.back() != (int)members[k].size())
+                return fail(OG_ERR_ARG, "system %d: atom %d carries a repeated constraint", s, k);
+            cl_start.push_back((int)cl_atoms.size());
             cn_start.push_back((int)cn_len.size());
         }
     }
@@ -390,6 +393,24 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     e->n_work = (int)work.size();
     e->n_masked = (int)(masks.size() / 128);
     e->n_clusters = (int)cl_start.size() - 1;
+    // clusters of at most OG_SMALL_CL atoms first (the narrow cluster kernel)
+    {
+        std::vector<int> order(e->n_clusters), s_start{0}, s_atoms, s_cstart{0};
+        std::vector<double> s_len;
+        for (int c = 0; c < e->n_clusters; ++c) order[c] = c;
+        std::stable_partition(order.begin(), order.end(), [&](int c) {
+            return cl_start[c + 1] - cl_start[c] <= OG_SMALL_CL;
+        });
+        e->n_clusters_small = 0;
+        for (int c : order) {
+            if (cl_start[c + 1] - cl_start[c] <= OG_SMALL_CL) e->n_clusters_small++;
+            for (int q = cl_start[c]; q < cl_start[c + 1]; ++q) s_atoms.push_back(cl_atoms[q]);
+            for (int q = cn_start[c]; q < cn_start[c + 1]; ++q) s_len.push_back(cn_len[q]);
+            s_start.push_back((int)s_atoms.size());
+            s_cstart.push_back((int)s_len.size());
+        }
+        cl_start.swap(s_start); cl_atoms.swap(s_atoms); cn_start.swap(s_cstart); cn_len.swap(s_len);
+    }
     e->nb = (int)bond_p.size() / 2;
     e->na = (int)angle_p.size() / 2;
     e->nd = (int)dih_p.size() / 3;
@@ -459,7 +480,6 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->cl_start, cl_start));
     HIPCHK(upload(&e->cl_atoms, cl_atoms));
     HIPCHK(upload(&e->cn_start, cn_start));
-    HIPCHK(upload(&e->cn_pair, cn_pair));
     HIPCHK(upload(&e->cn_len, cn_len));
     std::vector<double> z_sys(n_sys, 0.0), o_sys(n_sys, 1.0);
     HIPCHK(upload(&e->sys_temp, z_sys));
@@ -769,7 +789,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     c.pos = e->pos; c.vel = e->vel; c.ref = e->ref; c.force = e->force; c.facc = e->facc;
     c.iacc = e->iacc; c.mass = e->mass; c.atom_sys = e->atom_sys; c.atom_local = e->atom_local;
     c.cl_start = e->cl_start; c.cl_atoms = e->cl_atoms; c.cn_start = e->cn_start;
-    c.cn_pair = e->cn_pair; c.cn_len = e->cn_len; c.n_clusters = e->n_clusters;
+    c.cn_len = e->cn_len; c.n_clusters = e->n_clusters; c.n_small = e->n_clusters_small;
     c.sys_temp = e->sys_temp; c.sys_dofcorr = e->sys_dofc; c.shake_fail = e->shake_fail;
     c.diverged = e->diverged; c.dt = p.dt_fs; c.alpha = e->alpha;
     c.use_shake = p.constrain_h_bonds ? 1 : 0;

@@ -310,99 +310,126 @@ __device__ void noise3(uint32_t k0, uint32_t k1, uint32_t sys, uint32_t atom, ui
 // B, A, SHAKE, O, A, SHAKE; also zeroes this cluster's force and integral
 // accumulators for the force evaluation that follows.
 // ---------------------------------------------------------------------------
-#define OG_MAX_CL 8
 
-__device__ int shake_cluster(const og_cluster_args& a, int c, double* xs, const double* rs,
-                             const int* at, int na) {
-    const int k0 = a.cn_start[c], k1 = a.cn_start[c + 1];
-    if (k1 == k0) return 1;
+// SHAKE on one X-H cluster (shake.rs:35-125): local atom 0 is the heavy
+// atom X and local atom m + 1 the hydrogen of the cluster's constraint m,
+// the constraints in origami's sorted bond order (Gauss-Seidel).  Every
+// constraint is taken as (X, H); for a bond given as (H, X) that flips the
+// sign of both the current and the reference pair difference, which leaves
+// err, denom and lambda unchanged and flips the correction, so each update is
+// bit-identical to the given orientation.  All indices are compile-time, so
+// the cluster lives in registers.
+template <int MC>
+__device__ __forceinline__ bool shake_cluster(double (&xs)[3 * MC], const double (&rs)[3 * MC],
+                                              const double (&im)[MC], const double (&dl)[MC], int nc) {
+    if (nc == 0) return true;
     for (int iter = 1; iter <= 64; ++iter) {
         double max_err = 0.0;
-        for (int k = k0; k < k1; ++k) {
-            const int gi = a.cn_pair[2 * k], gj = a.cn_pair[2 * k + 1];
-            int li = 0, lj = 0;
-            for (int t = 0; t < na; ++t) {
-                if (at[t] == gi) li = t;
-                if (at[t] == gj) lj = t;
+#pragma unroll
+        for (int m = 0; m < MC - 1; ++m) {
+            if (m < nc) {
+                const int j = m + 1;
+                const double d = dl[m];
+                const double rnx = xs[0] - xs[3 * j], rny = xs[1] - xs[3 * j + 1], rnz = xs[2] - xs[3 * j + 2];
+                const double err = (rnx * rnx + rny * rny + rnz * rnz) - d * d;
+                if (fabs(err) > max_err) max_err = fabs(err);
+                if (!(fabs(err) < 1e-6)) {
+                    const double rox = rs[0] - rs[3 * j], roy = rs[1] - rs[3 * j + 1], roz = rs[2] - rs[3 * j + 2];
+                    const double imi = im[0], imj = im[j];
+                    const double denom = 2.0 * (imi + imj) * (rnx * rox + rny * roy + rnz * roz);
+                    if (!(fabs(denom) < 1e-18)) {
+                        const double lam = err / denom;
+                        const double dx = rox * lam, dy = roy * lam, dz = roz * lam;
+                        xs[0] -= dx * imi; xs[1] -= dy * imi; xs[2] -= dz * imi;
+                        xs[3 * j] += dx * imj; xs[3 * j + 1] += dy * imj; xs[3 * j + 2] += dz * imj;
+                    }
+                }
             }
-            const double d = a.cn_len[k];
-            const double rnx = xs[3 * li] - xs[3 * lj], rny = xs[3 * li + 1] - xs[3 * lj + 1],
-                         rnz = xs[3 * li + 2] - xs[3 * lj + 2];
-            const double err = (rnx * rnx + rny * rny + rnz * rnz) - d * d;
-            if (fabs(err) > max_err) max_err = fabs(err);
-            if (fabs(err) < 1e-6) continue;
-            const double rox = rs[3 * li] - rs[3 * lj], roy = rs[3 * li + 1] - rs[3 * lj + 1],
-                         roz = rs[3 * li + 2] - rs[3 * lj + 2];
-            const double imi = 1.0 / a.mass[gi], imj = 1.0 / a.mass[gj];
-            const double denom = 2.0 * (imi + imj) * (rnx * rox + rny * roy + rnz * roz);
-            if (fabs(denom) < 1e-18) continue;
-            const double lam = err / denom;
-            const double dx = rox * lam, dy = roy * lam, dz = roz * lam;
-            xs[3 * li] -= dx * imi; xs[3 * li + 1] -= dy * imi; xs[3 * li + 2] -= dz * imi;
-            xs[3 * lj] += dx * imj; xs[3 * lj + 1] += dy * imj; xs[3 * lj + 2] += dz * imj;
         }
-        if (max_err < 1e-6) return 1;
+        if (max_err < 1e-6) return true;
     }
-    return 0;
+    return false;
 }
 
-__global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= a.n_clusters) return;
+// One thread per cluster of at most MC atoms: clusters [c_begin, c_end).
+template <int MC>
+__global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a, int c_begin, int c_end) {
+    const int c = c_begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= c_end) return;
     const int s0 = a.cl_start[c], na = a.cl_start[c + 1] - s0;
-    int at[OG_MAX_CL];
-    double xs[3 * OG_MAX_CL], vs[3 * OG_MAX_CL], rs[3 * OG_MAX_CL];
+    const int k0 = a.cn_start[c], nc = a.cn_start[c + 1] - k0;
+    int at[MC];
+    double xs[3 * MC], vs[3 * MC], rs[3 * MC], im[MC], dl[MC];
     const int sys = a.atom_sys[a.cl_atoms[s0]];
     const bool frozen = a.diverged[sys] != 0;
     const double hdt = 0.5 * a.dt;
     const double kbt = OG_KB * a.sys_temp[sys];
     const double dofc = a.sys_dofcorr[sys];
     const double om = 1.0 - a.alpha * a.alpha;
-    for (int t = 0; t < na; ++t) {
-        const int g = a.cl_atoms[s0 + t];
-        at[t] = g;
-        for (int d = 0; d < 3; ++d) {
-            xs[3 * t + d] = a.pos[3 * (size_t)g + d];
-            vs[3 * t + d] = a.vel[3 * (size_t)g + d];
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+        at[t] = 0;
+        im[t] = 0.0;
+        dl[t] = t < nc ? a.cn_len[k0 + t] : 0.0;
+        for (int d = 0; d < 3; ++d) xs[3 * t + d] = vs[3 * t + d] = rs[3 * t + d] = 0.0;
+        if (t < na) {
+            const int g = a.cl_atoms[s0 + t];
+            at[t] = g;
+            im[t] = 1.0 / a.mass[g];
+            for (int d = 0; d < 3; ++d) {
+                xs[3 * t + d] = a.pos[3 * (size_t)g + d];
+                vs[3 * t + d] = a.vel[3 * (size_t)g + d];
+            }
+            a.facc[3 * (size_t)g] = 0; a.facc[3 * (size_t)g + 1] = 0; a.facc[3 * (size_t)g + 2] = 0;
+            a.iacc[g] = 0;
         }
-        a.facc[3 * (size_t)g] = 0; a.facc[3 * (size_t)g + 1] = 0; a.facc[3 * (size_t)g + 2] = 0;
-        a.iacc[g] = 0;
     }
     if (frozen) return;
     int fails = 0;
+    const uint32_t step = a.step_dev ? *a.step_dev : a.step;
     // B: v += F/m * ACCEL * dt/2
-    for (int t = 0; t < na; ++t) {
-        const int g = at[t];
-        const double k = OG_ACCEL / a.mass[g] * hdt;
-        for (int d = 0; d < 3; ++d) vs[3 * t + d] += a.force[3 * (size_t)g + d] * k;
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+        if (t < na) {
+            const int g = at[t];
+            const double k = OG_ACCEL / a.mass[g] * hdt;
+            for (int d = 0; d < 3; ++d) vs[3 * t + d] += a.force[3 * (size_t)g + d] * k;
+        }
     }
     for (int half = 0; half < 2; ++half) {
         if (half == 1) {  // O
-            for (int t = 0; t < na; ++t) {
-                const int g = at[t];
-                const double sigma = sqrt(om * kbt * dofc * OG_ACCEL / a.mass[g]);
-                double g0, g1, g2;
-                noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], a.step_dev ? *a.step_dev : a.step,
-                       g0, g1, g2);
-                vs[3 * t] = a.alpha * vs[3 * t] + sigma * g0;
-                vs[3 * t + 1] = a.alpha * vs[3 * t + 1] + sigma * g1;
-                vs[3 * t + 2] = a.alpha * vs[3 * t + 2] + sigma * g2;
+#pragma unroll
+            for (int t = 0; t < MC; ++t) {
+                if (t < na) {
+                    const int g = at[t];
+                    const double sigma = sqrt(om * kbt * dofc * OG_ACCEL / a.mass[g]);
+                    double g0, g1, g2;
+                    noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], step, g0, g1, g2);
+                    vs[3 * t] = a.alpha * vs[3 * t] + sigma * g0;
+                    vs[3 * t + 1] = a.alpha * vs[3 * t + 1] + sigma * g1;
+                    vs[3 * t + 2] = a.alpha * vs[3 * t + 2] + sigma * g2;
+                }
             }
         }
-        for (int k = 0; k < 3 * na; ++k) {  // A
+#pragma unroll
+        for (int k = 0; k < 3 * MC; ++k) {  // A
             rs[k] = xs[k];
             xs[k] += vs[k] * hdt;
         }
         if (a.use_shake) {
-            if (!shake_cluster(a, c, xs, rs, at, na)) fails++;
-            for (int k = 0; k < 3 * na; ++k) vs[k] = (xs[k] - rs[k]) / hdt;
+            if (!shake_cluster<MC>(xs, rs, im, dl, nc)) fails++;
+#pragma unroll
+            for (int k = 0; k < 3 * MC; ++k) vs[k] = (xs[k] - rs[k]) / hdt;
         }
     }
-    for (int t = 0; t < na; ++t) {
-        const int g = at[t];
-        for (int d = 0; d < 3; ++d) {
-            a.pos[3 * (size_t)g + d] = xs[3 * t + d];
-            a.vel[3 * (size_t)g + d] = vs[3 * t + d];
+#pragma unroll
+    for (int t = 0; t < MC; ++t) {
+        if (t < na) {
+            const int g = at[t];
+            for (int d = 0; d < 3; ++d) {
+                a.pos[3 * (size_t)g + d] = xs[3 * t + d];
+                a.vel[3 * (size_t)g + d] = vs[3 * t + d];
+            }
         }
     }
     if (fails) atomicAdd(a.shake_fail + sys, fails);
@@ -410,7 +437,12 @@ __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a) {
 
 void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s) {
     if (a.n_clusters <= 0) return;
-    hipLaunchKernelGGL(og_k_cluster_step, dim3((a.n_clusters + 255) / 256), dim3(256), 0, s, a);
+    if (a.n_small > 0)
+        hipLaunchKernelGGL(og_k_cluster_step<OG_SMALL_CL>, dim3((a.n_small + 255) / 256), dim3(256), 0, s, a,
+                           0, a.n_small);
+    if (a.n_clusters > a.n_small)
+        hipLaunchKernelGGL(og_k_cluster_step<OG_MAX_CL>, dim3((a.n_clusters - a.n_small + 255) / 256), dim3(256),
+                           0, s, a, a.n_small, a.n_clusters);
 }
 
 __global__ void __launch_bounds__(256) og_k_zero_acc(long long* facc, long long* iacc, int n) {

@@ -66,18 +66,21 @@ struct og_radii_args {
     int check;                       // plain radii were computed for every system this call
 };
 
+#define OG_MAX_CL 8                  // atoms per SHAKE cluster (X + up to 7 H)
+#define OG_SMALL_CL 4                // the narrow cluster kernel's limit (CH3, NH3+)
+
 struct og_cluster_args {
     double* pos; double* vel; double* ref; const double* force;
     long long* facc; long long* iacc;
     const double* mass;
     const int* atom_sys;
     const int* atom_local;           // index within its system (RNG counter)
-    const int* cl_start;             // [C+1] atoms of cluster c: cl_atoms[cl_start[c]..]
-    const int* cl_atoms;
-    const int* cn_start;             // [C+1] constraints of cluster c
-    const int* cn_pair;              // [2*K] atom pairs
+    const int* cl_start;             // [C+1] atoms of cluster c: cl_atoms[cl_start[c]..],
+    const int* cl_atoms;             //   heavy atom first, then the hydrogen of each constraint
+    const int* cn_start;             // [C+1] constraints of cluster c (constraint m: atoms 0, m + 1)
     const double* cn_len;            // [K]
     int n_clusters;
+    int n_small;                     // clusters [0, n_small) hold at most OG_SMALL_CL atoms
     const double* sys_temp;          // [S]
     const double* sys_dofcorr;       // [S]
     int* shake_fail;                 // [S]
