@@ -363,6 +363,233 @@ OG_HD bool vertex_epsilon_jvp(v3 pi, v3 pk, v3 pl, double ri, double rk, double 
     return true;
 }
 
+// ---------------------------------------------------------------------------
+// The same JVPs split into a primal part, formed once per arc or vertex, and
+// a tangent part per direction (role x axis).  Each tangent repeats the
+// operations of the functions above on the same primal values, so every
+// term is bit-identical to theirs; only the repeated primal work goes.
+// ---------------------------------------------------------------------------
+struct CosAlphaP {
+    v3 gk;              // d cos(a) / d r_k (d cos(a) / d r_i = -gk)
+    bool ok;
+};
+
+OG_HD inline CosAlphaP cos_alpha_primal(v3 pi, v3 pk, double ri, double rk) {
+    const v3 rv = pk - pi;
+    const double d2 = norm2(rv);
+    const double d = sqrt(d2);
+    if (d < 1e-12) return {mk(0, 0, 0), false};
+    const v3 u = rv / d;
+    const double dcos_dd = (d2 - ri * ri + rk * rk) / (2.0 * d2 * ri);
+    return {u * dcos_dd, true};
+}
+
+OG_HD inline double cos_alpha_tangent(const CosAlphaP& P, v3 dri, v3 drk) {
+    if (!P.ok) return 0.0;
+    const v3 gi = P.gk * -1.0;
+    return dot(gi, dri) + dot(P.gk, drk);
+}
+
+struct VertexJvpP {
+    v3 wk, wl, np;
+    double dik, dil, cak, cal, sigma, denom, denom2, a, b, c, mn, num_a, num_b, sign;
+    CosAlphaP ck, cl;
+    bool ok;
+};
+
+OG_HD inline VertexJvpP vertex_jvp_primal(v3 pi, v3 pk, v3 pl, double ri, double rk, double rl, double sign) {
+    VertexJvpP P{};
+    P.ok = false;
+    const v3 rkv = pk - pi;
+    P.dik = norm(rkv);
+    if (P.dik < 1e-12) return P;
+    P.wk = rkv / P.dik;
+    const v3 rlv = pl - pi;
+    P.dil = norm(rlv);
+    if (P.dil < 1e-12) return P;
+    P.wl = rlv / P.dil;
+    P.cak = (P.dik * P.dik + ri * ri - rk * rk) / (2.0 * P.dik * ri);
+    P.cal = (P.dil * P.dil + ri * ri - rl * rl) / (2.0 * P.dil * ri);
+    P.sigma = dot(P.wk, P.wl);
+    P.denom = 1.0 - P.sigma * P.sigma;
+    if (fabs(P.denom) < 1e-12) return P;
+    P.a = (P.cak - P.sigma * P.cal) / P.denom;
+    P.b = (P.cal - P.sigma * P.cak) / P.denom;
+    const double c_sq = 1.0 - P.a * P.a - P.b * P.b - 2.0 * P.a * P.b * P.sigma;
+    if (c_sq <= 0.0) return P;
+    P.c = sqrt(c_sq);
+    const v3 m = cross(P.wk, P.wl);
+    P.mn = sqrt(norm2(m));
+    if (P.mn < 1e-12) return P;
+    P.np = m / P.mn;
+    P.ck = cos_alpha_primal(pi, pk, ri, rk);
+    P.cl = cos_alpha_primal(pi, pl, ri, rl);
+    P.num_a = P.cak - P.sigma * P.cal;
+    P.num_b = P.cal - P.sigma * P.cak;
+    P.denom2 = P.denom * P.denom;
+    P.sign = sign;
+    P.ok = true;
+    return P;
+}
+
+OG_HD inline v3 vertex_jvp_tangent(const VertexJvpP& P, v3 dri, v3 drk, v3 drl) {
+    const v3 dvk = drk - dri;
+    const v3 dwk = (dvk - P.wk * dot(P.wk, dvk)) / P.dik;
+    const v3 dvl = drl - dri;
+    const v3 dwl = (dvl - P.wl * dot(P.wl, dvl)) / P.dil;
+    const double dcos_k = cos_alpha_tangent(P.ck, dri, drk);
+    const double dcos_l = cos_alpha_tangent(P.cl, dri, drl);
+    const double dsig = dot(dwk, P.wl) + dot(P.wk, dwl);
+    const double dden = -2.0 * P.sigma * dsig;
+    const double dnum_a = dcos_k - dsig * P.cal - P.sigma * dcos_l;
+    const double da = (dnum_a * P.denom - P.num_a * dden) / P.denom2;
+    const double dnum_b = dcos_l - dsig * P.cak - P.sigma * dcos_k;
+    const double db = (dnum_b * P.denom - P.num_b * dden) / P.denom2;
+    const double dc_sq = -2.0 * P.a * da - 2.0 * P.b * db - 2.0 * (da * P.b + P.a * db) * P.sigma -
+                         2.0 * P.a * P.b * dsig;
+    const double dc = dc_sq / (2.0 * P.c);
+    const v3 dbase = P.wk * da + dwk * P.a + P.wl * db + dwl * P.b;
+    const v3 dm = cross(dwk, P.wl) + cross(P.wk, dwl);
+    const v3 dnp = (dm - P.np * dot(P.np, dm)) / P.mn;
+    return dbase + (P.np * dc + dnp * P.c) * P.sign;
+}
+
+// arc_theta_jvp (derivatives.rs:346-444) in primal / tangent form
+struct ArcThetaP {
+    VertexJvpP vs_p, ve_p;
+    v3 wk, vs, ve, us, ue, x;
+    double dik, vdo_s, vdo_e, nn_s, nn_e, cos_t, sin_t;
+    bool ok;
+};
+
+OG_HD inline ArcThetaP arc_theta_primal(v3 pi, v3 pk, v3 pl, v3 pm, double ri, double rk, double rl, double rm,
+                                        double sign_s, double sign_e) {
+    ArcThetaP P{};
+    P.ok = false;
+    const v3 rkv = pk - pi;
+    P.dik = norm(rkv);
+    if (P.dik < 1e-12) return P;
+    P.wk = rkv / P.dik;
+    const double cak = (P.dik * P.dik + ri * ri - rk * rk) / (2.0 * P.dik * ri);
+    const Cap ck = small_circle_new(P.wk, cak);
+    if (norm(pl - pi) < 1e-12) return P;
+    const Cap cl = cap_from(pi, pl, ri, rl);
+    if (norm(pm - pi) < 1e-12) return P;
+    const Cap cm = cap_from(pi, pm, ri, rm);
+    if (!vertex_point(ck, cl, sign_s, P.vs)) return P;
+    if (!vertex_point(ck, cm, sign_e, P.ve)) return P;
+    P.vs_p = vertex_jvp_primal(pi, pk, pl, ri, rk, rl, sign_s);
+    if (!P.vs_p.ok) return P;
+    P.ve_p = vertex_jvp_primal(pi, pk, pm, ri, rk, rm, sign_e);
+    if (!P.ve_p.ok) return P;
+    {
+        P.vdo_s = dot(P.vs, P.wk);
+        const v3 proj = P.vs - P.wk * P.vdo_s;
+        P.nn_s = norm(proj);
+        if (P.nn_s < 1e-12) return P;
+        P.us = proj / P.nn_s;
+    }
+    {
+        P.vdo_e = dot(P.ve, P.wk);
+        const v3 proj = P.ve - P.wk * P.vdo_e;
+        P.nn_e = norm(proj);
+        if (P.nn_e < 1e-12) return P;
+        P.ue = proj / P.nn_e;
+    }
+    P.cos_t = dot(P.us, P.ue);
+    P.x = cross(P.us, P.ue);
+    P.sin_t = dot(P.wk, P.x);
+    P.ok = true;
+    return P;
+}
+
+OG_HD inline double arc_theta_tangent(const ArcThetaP& P, v3 dri, v3 drk, v3 drl, v3 drm) {
+    const v3 dvv = drk - dri;
+    const v3 dwk = (dvv - P.wk * dot(P.wk, dvv)) / P.dik;
+    const v3 dvs = vertex_jvp_tangent(P.vs_p, dri, drk, drl);
+    const v3 dve = vertex_jvp_tangent(P.ve_p, dri, drk, drm);
+    v3 dus, due;
+    {
+        const double dvdo = dot(dvs, P.wk) + dot(P.vs, dwk);
+        const v3 dproj = dvs - dwk * P.vdo_s - P.wk * dvdo;
+        dus = (dproj - P.us * dot(P.us, dproj)) / P.nn_s;
+    }
+    {
+        const double dvdo = dot(dve, P.wk) + dot(P.ve, dwk);
+        const v3 dproj = dve - dwk * P.vdo_e - P.wk * dvdo;
+        due = (dproj - P.ue * dot(P.ue, dproj)) / P.nn_e;
+    }
+    const double dcos_t = dot(dus, P.ue) + dot(P.us, due);
+    const v3 dx = cross(dus, P.ue) + cross(P.us, due);
+    const double dsin_t = dot(dwk, P.x) + dot(P.wk, dx);
+    return P.cos_t * dsin_t - P.sin_t * dcos_t;
+}
+
+// vertex_epsilon_jvp (derivatives.rs:466-545) in primal / tangent form
+struct VertexEpsP {
+    VertexJvpP vp;
+    v3 wk, wl, v, tin, tout, x;
+    double dik, dil, nn_in, nn_out, ce, se;
+    bool ok;
+};
+
+OG_HD inline VertexEpsP vertex_eps_primal(v3 pi, v3 pk, v3 pl, double ri, double rk, double rl, double sign) {
+    VertexEpsP P{};
+    P.ok = false;
+    const v3 rkv = pk - pi;
+    P.dik = norm(rkv);
+    if (P.dik < 1e-12) return P;
+    P.wk = rkv / P.dik;
+    const v3 rlv = pl - pi;
+    P.dil = norm(rlv);
+    if (P.dil < 1e-12) return P;
+    P.wl = rlv / P.dil;
+    const double cak = (P.dik * P.dik + ri * ri - rk * rk) / (2.0 * P.dik * ri);
+    const double cal = (P.dil * P.dil + ri * ri - rl * rl) / (2.0 * P.dil * ri);
+    const Cap ck = small_circle_new(P.wk, cak), cl = small_circle_new(P.wl, cal);
+    if (!vertex_point(ck, cl, sign, P.v)) return P;
+    P.vp = vertex_jvp_primal(pi, pk, pl, ri, rk, rl, sign);
+    if (!P.vp.ok) return P;
+    {
+        const v3 cr = cross(P.v, P.wk);
+        P.nn_in = norm(cr);
+        if (P.nn_in < 1e-12) return P;
+        P.tin = cr / P.nn_in;
+    }
+    {
+        const v3 cr = cross(P.v, P.wl);
+        P.nn_out = norm(cr);
+        if (P.nn_out < 1e-12) return P;
+        P.tout = cr / P.nn_out;
+    }
+    P.ce = dot(P.tin, P.tout);
+    P.x = cross(P.tin, P.tout);
+    P.se = dot(P.v, P.x);
+    P.ok = true;
+    return P;
+}
+
+OG_HD inline double vertex_eps_tangent(const VertexEpsP& P, v3 dri, v3 drk, v3 drl) {
+    const v3 dvk = drk - dri;
+    const v3 dwk = (dvk - P.wk * dot(P.wk, dvk)) / P.dik;
+    const v3 dvl = drl - dri;
+    const v3 dwl = (dvl - P.wl * dot(P.wl, dvl)) / P.dil;
+    const v3 dv = vertex_jvp_tangent(P.vp, dri, drk, drl);
+    v3 dtin, dtout;
+    {
+        const v3 dcr = cross(dv, P.wk) + cross(P.v, dwk);
+        dtin = (dcr - P.tin * dot(P.tin, dcr)) / P.nn_in;
+    }
+    {
+        const v3 dcr = cross(dv, P.wl) + cross(P.v, dwl);
+        dtout = (dcr - P.tout * dot(P.tout, dcr)) / P.nn_out;
+    }
+    const double dce = dot(dtin, P.tout) + dot(P.tin, dtout);
+    const v3 dx = cross(dtin, P.tout) + cross(P.tin, dtout);
+    const double dse = dot(dv, P.x) + dot(P.v, dx);
+    return P.ce * dse - P.se * dce;
+}
+
 OG_HD inline v3 ldp(const double* pos, int a) {
     return mk(pos[3 * (size_t)a], pos[3 * (size_t)a + 1], pos[3 * (size_t)a + 2]);
 }

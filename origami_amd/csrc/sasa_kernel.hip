@@ -30,7 +30,8 @@
 //      ballot-compacted in slot order, which is origami's (k, segment) order.
 //      Vertex root signs and arc->vertex links follow, and the atom's arcs,
 //      vertices and cap owners are appended to a pool.
-//  og_k_sasa_terms    thread per (arc|vertex, role, axis): one JVP term each.
+//  og_k_sasa_terms    thread per arc or vertex: its JVP terms (role x axis),
+//      the primal geometry formed once.
 //  og_k_sasa_scatter  one wave per atom, lane per affected atom x: the terms
 //      of every arc (in arc order) and then every vertex (in vertex order)
 //      that involve x are summed, which is the order origami sums them in
@@ -1045,59 +1046,75 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     }
 }
 
-// One JVP term per thread: arc items (12 per arc: role i, k, l, m x axis)
-// then vertex items (9 per vertex: role i, in, out x axis).
-__global__ void __launch_bounds__(256) og_k_sasa_terms(og_sasa_data d, const double* pos) {
+// The JVP terms of one arc (12: role i, k, l, m x axis) or one vertex (9:
+// role i, in, out x axis) per thread: the primal geometry once, then one
+// tangent per direction (bit-identical to one full JVP per term).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) og_k_sasa_terms(og_sasa_data d, const double* pos) {
     const int n_a = min(d.counters[0], d.arc_cap);
     const int n_v = min(d.counters[1], d.vtx_cap);
-    const long long total = 12ll * n_a + 9ll * n_v;
+    const int total = n_a + n_v;
     const v3 zero = mk(0, 0, 0);
-    for (long long it = (long long)blockIdx.x * blockDim.x + threadIdx.x; it < total;
-         it += (long long)gridDim.x * blockDim.x) {
-        if (it < 12ll * n_a) {
-            const int A = (int)(it / 12), role = (int)(it % 12) / 3, axis = (int)(it % 3);
+    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < total; it += gridDim.x * blockDim.x) {
+        if (it < n_a) {
+            const int A = it;
             const int4 at = d.arc_atoms[A];
-            const int xa = role == 0 ? at.x : role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
-            double term = 0.0;
-            if (xa >= 0) {
-                const double4 pr = d.arc_par[A];
-                const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y);
-                const double ri = d.radius[at.x], rk = d.radius[at.y];
-                const v3 b = unit(axis);
-                const v3 dri = xa == at.x ? b : zero;
-                const v3 drk = xa == at.y ? b : zero;
-                const double dcos = cos_alpha_directional(pi, pk, ri, rk, dri, drk);
-                if (at.z < 0) {
-                    term = pr.x * dcos;
-                } else {
-                    const v3 pl = ldp(pos, at.z), pm = ldp(pos, at.w);
-                    const double rl = d.radius[at.z], rm = d.radius[at.w];
-                    const v3 drl = xa == at.z ? b : zero;
-                    const v3 drm = xa == at.w ? b : zero;
-                    const double dik = norm(pk - pi);
-                    const double cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
-                    double dth = 0.0;
-                    if (!arc_theta_jvp(pi, pk, pl, pm, ri, rk, rl, rm, pr.y, pr.z, dri, drk, drl, drm, dth))
-                        dth = 0.0;
-                    term = pr.x * dcos + cak * dth;
+            const double4 pr = d.arc_par[A];
+            const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y);
+            const double ri = d.radius[at.x], rk = d.radius[at.y];
+            const CosAlphaP cp = cos_alpha_primal(pi, pk, ri, rk);
+            const bool has_v = at.z >= 0;
+            v3 pl = zero, pm = zero;
+            double rl = 0.0, rm = 0.0, cak = 0.0;
+            ArcThetaP tp{};
+            tp.ok = false;
+            if (has_v) {
+                pl = ldp(pos, at.z);
+                pm = ldp(pos, at.w);
+                rl = d.radius[at.z];
+                rm = d.radius[at.w];
+                const double dik = norm(pk - pi);
+                cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
+                tp = arc_theta_primal(pi, pk, pl, pm, ri, rk, rl, rm, pr.y, pr.z);
+            }
+            double* out = d.arc_terms + 12ll * A;
+            for (int role = 0; role < 4; ++role) {
+                const int xa = role == 0 ? at.x : role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
+                for (int axis = 0; axis < 3; ++axis) {
+                    double term = 0.0;
+                    if (xa >= 0) {
+                        const v3 b = unit(axis);
+                        const v3 dri = xa == at.x ? b : zero;
+                        const v3 drk = xa == at.y ? b : zero;
+                        const double dcos = cos_alpha_tangent(cp, dri, drk);
+                        if (!has_v) {
+                            term = pr.x * dcos;
+                        } else {
+                            const v3 drl = xa == at.z ? b : zero;
+                            const v3 drm = xa == at.w ? b : zero;
+                            const double dth = tp.ok ? arc_theta_tangent(tp, dri, drk, drl, drm) : 0.0;
+                            term = pr.x * dcos + cak * dth;
+                        }
+                    }
+                    out[3 * role + axis] = term;
                 }
             }
-            d.arc_terms[it] = term;
         } else {
-            const long long jt = it - 12ll * n_a;
-            const int V = (int)(jt / 9), role = (int)(jt % 9) / 3, axis = (int)(jt % 3);
+            const int V = it - n_a;
             const int4 at = d.vtx_atoms[V];
-            const int xa = role == 0 ? at.x : role == 1 ? at.y : at.z;
             const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y), pl = ldp(pos, at.z);
-            const v3 b = unit(axis);
-            const v3 dri = xa == at.x ? b : zero;
-            const v3 drk = xa == at.y ? b : zero;
-            const v3 drl = xa == at.z ? b : zero;
-            double de = 0.0;
-            if (!vertex_epsilon_jvp(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z], d.vtx_sign[V],
-                                    dri, drk, drl, de))
-                de = 0.0;
-            d.vtx_terms[jt] = de;
+            const VertexEpsP ep = vertex_eps_primal(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z],
+                                                    d.vtx_sign[V]);
+            double* out = d.vtx_terms + 9ll * V;
+            for (int role = 0; role < 3; ++role) {
+                const int xa = role == 0 ? at.x : role == 1 ? at.y : at.z;
+                for (int axis = 0; axis < 3; ++axis) {
+                    const v3 b = unit(axis);
+                    const v3 dri = xa == at.x ? b : zero;
+                    const v3 drk = xa == at.y ? b : zero;
+                    const v3 drl = xa == at.z ? b : zero;
+                    out[3 * role + axis] = ep.ok ? vertex_eps_tangent(ep, dri, drk, drl) : 0.0;
+                }
+            }
         }
     }
 }

@@ -64,4 +64,40 @@ void host_cover_arcs(int n, const double* caps, float* arcs, double* basis) {
     }
 }
 
+// JVP terms of random arcs / vertices both ways: one full JVP per direction
+// (the functions origami's derivatives are restated in) and the primal /
+// tangent split the terms kernel uses.  rows: 4 atoms (12 doubles), 4 radii,
+// 2 signs; out: 12 arc terms each way, 9 vertex terms each way.
+void host_jvp_split(int n, const double* rows, double* arc_full, double* arc_split,
+                    double* vtx_full, double* vtx_split) {
+    for (int r = 0; r < n; ++r) {
+        const double* q = rows + 18 * r;
+        const v3 p[4] = {mk(q[0], q[1], q[2]), mk(q[3], q[4], q[5]), mk(q[6], q[7], q[8]), mk(q[9], q[10], q[11])};
+        const double rad[4] = {q[12], q[13], q[14], q[15]};
+        const double ss = q[16], se = q[17];
+        const v3 z = mk(0, 0, 0);
+        const ArcThetaP tp = arc_theta_primal(p[0], p[1], p[2], p[3], rad[0], rad[1], rad[2], rad[3], ss, se);
+        const VertexEpsP ep = vertex_eps_primal(p[0], p[1], p[2], rad[0], rad[1], rad[2], ss);
+        for (int role = 0; role < 4; ++role) {
+            for (int axis = 0; axis < 3; ++axis) {
+                const v3 b = unit(axis);
+                const v3 d[4] = {role == 0 ? b : z, role == 1 ? b : z, role == 2 ? b : z, role == 3 ? b : z};
+                double full = 0.0;
+                if (!arc_theta_jvp(p[0], p[1], p[2], p[3], rad[0], rad[1], rad[2], rad[3], ss, se,
+                                   d[0], d[1], d[2], d[3], full))
+                    full = 0.0;
+                arc_full[12 * r + 3 * role + axis] = full;
+                arc_split[12 * r + 3 * role + axis] = tp.ok ? arc_theta_tangent(tp, d[0], d[1], d[2], d[3]) : 0.0;
+                if (role < 3) {
+                    double vf = 0.0;
+                    if (!vertex_epsilon_jvp(p[0], p[1], p[2], rad[0], rad[1], rad[2], ss, d[0], d[1], d[2], vf))
+                        vf = 0.0;
+                    vtx_full[9 * r + 3 * role + axis] = vf;
+                    vtx_split[9 * r + 3 * role + axis] = ep.ok ? vertex_eps_tangent(ep, d[0], d[1], d[2]) : 0.0;
+                }
+            }
+        }
+    }
+}
+
 }

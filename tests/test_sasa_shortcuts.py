@@ -246,3 +246,24 @@ def test_cover_walk_on_chain_atoms(host):
             assert val.min() > 1e-6 - 1e-9
             checked += 1
     assert checked > 200
+
+
+def test_jvp_split_is_bit_identical(host):
+    # the terms kernel forms each arc's / vertex's primal geometry once and
+    # one tangent per direction; every term must equal the full JVP's bits
+    host.host_jvp_split.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 5
+    rng = np.random.default_rng(8)
+    n = 20000
+    pi = np.zeros((n, 3))
+    dirs = [_unit(rng, n) for _ in range(3)]
+    rad = rng.uniform(2.6, 3.2, (n, 4))
+    dist = rng.uniform(1.0, 5.5, (n, 3))
+    pts = [pi] + [pi + dirs[q] * dist[:, q:q + 1] for q in range(3)]
+    rows = np.ascontiguousarray(np.concatenate(pts + [rad, rng.choice([-1.0, 1.0], (n, 2))], axis=1))
+    af, asp = np.zeros((n, 12)), np.zeros((n, 12))
+    vf, vsp = np.zeros((n, 9)), np.zeros((n, 9))
+    host.host_jvp_split(n, _ptr(rows, C.c_double), _ptr(af, C.c_double), _ptr(asp, C.c_double),
+                        _ptr(vf, C.c_double), _ptr(vsp, C.c_double))
+    assert np.count_nonzero(af) > n and np.count_nonzero(vf) > n
+    assert np.array_equal(af.view(np.int64), asp.view(np.int64))
+    assert np.array_equal(vf.view(np.int64), vsp.view(np.int64))
