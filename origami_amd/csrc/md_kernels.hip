@@ -70,100 +70,124 @@ __device__ __forceinline__ double wrap_pi(double x) {
     return x;
 }
 
+// One bonded term k: its per-atom forces into its own slots; with ENERGY
+// its energy (fixed point), energy slot and system are returned.
+template <bool ENERGY>
+__device__ __forceinline__ void bonded_term(const og_bonded_args& a, int k, int& e_slot, int& e_sys, long long& e_fix) {
+    if (k < a.nb) {
+        const int ia = a.bonds[2 * k], ib = a.bonds[2 * k + 1];
+        const double K = a.bond_p[2 * k], r0 = a.bond_p[2 * k + 1];
+        d3 d = sub(ld3(a.pos + 3 * ia), ld3(a.pos + 3 * ib));
+        d = neg(d);  // d = x_b - x_a
+        const double r = nrm(d);
+        d3 f = {0.0, 0.0, 0.0};
+        if (r >= 1e-9) {
+            const double mag = 2.0 * K * (r - r0) * OG_KCAL_TO_KJ;
+            f = mul(d, mag / r);
+        }
+        put(a.tf, 2ll * k, f);
+        put(a.tf, 2ll * k + 1, neg(f));
+        if (ENERGY) {
+            const double dr = r - r0;
+            e_slot = OG_E_BOND;
+            e_sys = a.atom_sys[ia];
+            e_fix = og_to_fix(K * dr * dr * OG_KCAL_TO_KJ);
+        }
+        return;
+    }
+    int kk = k - a.nb;
+    if (kk < a.na) {
+        const int ia = a.angles[3 * kk], ib = a.angles[3 * kk + 1], ic = a.angles[3 * kk + 2];
+        const double K = a.angle_p[2 * kk], th0 = a.angle_p[2 * kk + 1];
+        const d3 pb = ld3(a.pos + 3 * ib);
+        const d3 u = sub(ld3(a.pos + 3 * ia), pb), v = sub(ld3(a.pos + 3 * ic), pb);
+        const double un = nrm(u), vn = nrm(v);
+        const long long s0 = 2ll * a.nb + 3ll * kk;
+        d3 fa = {0.0, 0.0, 0.0}, fc = {0.0, 0.0, 0.0};
+        if (un >= 1e-9 && vn >= 1e-9) {
+            const d3 uh = dv(u, un), vh = dv(v, vn);
+            double ct = fmin(fmax(dot(uh, vh), -1.0), 1.0);
+            const double st = sqrt(1.0 - ct * ct);
+            if (st >= 1e-9) {
+                const double theta = acos(ct);
+                const double coeff = 2.0 * K * (theta - th0) * OG_KCAL_TO_KJ / st;
+                fa = mul(sub(vh, mul(uh, ct)), coeff / un);
+                fc = mul(sub(uh, mul(vh, ct)), coeff / vn);
+            }
+            if (ENERGY) {
+                const double dt = acos(fmin(fmax(dot(uh, vh), -1.0), 1.0)) - th0;
+                e_slot = OG_E_ANGLE;
+                e_sys = a.atom_sys[ia];
+                e_fix = og_to_fix(K * dt * dt * OG_KCAL_TO_KJ);
+            }
+        }
+        put(a.tf, s0, fa);
+        put(a.tf, s0 + 1, neg(add(fa, fc)));
+        put(a.tf, s0 + 2, fc);
+        return;
+    }
+    kk -= a.na;
+    const bool is_dih = kk < a.nd;
+    if (!is_dih) kk -= a.nd;
+    const int* q = is_dih ? a.dih + 4 * kk : a.imp + 4 * kk;
+    const d3 pa = ld3(a.pos + 3 * q[0]), pb = ld3(a.pos + 3 * q[1]), pc = ld3(a.pos + 3 * q[2]),
+             pd = ld3(a.pos + 3 * q[3]);
+    d3 da, db, dc, dd;
+    double phi;
+    const long long s0 = 2ll * a.nb + 3ll * a.na + (is_dih ? 4ll * kk : 4ll * a.nd + 4ll * kk);
+    d3 f0 = {0.0, 0.0, 0.0}, f1 = f0, f2 = f0, f3 = f0;
+    if (dihedral_gradient(pa, pb, pc, pd, da, db, dc, dd, phi)) {
+        double dvdphi;
+        if (is_dih) {
+            const double kd = a.dih_p[3 * kk], nn = a.dih_p[3 * kk + 1], delta = a.dih_p[3 * kk + 2];
+            dvdphi = -kd * nn * sin(nn * phi - delta) * OG_KCAL_TO_KJ;
+        } else {
+            dvdphi = 2.0 * a.imp_p[2 * kk] * wrap_pi(phi - a.imp_p[2 * kk + 1]) * OG_KCAL_TO_KJ;
+        }
+        f0 = mul(da, -dvdphi);
+        f1 = mul(db, -dvdphi);
+        f2 = mul(dc, -dvdphi);
+        f3 = mul(dd, -dvdphi);
+    }
+    put(a.tf, s0, f0);
+    put(a.tf, s0 + 1, f1);
+    put(a.tf, s0 + 2, f2);
+    put(a.tf, s0 + 3, f3);
+    if (ENERGY) {
+        const double chi = measure_dihedral(pa, pb, pc, pd);
+        double e;
+        int slot;
+        if (is_dih) {
+            e = a.dih_p[3 * kk] * (1.0 + cos(a.dih_p[3 * kk + 1] * chi - a.dih_p[3 * kk + 2]));
+            slot = OG_E_DIHEDRAL;
+        } else {
+            const double dom = wrap_pi(chi - a.imp_p[2 * kk + 1]);
+            e = a.imp_p[2 * kk] * dom * dom;
+            slot = OG_E_IMPROPER;
+        }
+        e_slot = slot;
+        e_sys = a.atom_sys[q[0]];
+        e_fix = og_to_fix(e * OG_KCAL_TO_KJ);
+    }
+}
+
+// Grid-stride over the terms, a wave at a time so that the energy sums run
+// in uniform control flow: one atomic per (system, term kind) in the wave.
 template <bool ENERGY>
 __device__ void bonded_body(const og_bonded_args& a) {
     const int total = a.nb + a.na + a.nd + a.ni;
-    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < total; k += gridDim.x * blockDim.x) {
-        if (k < a.nb) {
-            const int ia = a.bonds[2 * k], ib = a.bonds[2 * k + 1];
-            const double K = a.bond_p[2 * k], r0 = a.bond_p[2 * k + 1];
-            d3 d = sub(ld3(a.pos + 3 * ia), ld3(a.pos + 3 * ib));
-            d = neg(d);  // d = x_b - x_a
-            const double r = nrm(d);
-            d3 f = {0.0, 0.0, 0.0};
-            if (r >= 1e-9) {
-                const double mag = 2.0 * K * (r - r0) * OG_KCAL_TO_KJ;
-                f = mul(d, mag / r);
-            }
-            put(a.tf, 2ll * k, f);
-            put(a.tf, 2ll * k + 1, neg(f));
-            if (ENERGY) {
-                const double dr = r - r0;
-                og_atomic_add_fix(a.eacc + a.atom_sys[ia] * OG_NE + OG_E_BOND,
-                                  og_to_fix(K * dr * dr * OG_KCAL_TO_KJ));
-            }
-            continue;
-        }
-        int kk = k - a.nb;
-        if (kk < a.na) {
-            const int ia = a.angles[3 * kk], ib = a.angles[3 * kk + 1], ic = a.angles[3 * kk + 2];
-            const double K = a.angle_p[2 * kk], th0 = a.angle_p[2 * kk + 1];
-            const d3 pb = ld3(a.pos + 3 * ib);
-            const d3 u = sub(ld3(a.pos + 3 * ia), pb), v = sub(ld3(a.pos + 3 * ic), pb);
-            const double un = nrm(u), vn = nrm(v);
-            const long long s0 = 2ll * a.nb + 3ll * kk;
-            d3 fa = {0.0, 0.0, 0.0}, fc = {0.0, 0.0, 0.0};
-            if (un >= 1e-9 && vn >= 1e-9) {
-                const d3 uh = dv(u, un), vh = dv(v, vn);
-                double ct = fmin(fmax(dot(uh, vh), -1.0), 1.0);
-                const double st = sqrt(1.0 - ct * ct);
-                if (st >= 1e-9) {
-                    const double theta = acos(ct);
-                    const double coeff = 2.0 * K * (theta - th0) * OG_KCAL_TO_KJ / st;
-                    fa = mul(sub(vh, mul(uh, ct)), coeff / un);
-                    fc = mul(sub(uh, mul(vh, ct)), coeff / vn);
-                }
-                if (ENERGY) {
-                    const double dt = acos(fmin(fmax(dot(uh, vh), -1.0), 1.0)) - th0;
-                    og_atomic_add_fix(a.eacc + a.atom_sys[ia] * OG_NE + OG_E_ANGLE,
-                                      og_to_fix(K * dt * dt * OG_KCAL_TO_KJ));
-                }
-            }
-            put(a.tf, s0, fa);
-            put(a.tf, s0 + 1, neg(add(fa, fc)));
-            put(a.tf, s0 + 2, fc);
-            continue;
-        }
-        kk -= a.na;
-        const bool is_dih = kk < a.nd;
-        if (!is_dih) kk -= a.nd;
-        const int* q = is_dih ? a.dih + 4 * kk : a.imp + 4 * kk;
-        const d3 pa = ld3(a.pos + 3 * q[0]), pb = ld3(a.pos + 3 * q[1]), pc = ld3(a.pos + 3 * q[2]),
-                 pd = ld3(a.pos + 3 * q[3]);
-        d3 da, db, dc, dd;
-        double phi;
-        const long long s0 = 2ll * a.nb + 3ll * a.na + (is_dih ? 4ll * kk : 4ll * a.nd + 4ll * kk);
-        d3 f0 = {0.0, 0.0, 0.0}, f1 = f0, f2 = f0, f3 = f0;
-        if (dihedral_gradient(pa, pb, pc, pd, da, db, dc, dd, phi)) {
-            double dvdphi;
-            if (is_dih) {
-                const double kd = a.dih_p[3 * kk], nn = a.dih_p[3 * kk + 1], delta = a.dih_p[3 * kk + 2];
-                dvdphi = -kd * nn * sin(nn * phi - delta) * OG_KCAL_TO_KJ;
-            } else {
-                dvdphi = 2.0 * a.imp_p[2 * kk] * wrap_pi(phi - a.imp_p[2 * kk + 1]) * OG_KCAL_TO_KJ;
-            }
-            f0 = mul(da, -dvdphi);
-            f1 = mul(db, -dvdphi);
-            f2 = mul(dc, -dvdphi);
-            f3 = mul(dd, -dvdphi);
-        }
-        put(a.tf, s0, f0);
-        put(a.tf, s0 + 1, f1);
-        put(a.tf, s0 + 2, f2);
-        put(a.tf, s0 + 3, f3);
+    const int lane = threadIdx.x & 63;
+    const int stride = gridDim.x * blockDim.x;
+    for (int kb = blockIdx.x * blockDim.x + (threadIdx.x & ~63); kb < total; kb += stride) {
+        const int k = kb + lane;
+        int e_slot = -1, e_sys = 0;
+        long long e_fix = 0;
+        if (k < total) bonded_term<ENERGY>(a, k, e_slot, e_sys, e_fix);
         if (ENERGY) {
-            const double chi = measure_dihedral(pa, pb, pc, pd);
-            double e;
-            int slot;
-            if (is_dih) {
-                e = a.dih_p[3 * kk] * (1.0 + cos(a.dih_p[3 * kk + 1] * chi - a.dih_p[3 * kk + 2]));
-                slot = OG_E_DIHEDRAL;
-            } else {
-                const double dom = wrap_pi(chi - a.imp_p[2 * kk + 1]);
-                e = a.imp_p[2 * kk] * dom * dom;
-                slot = OG_E_IMPROPER;
-            }
-            og_atomic_add_fix(a.eacc + a.atom_sys[q[0]] * OG_NE + slot, og_to_fix(e * OG_KCAL_TO_KJ));
+            og_wave_add_fix_keyed(a.eacc, e_sys, OG_NE, OG_E_BOND, e_fix, e_slot == OG_E_BOND);
+            og_wave_add_fix_keyed(a.eacc, e_sys, OG_NE, OG_E_ANGLE, e_fix, e_slot == OG_E_ANGLE);
+            og_wave_add_fix_keyed(a.eacc, e_sys, OG_NE, OG_E_DIHEDRAL, e_fix, e_slot == OG_E_DIHEDRAL);
+            og_wave_add_fix_keyed(a.eacc, e_sys, OG_NE, OG_E_IMPROPER, e_fix, e_slot == OG_E_IMPROPER);
         }
     }
 }
@@ -208,46 +232,53 @@ void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
 template <bool ENERGY>
 __global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_atoms) return;
-    const int sys = a.atom_sys[i];
-    const bool sur = a.sur_mode && a.sur_mode[sys];
-    double R = 0.0;
-    if (!sur || a.check) {
-        const double rho = a.rho[i];
-        const double rt = rho - OG_OBC_OFFSET;
-        const double integral = (double)a.iacc[i] * OG_FIX_INV;
-        const double psi = integral * rt;
-        const double arg = 1.0 * psi - 0.8 * psi * psi + 4.85 * psi * psi * psi;
-        const double inv = 1.0 / rt - tanh(arg) / rho;
-        const double floor_r = rt > 0.5 ? rt : 0.5;
-        if (inv <= 0.0 || !isfinite(inv)) {
-            if (!sur) atomicAdd(a.clamped + sys, 1);
-            R = floor_r;
-        } else {
-            R = 1.0 / inv;
-            R = R > floor_r ? R : floor_r;
+    int sys = 0;
+    long long e_fix = 0;
+    bool has_e = false;
+    if (i < a.n_atoms) {
+        sys = a.atom_sys[i];
+        const bool sur = a.sur_mode && a.sur_mode[sys];
+        double R = 0.0;
+        if (!sur || a.check) {
+            const double rho = a.rho[i];
+            const double rt = rho - OG_OBC_OFFSET;
+            const double integral = (double)a.iacc[i] * OG_FIX_INV;
+            const double psi = integral * rt;
+            const double arg = 1.0 * psi - 0.8 * psi * psi + 4.85 * psi * psi * psi;
+            const double inv = 1.0 / rt - tanh(arg) / rho;
+            const double floor_r = rt > 0.5 ? rt : 0.5;
+            if (inv <= 0.0 || !isfinite(inv)) {
+                if (!sur) atomicAdd(a.clamped + sys, 1);
+                R = floor_r;
+            } else {
+                R = 1.0 / inv;
+                R = R > floor_r ? R : floor_r;
+            }
+        }
+        if (sur) {
+            const double Rs = a.r_sur[i];
+            if (a.check) {
+                const float dev = (float)(fabs(Rs - R) / R);
+                atomicMax(a.dev + sys, isfinite(dev) ? __float_as_uint(dev) : 0x7F800000u);
+            }
+            R = Rs;
+        }
+        a.radii[i] = R;
+        float4 g = a.gb[a.atom_slot[i]];
+        g.z = (float)R;
+        g.w = (float)(0.60056120439322491 / R);  // sqrt(log2(e)/4) / R
+        a.gb[a.atom_slot[i]] = g;
+        if (ENERGY) {
+            const double q = a.charge[i];
+            if (q != 0.0) {
+                const double pref = -0.5 * (1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
+                e_fix = og_to_fix(pref * q * q / R);
+                has_e = true;
+            }
         }
     }
-    if (sur) {
-        const double Rs = a.r_sur[i];
-        if (a.check) {
-            const float dev = (float)(fabs(Rs - R) / R);
-            atomicMax(a.dev + sys, isfinite(dev) ? __float_as_uint(dev) : 0x7F800000u);
-        }
-        R = Rs;
-    }
-    a.radii[i] = R;
-    float4 g = a.gb[a.atom_slot[i]];
-    g.z = (float)R;
-    g.w = (float)(0.60056120439322491 / R);  // sqrt(log2(e)/4) / R
-    a.gb[a.atom_slot[i]] = g;
-    if (ENERGY) {
-        const double q = a.charge[i];
-        if (q != 0.0) {
-            const double pref = -0.5 * (1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
-            og_atomic_add_fix(a.eacc + sys * OG_NE + OG_E_GB_SELF, og_to_fix(pref * q * q / R));
-        }
-    }
+    // GB self energy: one atomic per system in the wave
+    if (ENERGY) og_wave_add_fix_keyed(a.eacc, sys, OG_NE, OG_E_GB_SELF, e_fix, has_e);
 }
 
 void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s) {

@@ -47,6 +47,26 @@ __device__ __forceinline__ void og_atomic_add_fix(long long* p, long long v) {
     atomicAdd(reinterpret_cast<unsigned long long*>(p), static_cast<unsigned long long>(v));
 }
 
+// wave-wide int64 sum (device library reduction over DPP)
+extern "C" __device__ __attribute__((const)) long long __ockl_wfred_add_i64(long long);
+
+// Adds each active lane's fixed-point value v to acc[key * stride + slot].
+// Lanes with the same key are summed first (int64, exact and order-free), so
+// each distinct key in the wave costs one atomic instead of one per lane.
+// Call from uniform control flow (every lane of the wave).
+__device__ __forceinline__ void og_wave_add_fix_keyed(long long* acc, int key, int stride, int slot, long long v,
+                                                      bool active) {
+    unsigned long long pending = __ballot(active);
+    while (pending) {
+        const int leader = __builtin_ctzll(pending);
+        const int k = __shfl(key, leader, 64);
+        const bool mine = active && key == k;
+        const long long part = __ockl_wfred_add_i64(mine ? v : 0ll);
+        if ((int)(threadIdx.x & 63) == leader) og_atomic_add_fix(acc + (size_t)k * stride + slot, part);
+        pending &= ~__ballot(mine);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Wave64 cross-lane sum.  Row (16-lane) stages use DPP; the two cross-row
 // stages use ds_swizzle/bpermute through __shfl_xor.  Every lane ends with
