@@ -238,24 +238,91 @@ __device__ __forceinline__ void mlp_forward(const unsigned short* X, int row0, c
 
 }  // namespace
 
-// Inference: 4 waves x 16 rows per block.
-__global__ void __launch_bounds__(256) og_k_sur_infer(const unsigned short* X, int n_valid, MlpParams P,
-                                                      double* r_out) {
-    __shared__ __attribute__((aligned(16))) unsigned short hs[4][16][136];
+// Inference, 64 rows per wave: four 16-row groups share every weight
+// fragment load (the weights are read once per 64 rows instead of once per
+// 16).  Each row goes through the 16-row forward's exact operations, so the
+// radii are unchanged.
+__global__ void __launch_bounds__(256) og_k_sur_infer64(const unsigned short* X, int n_valid, int n_rows,
+                                                        MlpParams P, double* r_out) {
+    __shared__ __attribute__((aligned(16))) unsigned short hs[4][64][136];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int row0 = (blockIdx.x * 4 + w) * 16;
+    const int r = lane & 15, g = lane >> 4;
+    const int row0 = (blockIdx.x * 4 + w) * 64;
     if (row0 >= n_valid) return;
-    og_f32x4 z1[8], z2[8];
-    float y[4];
-    mlp_forward(X, row0, P, hs[w], lane, z1, z2, y);
-    if ((lane & 15) == 0) {
-        const int g = lane >> 4;
+    unsigned short (*h)[136] = hs[w];
+    og_bf16x8 a1[4][6];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg) {
+        const int rr = row0 + rg * 16 + r;
+        const int rs = rr < n_rows ? rr : 0;
+#pragma unroll
+        for (int kk = 0; kk < 6; ++kk) a1[rg][kk] = ld8(X + (size_t)rs * OG_SUR_K1 + kk * 32 + g * 8);
+    }
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+        og_f32x4 acc[4];
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg) acc[rg] = og_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 6; ++kk) {
+            const og_bf16x8 bw = ld8(P.W1t + (size_t)(nt * 16 + r) * OG_SUR_K1 + kk * 32 + g * 8);
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) acc[rg] = mfma(a1[rg][kk], bw, acc[rg]);
+        }
+        const float b = P.p[OG_SUR_OFF_B1 + nt * 16 + r];
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float z = acc[rg][i] + b;
+                h[rg * 16 + g * 4 + i][nt * 16 + r] = f2bf(silu(z));
+            }
+    }
+    wave_sync();
+    og_bf16x8 a2[4][4];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) a2[rg][kk] = ld8(&h[rg * 16 + r][kk * 32 + g * 8]);
+    float ys[4][4];
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+        for (int i = 0; i < 4; ++i) ys[rg][i] = 0.f;
+#pragma unroll
+    for (int nt = 0; nt < 8; ++nt) {
+        og_f32x4 acc[4];
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg) acc[rg] = og_f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const og_bf16x8 bw = ld8(P.W2t + (size_t)(nt * 16 + r) * OG_SUR_H + kk * 32 + g * 8);
+#pragma unroll
+            for (int rg = 0; rg < 4; ++rg) acc[rg] = mfma(a2[rg][kk], bw, acc[rg]);
+        }
+        const float b = P.p[OG_SUR_OFF_B2 + nt * 16 + r];
+        const float wv = P.p[OG_SUR_OFF_W3 + nt * 16 + r];
+#pragma unroll
+        for (int rg = 0; rg < 4; ++rg)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const float z = acc[rg][i] + b;
+                ys[rg][i] += silu(z) * wv;
+            }
+    }
+#pragma unroll
+    for (int rg = 0; rg < 4; ++rg)
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const int row = row0 + g * 4 + i;
-            if (row < n_valid) r_out[row] = OG_SUR_R0 + OG_SUR_RS * (double)y[i];
+            float v = ys[rg][i];
+            v += __shfl_xor(v, 1, 64);
+            v += __shfl_xor(v, 2, 64);
+            v += __shfl_xor(v, 4, 64);
+            v += __shfl_xor(v, 8, 64);
+            const float y = v + P.p[OG_SUR_OFF_B3];
+            const int row = row0 + rg * 16 + g * 4 + i;
+            if (r == 0 && row < n_valid) r_out[row] = OG_SUR_R0 + OG_SUR_RS * (double)y;
         }
-    }
 }
 
 // Training forward + backward for 16 batch rows per wave.
@@ -593,9 +660,9 @@ int og_sur_descriptors(og_sur* s, bool all) {
 
 int og_sur_infer(og_sur* s) {
     if (s->n_atoms == 0) return 0;
-    const int waves = s->n_pad / 16;
-    hipLaunchKernelGGL(og_k_sur_infer, dim3((waves + 3) / 4), dim3(256), 0, s->stream, s->X, s->n_atoms,
-                       params_of(s), s->r_sur);
+    const int waves64 = (s->n_atoms + 63) / 64;
+    hipLaunchKernelGGL(og_k_sur_infer64, dim3((waves64 + 3) / 4), dim3(256), 0, s->stream, s->X, s->n_atoms,
+                       s->n_pad, params_of(s), s->r_sur);
     return hipGetLastError() == hipSuccess ? 0 : fail("kernel launch failed");
 }
 
