@@ -340,6 +340,22 @@ def main():
                                     "13.2 MB, the rest is int64 fixed-point atomics")
     dominant = {"kernel_class": dom, "ms_per_step": totals[dom] / max(args.profile_steps, 1),
                 "bound": "latency / divergence (FP64 PowerSasa arrangement)" if dom == "sasa" else None}
+    # VALU activity of the SASA arrangement and the force kernel, from the
+    # rocprofv3 --pmc run summarised in profiles/ (same batch)
+    vf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "valu_64chains_157k_atoms.json")
+    if os.path.exists(vf):
+        vj = json.load(open(vf))
+        if vj.get("atoms") == info["atoms"]:
+            ka = vj["kernels"].get("void og_k_sasa_arrange<false, 80>")
+            if ka and dom == "sasa":
+                dominant["valu_busy"] = ka["valu_busy"]
+                dominant["valu_lane_utilization"] = ka["valu_lane_utilization"]
+                dominant["valu_insts_per_sasa_atom"] = ka["valu_insts"] / max(vj.get("sasa_atoms", 1), 1)
+                dominant["valu_note"] = "og_k_sasa_arrange from profiles/valu_64chains_157k_atoms.md"
+            kf = vj["kernels"].get("og_k_force_b")
+            if kf:
+                roof["valu_busy"] = kf["valu_busy"]
+                roof["valu_lane_utilization"] = kf["valu_lane_utilization"]
     # FP32 pair kernels beside it
     pk = {}
     for k in ("descreen", "force"):

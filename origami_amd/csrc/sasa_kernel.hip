@@ -572,7 +572,8 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                         float t = 0.0f;
                         for (int step = 0; step <= nper + 1; ++step) {
                             const float p = start + t;
-                            const float best = wave_max_f(fmaxf(cover_ext(p, e0), cover_ext(p, e1)));
+                            const float ext = nper > 64 ? fmaxf(cover_ext(p, e0), cover_ext(p, e1)) : cover_ext(p, e0);
+                            const float best = wave_max_f(ext);
                             if (!(best > 0.0f)) break;
                             t += best;
                             if (t >= len + SA_COV_EPS) { covered = true; break; }
