@@ -1135,29 +1135,65 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
     for (int x = lane; x <= nc; x += 64) {
         double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
         bool any = false;
-        for (int A = rec.x; A < rec.x + na; ++A) {
-            const unsigned xr = d.arc_code[A];
-            int role = -1;
-            if (x == 0) role = 0;
-            else if (x == (int)(xr & 0xFF)) role = 1;
-            else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
-            else if (x == (int)((xr >> 16) & 0xFF)) role = 3;
-            if (role >= 0) {
-                const double* T = d.arc_terms + 12ll * A + 3 * role;
-                sa0 += T[0]; sa1 += T[1]; sa2 += T[2];
-                any = true;
+        // four arcs (vertices) per step: their loads are issued together and
+        // the sums still run in arc (vertex) order
+        for (int A0 = 0; A0 < na; A0 += 4) {
+            double t[4][3];
+            bool hit[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                hit[q] = false;
+                t[q][0] = t[q][1] = t[q][2] = 0.0;
+                if (A0 + q < na) {
+                    const int A = rec.x + A0 + q;
+                    const unsigned xr = d.arc_code[A];
+                    int role = -1;
+                    if (x == 0) role = 0;
+                    else if (x == (int)(xr & 0xFF)) role = 1;
+                    else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
+                    else if (x == (int)((xr >> 16) & 0xFF)) role = 3;
+                    if (role >= 0) {
+                        const double* T = d.arc_terms + 12ll * A + 3 * role;
+                        t[q][0] = T[0]; t[q][1] = T[1]; t[q][2] = T[2];
+                        hit[q] = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (hit[q]) {
+                    sa0 += t[q][0]; sa1 += t[q][1]; sa2 += t[q][2];
+                    any = true;
+                }
             }
         }
-        for (int V = rec.y; V < rec.y + nv; ++V) {
-            const unsigned xr = d.vtx_code[V];
-            int role = -1;
-            if (x == 0) role = 0;
-            else if (x == (int)(xr & 0xFF)) role = 1;
-            else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
-            if (role >= 0) {
-                const double* T = d.vtx_terms + 9ll * V + 3 * role;
-                sv0 += T[0]; sv1 += T[1]; sv2 += T[2];
-                any = true;
+        for (int V0 = 0; V0 < nv; V0 += 4) {
+            double t[4][3];
+            bool hit[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                hit[q] = false;
+                t[q][0] = t[q][1] = t[q][2] = 0.0;
+                if (V0 + q < nv) {
+                    const int V = rec.y + V0 + q;
+                    const unsigned xr = d.vtx_code[V];
+                    int role = -1;
+                    if (x == 0) role = 0;
+                    else if (x == (int)(xr & 0xFF)) role = 1;
+                    else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
+                    if (role >= 0) {
+                        const double* T = d.vtx_terms + 9ll * V + 3 * role;
+                        t[q][0] = T[0]; t[q][1] = T[1]; t[q][2] = T[2];
+                        hit[q] = true;
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                if (hit[q]) {
+                    sv0 += t[q][0]; sv1 += t[q][1]; sv2 += t[q][2];
+                    any = true;
+                }
             }
         }
         if (!any) continue;
