@@ -904,7 +904,9 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     float best_s = 0.0f;
     if ((flags & OG_SASA) && e->n_sasa_work > 0) {
         best_s = 1e30f;
-        for (int per_cu : {2, 4, 8}) {
+        // 6 blocks of 2 waves per CU is the arrangement kernel's residency
+        // (LDS and VGPRs): one round of waves, no late starters
+        for (int per_cu : {2, 3, 4, 6, 8}) {
             const int g = std::min(per_cu * n_cu, og_sasa_max_blocks(&e->sasa));
             og_sasa_set_blocks(&e->sasa, g);
             float ms = 0.0f;

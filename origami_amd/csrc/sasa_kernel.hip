@@ -1039,10 +1039,25 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     const unsigned long long lt = lanemask_lt(lane);
     const bool second = NC == MAXC;
     const int n_items = second ? __builtin_amdgcn_readfirstlane(d.counters[3]) : d.n_work;
-    for (int it = slot; it < n_items; it += n_slots) {
+    // The first three quarters of the atoms are split statically; the rest
+    // are handed out one at a time by a queue counter, which evens out the
+    // several-fold spread in cost per atom.  Every wave leaves the loop once
+    // the counter has passed the last atom.  (The second pass, with few
+    // atoms, is split statically.)
+    const int n_static = second ? n_items : (n_items / 4) * 3;
+    for (int it = slot; it < n_static; it += n_slots) {
         const int w = second ? d.deferred[it] : it;
         const int4 rec = arrange_atom<AREA, NC>(L, d, pos, w, lane, lt, wd, wi);
         if (lane == 0) d.rec[w] = rec;
+        wave_sync();
+    }
+    while (!second) {
+        int it = 0;
+        if (lane == 0) it = atomicAdd(d.counters + 4, 1);
+        it = n_static + __shfl(it, 0, 64);
+        if (it >= n_items) break;
+        const int4 rec = arrange_atom<AREA, NC>(L, d, pos, it, lane, lt, wd, wi);
+        if (lane == 0) d.rec[it] = rec;
         wave_sync();
     }
 }
@@ -1225,7 +1240,7 @@ __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const
 }
 
 __global__ void og_k_sasa_reset(int* counters) {
-    if (threadIdx.x < 4) counters[threadIdx.x] = 0;
+    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
 int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys,
@@ -1255,7 +1270,7 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
             d->n_cu = prop.multiProcessorCount;
     }
     int blocks = (d->n_work + SA_WAVES - 1) / SA_WAVES;
-    blocks = std::min(blocks, d->n_cu * 8);
+    blocks = std::min(blocks, d->n_cu * 8);   // staging for up to 8 blocks per CU (autotuner range)
     d->blocks = std::max(blocks, 1);
     d->stride = d->blocks * SA_WAVES;
     // pool capacity: OG_SASA_POOL_ARCS arcs and vertices per SASA atom on average
@@ -1310,7 +1325,7 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
         if (!up(&d->probes, pr) || !up(&d->padj_start, st) || !up(&d->padj, adj))
             return fail("device allocation failed");
     }
-    if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 4) ||
+    if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 8) ||
         !zalloc(&d->rec, (size_t)d->n_work) || !zalloc(&d->deferred, (size_t)d->n_work))
         return fail("device allocation failed");
     if (d->n_work > 0) {
