@@ -58,6 +58,11 @@ struct og_engine {
     std::vector<double> temps, dofcorr;
     std::vector<int> n_cons_sys;
     hipStream_t stream = nullptr;
+    // SASA runs on its own stream beside the all-pairs chain (fork after the
+    // positions are final, join before the kick): the arrangement kernel is
+    // latency bound and the pair kernels fill its idle issue slots
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // device buffers
     double *pos = nullptr, *vel = nullptr, *force = nullptr, *ref = nullptr;
     long long *facc = nullptr, *iacc = nullptr, *eacc = nullptr, *ke = nullptr;
@@ -174,6 +179,9 @@ static void free_all(og_engine* e) {
         hipEventDestroy(p.second.first);
         hipEventDestroy(p.second.second);
     }
+    if (e->fork_ev) hipEventDestroy(e->fork_ev);
+    if (e->join_ev) hipEventDestroy(e->join_ev);
+    if (e->side) hipStreamDestroy(e->side);
     if (e->stream) hipStreamDestroy(e->stream);
 }
 
@@ -240,12 +248,16 @@ extern "C" int og_engine_create(const og_system_desc* systems, int32_t n_systems
 extern "C" void og_engine_destroy(og_engine* e) {
     if (!e) return;
     if (e->stream) hipStreamSynchronize(e->stream);
+    if (e->side) hipStreamSynchronize(e->side);
     free_all(e);
     delete e;
 }
 
 static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&e->fork_ev, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&e->join_ev, hipEventDisableTiming));
     e->n_sys = n_sys;
     e->atom_off.resize(n_sys + 1, 0);
     e->sys_n.resize(n_sys);
@@ -564,6 +576,19 @@ static void invalidate_graphs(og_engine* e) {
 // sur_check: -1 = decide from the call counter; 0 / 1 = forced (graph capture)
 static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1) {
     const bool mode_a = flags & OG_MODE_A;
+    // SASA on the side stream, overlapping everything up to the join below.
+    // Both branches add into the force and energy accumulators with integer
+    // atomics only, so the sums do not depend on the interleaving.  The
+    // per-kernel profile keeps one stream so its event pairs time each class.
+    const bool fork = (flags & OG_SASA) && !e->prof;
+    if (fork) {
+        HIPCHK(hipEventRecord(e->fork_ev, e->stream));
+        HIPCHK(hipStreamWaitEvent(e->side, e->fork_ev, 0));
+        const int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->side,
+                                      energy ? e->eacc : nullptr, e->atom_sys);
+        if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
+        HIPCHK(hipEventRecord(e->join_ev, e->side));
+    }
     {
         ProfScope p(e, OG_K_BONDED);
         og_bonded_args b{};
@@ -622,7 +647,9 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         en.rc2_nb = 100.0; en.kappa = 1.0 / 2000.0; en.crf = 3.0 / 20.0;
         og_launch_pair_energy64(a, en, mode_a, e->stream);
     }
-    if (flags & OG_SASA) {
+    if (fork) {
+        HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
+    } else if (flags & OG_SASA) {
         ProfScope p(e, OG_K_SASA);
         int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream,
                                 energy ? e->eacc : nullptr, e->atom_sys);
@@ -866,7 +893,10 @@ extern "C" int og_use_graphs(og_engine* e, int32_t on) {
 // Autotuner: times the candidate launch widths of the all-pairs kernels and
 // of the SASA arrangement kernel on the engine's current positions and keeps
 // the fastest.  Candidates: pair kernels 2..32 blocks of 256 threads per CU;
-// SASA 2..8 blocks of 2 waves per CU.
+// SASA 2..8 blocks of 2 waves per CU.  With SASA the widths are timed on the
+// whole force evaluation, SASA on its stream beside the pair chain, since the
+// best split of the CUs between the two depends on both: first the SASA width
+// with the pair width that is fastest alone, then the pair width again.
 extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     int dev = 0, n_cu = 256;
@@ -903,17 +933,32 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     int best_sasa = og_sasa_max_blocks(&e->sasa);
     float best_s = 0.0f;
     if ((flags & OG_SASA) && e->n_sasa_work > 0) {
-        best_s = 1e30f;
         // 6 blocks of 2 waves per CU is the arrangement kernel's residency
-        // (LDS and VGPRs): one round of waves, no late starters
-        for (int per_cu : {2, 3, 4, 6, 8}) {
+        // (LDS and VGPRs); fewer leave room for the pair kernels beside it
+        const uint32_t fl = (flags & OG_MODE_A) | OG_SASA;
+        int lrc = OG_OK;
+        auto evaluation = [&] { if (!lrc) lrc = launch_forces(e, fl, 1.0, false); };
+        best_s = 1e30f;
+        for (int per_cu : {2, 3, 4, 5, 6, 8}) {
             const int g = std::min(per_cu * n_cu, og_sasa_max_blocks(&e->sasa));
             og_sasa_set_blocks(&e->sasa, g);
             float ms = 0.0f;
-            if (int rc = timed([&] { og_sasa_launch(&e->sasa, e->pos, e->facc, 1.0, e->stream); }, ms)) return rc;
+            if (int rc = timed(evaluation, ms)) return rc;
+            if (lrc) return lrc;
             if (ms < best_s) { best_s = ms; best_sasa = g; }
         }
         og_sasa_set_blocks(&e->sasa, best_sasa);
+        for (int per_cu : {2, 4, 8, 16, 32}) {
+            const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
+            const int keep = e->pair_grid;
+            e->pair_grid = g;
+            float ms = 0.0f;
+            if (int rc = timed(evaluation, ms)) return rc;
+            if (lrc) return lrc;
+            if (ms < best_s) { best_s = ms; best_grid = g; } else { e->pair_grid = keep; }
+            if (g == std::max((e->n_work + 3) / 4, 1)) break;
+        }
+        e->pair_grid = best_grid;
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

@@ -156,17 +156,39 @@ __device__ __forceinline__ bool contains_margin(double4 c, v3 m) {
     return c.x * m.x + c.y * m.y + c.z * m.z > c.w + 1e-4;
 }
 
-// wave-wide maximum (device library reduction over DPP)
-extern "C" __device__ __attribute__((const)) float __ockl_wfred_max_f32(float);
-__device__ __forceinline__ float wave_max_f(float v) { return __ockl_wfred_max_f32(v); }
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ unsigned dpp_umax(unsigned acc, unsigned src) {
+    const unsigned m = (unsigned)__builtin_amdgcn_update_dpp(0, (int)src, CTRL, ROWS, BANKS, false);
+    return acc > m ? acc : m;
+}
+
+// Wave-wide maximum of non-negative floats (all lanes active): their bit
+// patterns order as unsigned integers, so the reduction is an integer one over
+// DPP (rows: shr 1, 2, 3 of the input, shr 4 and 8 within the row, then the
+// row broadcasts 15 and 31) and lane 63 ends with the maximum.  Lanes that a
+// DPP step does not write read 0, the identity.
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+    const unsigned x = __float_as_uint(v);
+    unsigned y = dpp_umax<0x111, 0xF, 0xF>(x, x);
+    y = dpp_umax<0x112, 0xF, 0xF>(y, x);
+    y = dpp_umax<0x113, 0xF, 0xF>(y, x);
+    y = dpp_umax<0x114, 0xF, 0xE>(y, y);
+    y = dpp_umax<0x118, 0xF, 0xC>(y, y);
+    y = dpp_umax<0x142, 0xA, 0xF>(y, y);
+    y = dpp_umax<0x143, 0xC, 0xF>(y, y);
+    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)y, 63));
+}
 
 // Union cover walk: how far past angle p the arc (start e.x, width e.y) reaches
-// when it contains p with SA_COV_EPS to spare, else 0.
+// when it contains p with SA_COV_EPS to spare, else 0.  The turn count comes
+// from a multiply by 1/(2 pi): where it differs from the rounded quotient the
+// reduced angle lies within 4e-6 of 0 or 2 pi, which both forms reject (arc
+// widths stay below 2 pi - 0.08), so the result equals the divided form's.
 __device__ __forceinline__ float cover_ext(float p, float2 e) {
     if (!(e.y > 0.0f)) return 0.0f;
     const float two_pi = 6.28318530717958647692f;
     float d = p - e.x;
-    d -= two_pi * floorf(d / two_pi);
+    d -= two_pi * floorf(d * (1.0f / two_pi));
     return (d > SA_COV_EPS && d < e.y - SA_COV_EPS) ? e.y - d : 0.0f;
 }
 
@@ -563,7 +585,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                 const float2 e1 = lane + 64 < nper ? iv[ci * nper + lane + 64] : make_float2(0.0f, -1.0f);
                 bool covered = __ballot(e0.y >= SA_COV_FULL || e1.y >= SA_COV_FULL) != 0;
                 if (!covered) {
-                    const float wmax = wave_max_f(fmaxf(e0.y, e1.y));
+                    const float wmax = wave_max_nonneg(fmaxf(fmaxf(e0.y, e1.y), 0.0f));
                     if (wmax > 0.0f) {
                         const unsigned long long b0 = __ballot(e0.y == wmax), b1 = __ballot(e1.y == wmax);
                         const int l1 = b0 ? __builtin_ctzll(b0) : __builtin_ctzll(b1);
@@ -573,7 +595,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                         for (int step = 0; step <= nper + 1; ++step) {
                             const float p = start + t;
                             const float ext = nper > 64 ? fmaxf(cover_ext(p, e0), cover_ext(p, e1)) : cover_ext(p, e0);
-                            const float best = wave_max_f(ext);
+                            const float best = wave_max_nonneg(ext);
                             if (!(best > 0.0f)) break;
                             t += best;
                             if (t >= len + SA_COV_EPS) { covered = true; break; }

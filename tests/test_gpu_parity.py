@@ -327,6 +327,34 @@ def test_autotune_keeps_results(E, c2_system):
     assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0
 
 
+def test_sasa_side_stream_matches_serial(E, c1_system, c2_system):
+    """SASA on its own stream beside the pair kernels (the default) against the
+    one-stream order that the per-kernel profile uses: forces, energies and
+    captured SASA dynamics bitwise equal."""
+    systems = [c1_system, c2_system]
+    ovl, ser = E.Engine(systems), E.Engine(systems)
+    ser.profile(True)
+    for eng in (ovl, ser):
+        eng.set_positions(np.concatenate([jittered(s, 11) for s in systems]))
+    fa = ovl.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    fb = ser.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+    assert np.array_equal(fa, fb)
+    ea = ovl.total_energy(include_sasa=True)
+    eb = ser.total_energy(include_sasa=True)
+    assert ea == eb
+    for eng in (ovl, ser):
+        v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(21 + k))
+                            for k, s in enumerate(systems)])
+        eng.set_velocities(v)
+        eng.dynamics_setup([300.0, 330.0], dt_fs=2.0, friction_ps_inv=2.0, seed=5,
+                           constrain_h_bonds=True, include_sasa=True, sasa_gamma_scale=0.25)
+    for rep in range(3):
+        ovl.run(4)                      # captured steps, two streams
+        ser.run(4)                      # profiled: direct launches, one stream
+        assert np.array_equal(ovl.positions(), ser.positions()), rep
+        assert np.array_equal(ovl.velocities(), ser.velocities()), rep
+
+
 def test_sasa_energy_matches_oracle(E, c1_system, c2_system):
     """PowerSasa energy sum_i gamma_i A_i (powersasa_energy, no gamma scale) on
     the GPU against the oracle's per-atom accessible areas, generic geometry,
