@@ -196,7 +196,7 @@ def _cover_walk(arcs):
     for _ in range(len(arcs) + 2):
         p = np.float32(start + t)
         d = (p - arcs[:, 0]).astype(np.float32)
-        d = d - two_pi * np.floor(d / two_pi)
+        d = d - two_pi * np.floor(d * (np.float32(1) / two_pi))
         ext = np.where((w > 0) & (d > eps) & (d < w - eps), w - d, 0).astype(np.float32)
         best = ext.max()
         if not best > 0:
@@ -205,6 +205,44 @@ def _cover_walk(arcs):
         if t >= length + eps:
             return True
     return False
+
+
+def _cover_ext(d, ey, inverse):
+    """The walk's extension from reduced offset d = p - start of an arc."""
+    eps = np.float32(1e-5)
+    two_pi = np.float32(2 * np.pi)
+    q = d * (np.float32(1) / two_pi) if inverse else d / two_pi
+    d = d - two_pi * np.floor(q)
+    return np.where((ey > 0) & (d > eps) & (d < ey - eps), ey - d, 0).astype(np.float32)
+
+
+def test_cover_turn_count_by_reciprocal():
+    # The walk counts whole turns with floor(d * (1 / 2 pi)) instead of the
+    # rounded quotient d / (2 pi).  The two quotients are within 2 ulp, so the
+    # floors can differ only where d lies within a few 1e-6 of a multiple of
+    # 2 pi; there the reduced angle is near 0 or near 2 pi and both forms give
+    # no extension, since arc widths stay below 2 pi - 0.08.  Every float32 d
+    # within 1e-4 of the multiples in the walk's range (d in [-3 pi, 7 pi])
+    # against a grid of widths, and uniform samples elsewhere.
+    two_pi = np.float32(2 * np.pi)
+    inv = np.float32(1) / two_pi
+    ds = []
+    for k in (-1, 1, 2, 3):
+        c = np.float32(k * 2 * np.pi)
+        x, hi = np.float32(c - np.float32(1e-4)), np.float32(c + np.float32(1e-4))
+        while x <= hi:
+            ds.append(x)
+            x = np.nextafter(x, np.float32(np.inf))
+    d = np.array(ds, dtype=np.float32)[:, None]
+    ey = np.concatenate([np.linspace(1e-3, 2 * np.pi - 0.09, 61),
+                         [2 * np.pi - 0.0894]]).astype(np.float32)[None, :]
+    assert np.array_equal(_cover_ext(d, ey, False), _cover_ext(d, ey, True))
+    rng = np.random.default_rng(12)
+    u = rng.uniform(-3 * np.pi, 7 * np.pi, 2_000_000).astype(np.float32)
+    far = np.abs(u / (2 * np.pi) - np.round(u / (2 * np.pi))) * 2 * np.pi > 1e-4
+    assert np.array_equal(np.floor(u[far] / two_pi), np.floor(u[far] * inv))
+    e = rng.uniform(1e-3, 2 * np.pi - 0.09, u.size).astype(np.float32)
+    assert np.array_equal(_cover_ext(u, e, False), _cover_ext(u, e, True))
 
 
 def test_cover_walk_on_chain_atoms(host):
