@@ -141,6 +141,15 @@ int og_use_graphs(og_engine* e, int32_t on);
  * accumulation is order-independent fixed point. */
 int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms);
 
+/* Launch widths without timing (no counterpart in origami), e.g. to restore
+ * an earlier og_autotune choice: pair_blocks and sasa_blocks as in choice[]
+ * (<= 0 keeps the current value; clamped to the staging the engine holds).
+ * sasa_passes: 0 = the arrangement runs in one pass holding 128 caps per atom
+ * when that grid fits the LDS, else a pass holding 80 and a second pass over
+ * the atoms with more; 1 or 2 forces the one- or two-pass form.  Results do
+ * not depend on any of these. */
+int og_set_launch(og_engine* e, int32_t pair_blocks, int32_t sasa_blocks, int32_t sasa_passes);
+
 /* Set the bath temperature of one system (REMD ladder assignment). */
 int og_set_temperature(og_engine* e, int32_t system, double temperature_k);
 

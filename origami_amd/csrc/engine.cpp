@@ -897,6 +897,16 @@ extern "C" int og_use_graphs(og_engine* e, int32_t on) {
 // whole force evaluation, SASA on its stream beside the pair chain, since the
 // best split of the CUs between the two depends on both: first the SASA width
 // with the pair width that is fastest alone, then the pair width again.
+extern "C" int og_set_launch(og_engine* e, int32_t pair_blocks, int32_t sasa_blocks, int32_t sasa_passes) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (sasa_passes < 0 || sasa_passes > 2) return fail(OG_ERR_ARG, "sasa_passes must be 0, 1 or 2");
+    if (pair_blocks > 0) e->pair_grid = std::min(pair_blocks, std::max((e->n_work + 3) / 4, 1));
+    if (sasa_blocks > 0) og_sasa_set_blocks(&e->sasa, sasa_blocks);
+    og_sasa_set_passes(&e->sasa, sasa_passes);
+    invalidate_graphs(e);
+    return OG_OK;
+}
+
 extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     int dev = 0, n_cu = 256;
@@ -933,6 +943,7 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     int best_sasa = og_sasa_max_blocks(&e->sasa);
     float best_s = 0.0f;
     if ((flags & OG_SASA) && e->n_sasa_work > 0) {
+        og_sasa_set_passes(&e->sasa, 0);
         // 6 blocks of 2 waves per CU is the arrangement kernel's residency
         // (LDS and VGPRs); fewer leave room for the pair kernels beside it
         const uint32_t fl = (flags & OG_MODE_A) | OG_SASA;

@@ -15,6 +15,9 @@ struct og_sasa_data {
     int n_work = 0;               // atoms with gamma != 0
     int n_cu = 0;
     int blocks = 0;               // launch width of the arrangement kernel
+    bool one_pass = false;        // arrangement in one pass of MAXC caps (no second pass)
+    int passes = 0;               // 0: one pass when the blocks fit its LDS; 1 or 2: forced
+    long long lds_cu = 0;         // LDS bytes per CU
     int stride = 0;               // waves launched (= per-wave staging slots)
     int n_tiles = 0;
     double max_radius = 0.0;      // largest SASA radius (neighbour culling)
@@ -66,6 +69,8 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
 // Arrangement-kernel width (blocks of 2 waves); at most the staging slots allocated.
 int og_sasa_max_blocks(const og_sasa_data* d);
 void og_sasa_set_blocks(og_sasa_data* d, int blocks);
+// 0: one pass when the grid fits the LDS of MAXC caps, else two; 1 or 2: forced
+void og_sasa_set_passes(og_sasa_data* d, int passes);
 // Reads and clears the overflow count.
 int og_sasa_overflow(og_sasa_data* d, int* count);
 const char* og_sasa_error();

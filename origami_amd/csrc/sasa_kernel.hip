@@ -71,9 +71,10 @@ using namespace og_sasa;
 
 namespace {
 
-// Per-wave LDS of the arrangement, for atoms with at most NC caps.  The
-// main pass uses NC = SA_NC_MAIN (2 waves per block, 6 blocks per CU); the
-// atoms with more caps are deferred to a second pass with NC = MAXC.
+// Per-wave LDS of the arrangement, for atoms with at most NC caps.  A grid
+// of up to 5 blocks per CU fits NC = MAXC and runs one pass; wider grids use
+// NC = SA_NC_MAIN (6 blocks per CU fit) and defer the atoms with more caps to
+// a second pass with NC = MAXC.
 template <int NC>
 struct ArrLds {
     double4 cap4[NC];                                // caps: axis, cos(alpha)
@@ -1045,9 +1046,10 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
 
 }  // namespace
 
-// Main pass (NC = SA_NC_MAIN): every SASA atom; atoms with more caps are
-// listed for the second pass (NC = MAXC), which runs over that list.
-template <bool AREA, int NC>
+// Main pass (SECOND = false): every SASA atom; with NC < MAXC the atoms with
+// more caps are listed for the second pass (SECOND, NC = MAXC), which runs
+// over that list.
+template <bool AREA, int NC, bool SECOND>
 __global__ void __launch_bounds__(64 * SA_WAVES) __attribute__((amdgpu_waves_per_eu(3)))
 og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     __shared__ ArrLds<NC> lds_all[SA_WAVES];
@@ -1059,7 +1061,8 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     double* wd = d.scratch + (size_t)slot * WD_TOTAL;
     int* wi = d.iscratch + (size_t)slot * WI_TOTAL;
     const unsigned long long lt = lanemask_lt(lane);
-    const bool second = NC == MAXC;
+    static_assert(!SECOND || NC == MAXC, "the second pass holds MAXC caps");
+    const bool second = SECOND;
     const int n_items = second ? __builtin_amdgcn_readfirstlane(d.counters[3]) : d.n_work;
     // The first three quarters of the atoms are split statically; the rest
     // are handed out one at a time by a queue counter, which evens out the
@@ -1286,15 +1289,19 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
     d->tile_cnt = tile_cnt_dev;
     int dev = 0;
     d->n_cu = 256;
+    d->lds_cu = 64 * 1024;
     if (hipGetDevice(&dev) == hipSuccess) {
         hipDeviceProp_t prop;
-        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+        if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) {
             d->n_cu = prop.multiProcessorCount;
+            if (prop.maxSharedMemoryPerMultiProcessor > 0) d->lds_cu = (long long)prop.maxSharedMemoryPerMultiProcessor;
+        }
     }
     int blocks = (d->n_work + SA_WAVES - 1) / SA_WAVES;
     blocks = std::min(blocks, d->n_cu * 8);   // staging for up to 8 blocks per CU (autotuner range)
     d->blocks = std::max(blocks, 1);
     d->stride = d->blocks * SA_WAVES;
+    og_sasa_set_blocks(d, d->blocks);
     // pool capacity: OG_SASA_POOL_ARCS arcs and vertices per SASA atom on average
     const long long per = OG_SASA_POOL_ARCS;
     d->arc_cap = (int)std::min<long long>(per * std::max(d->n_work, 1), 1ll << 30);
@@ -1383,12 +1390,20 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles), dim3(64), 0, s, pos, d->tile_atom0,
                        d->tile_cnt, d->n_tiles, d->tile_bound);
     hipLaunchKernelGGL(og_k_sasa_reset, dim3(1), dim3(64), 0, s, d->counters);
-    if (area) {
-        hipLaunchKernelGGL((og_k_sasa_arrange<true, SA_NC_MAIN>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
-        hipLaunchKernelGGL((og_k_sasa_arrange<true, MAXC>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+    // One pass with MAXC caps when the grid fits its LDS; otherwise the
+    // SA_NC_MAIN pass and the second pass over the atoms it deferred (usually
+    // few or none), one block per CU.
+    const int blocks2 = d->blocks < d->n_cu ? d->blocks : d->n_cu;
+    const dim3 blk(64 * SA_WAVES);
+    if (d->one_pass) {
+        if (area) hipLaunchKernelGGL((og_k_sasa_arrange<true, MAXC, false>), dim3(d->blocks), blk, 0, s, *d, pos);
+        else hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC, false>), dim3(d->blocks), blk, 0, s, *d, pos);
+    } else if (area) {
+        hipLaunchKernelGGL((og_k_sasa_arrange<true, SA_NC_MAIN, false>), dim3(d->blocks), blk, 0, s, *d, pos);
+        hipLaunchKernelGGL((og_k_sasa_arrange<true, MAXC, true>), dim3(blocks2), blk, 0, s, *d, pos);
     } else {
-        hipLaunchKernelGGL((og_k_sasa_arrange<false, SA_NC_MAIN>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
-        hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC>), dim3(d->blocks), dim3(64 * SA_WAVES), 0, s, *d, pos);
+        hipLaunchKernelGGL((og_k_sasa_arrange<false, SA_NC_MAIN, false>), dim3(d->blocks), blk, 0, s, *d, pos);
+        hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC, true>), dim3(blocks2), blk, 0, s, *d, pos);
     }
     hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
     hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
@@ -1405,6 +1420,14 @@ int og_sasa_max_blocks(const og_sasa_data* d) { return d->stride / SA_WAVES; }
 void og_sasa_set_blocks(og_sasa_data* d, int blocks) {
     const int mx = d->stride / SA_WAVES;
     d->blocks = blocks < 1 ? 1 : (blocks > mx ? mx : blocks);
+    const long long per_block = (long long)SA_WAVES * sizeof(ArrLds<MAXC>);
+    d->one_pass = d->passes == 1 ||
+                  (d->passes == 0 && (long long)d->blocks <= (long long)d->n_cu * (d->lds_cu / per_block));
+}
+
+void og_sasa_set_passes(og_sasa_data* d, int passes) {
+    d->passes = passes;
+    og_sasa_set_blocks(d, d->blocks);
 }
 
 int og_sasa_overflow(og_sasa_data* d, int* count) {

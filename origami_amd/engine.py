@@ -98,6 +98,7 @@ def lib():
         L.og_profile_read.argtypes = [vp, _dp, C.POINTER(C.c_int64)]
         L.og_use_graphs.argtypes = [vp, C.c_int32]
         L.og_autotune.argtypes = [vp, C.c_uint32, _ip, _dp]
+        L.og_set_launch.argtypes = [vp, C.c_int32, C.c_int32, C.c_int32]
         _fp = C.POINTER(C.c_float)
         L.og_surrogate_setup.argtypes = [vp, C.c_int64, C.c_uint64]
         L.og_surrogate_collect.argtypes = [vp, C.c_uint32, C.POINTER(C.c_int64)]
@@ -283,6 +284,11 @@ class Engine:
 
     def use_graphs(self, on=True):
         _check(lib().og_use_graphs(self._h, 1 if on else 0))
+
+    def set_launch(self, pair_blocks=0, sasa_blocks=0, sasa_passes=0):
+        """Launch widths (0 keeps the current one) and the SASA arrangement's
+        pass form (0 by LDS fit, 1 one pass of 128 caps, 2 passes 80 + 128)."""
+        _check(lib().og_set_launch(self._h, int(pair_blocks), int(sasa_blocks), int(sasa_passes)))
 
     def autotune(self, mode_a=False, include_sasa=False):
         ch = np.zeros(2, np.int32)

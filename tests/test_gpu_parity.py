@@ -99,8 +99,9 @@ def test_sasa_forces_c2_generic_geometry(E, c2_system):
 
 def dense_packing(n, seed, density=0.12, min_sep=1.1):
     """n points at a dense protein-like packing (atoms per A^3) in a ball:
-    interior atoms overlap 80-100 neighbours, more caps than the main
-    arrangement pass holds (80), so they go through the second (MAXC) pass."""
+    interior atoms overlap 80-100 neighbours, more caps than the two-pass
+    arrangement's first pass holds (80), so they go through its second (MAXC)
+    pass."""
     rng = np.random.default_rng(seed)
     R = (3 * n / (4 * math.pi * density)) ** (1 / 3)
     pts = []
@@ -123,18 +124,23 @@ def test_sasa_dense_packing_second_pass(E, c1_system):
     assert (over > 80).sum() >= 10
     orc = O.Oracle(s)
     fs, stats = orc.sasa_forces(pos, 0.25)
-    eng = E.Engine([s])
-    f0 = eng.total_force(pos)
-    f1 = eng.total_force(pos, include_sasa=True, sasa_gamma_scale=0.25)
-    err = np.abs((f1 - f0) - fs).max()
-    print("dense SASA max abs diff %.3e (max |F| %.3e), %d atoms over 80 neighbours, stats %s"
-          % (err, np.abs(fs).max(), (over > 80).sum(), stats))
-    assert stats["atoms_with_arcs"] > 20
-    assert err < 1e-8
-    e_gpu = eng.total_energy(pos, include_sasa=True)[0]["sasa"]
     total, per_atom = orc.sasa_area(pos)
     e_ref = float(np.sum(s.sasa_gamma * per_atom))
-    assert abs(e_gpu - e_ref) < 1e-9 * max(1.0, abs(e_ref))
+    eng = E.Engine([s])
+    f0 = eng.total_force(pos)
+    got = {}
+    for passes in (2, 1):       # 80 caps + second pass over the rest; one pass of 128
+        eng.set_launch(sasa_passes=passes)
+        f1 = eng.total_force(pos, include_sasa=True, sasa_gamma_scale=0.25)
+        err = np.abs((f1 - f0) - fs).max()
+        print("dense SASA, %d pass(es): max abs diff %.3e (max |F| %.3e), %d atoms over 80 "
+              "neighbours, stats %s" % (passes, err, np.abs(fs).max(), (over > 80).sum(), stats))
+        assert stats["atoms_with_arcs"] > 20
+        assert err < 1e-8
+        e_gpu = eng.total_energy(pos, include_sasa=True)[0]["sasa"]
+        assert abs(e_gpu - e_ref) < 1e-9 * max(1.0, abs(e_ref))
+        got[passes] = (f1, e_gpu)
+    assert np.array_equal(got[1][0], got[2][0]) and got[1][1] == got[2][1]
 
 
 def test_sasa_total_force_c2_built_geometry(E, c2_system):
