@@ -8,8 +8,11 @@
 // its j data and its running j-side reaction (Newton's third law) to the
 // neighbouring lane with one DPP wave rotate (v_mov_dpp wave_rol:1), so in
 // 64 steps every lane meets every j atom and the j-side sums arrive back
-// home.  A diagonal tile (I = J) runs steps 1..32 (step 32 on lanes 0..31)
-// and swaps the halves of the wave at the end.  Per tile pair every lane
+// home.  On off-diagonal tiles without exclusions (most of the work) the j
+// data are read from the wave's LDS instead, entry (l + k) mod 64 at step k,
+// which is exactly what the rotation would deliver, so only the three (one)
+// j-side sums rotate.  A diagonal tile (I = J) runs steps 1..32 (step 32 on
+// lanes 0..31) and swaps the halves of the wave at the end.  Per tile pair every lane
 // flushes one FP32 partial per component into the int64 fixed-point
 // accumulators.  The FP64 energy kernel keeps v_readlane broadcasts.
 //
@@ -71,7 +74,7 @@ __device__ __forceinline__ float swap32(float v) { return __shfl_xor(v, 32, 64);
 // Descreening integral term h(r, rho~_i, S_j) of energy/src/gb.rs:347-369,
 // written with one reciprocal and one log:  1/L - 1/U = (U-L)/(LU),
 // 1/U^2 - 1/L^2 = -(1/L - 1/U)(U+L)/(LU),  ln(L/U) = ln(L^2/(LU)).
-__device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, float S) {
+__device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, float S, float S2) {
     const float U = r + S;
     const float rmS = r - S;
     const bool overlap = rmS < rt;
@@ -80,7 +83,7 @@ __device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, floa
     const float UpL = overlap ? U + L : 2.0f * r;
     const float w = __builtin_amdgcn_rcpf(L * U);
     const float t = UmL * w;
-    const float s2q = S * S * (0.25f * inv_r);
+    const float s2q = S2 * (0.25f * inv_r);
     const float lg = __builtin_amdgcn_logf(L * L * w);   // log2
     float h = fmaf(0.5f, t, -(t * UpL * w) * s2q);
     h = fmaf(lg, kHalfLn2 * inv_r, h);
@@ -102,8 +105,8 @@ __device__ __forceinline__ void descreen_tile(const og_pair_args& a, const LaneD
         const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
         const float inv_r = __builtin_amdgcn_rsqf(r2);
         const float r = r2 * inv_r;
-        float hij = descreen_h(r, inv_r, di.gb.x, Sj);
-        float hji = descreen_h(r, inv_r, rtj, di.gb.y);
+        float hij = descreen_h(r, inv_r, di.gb.x, Sj, Sj * Sj);
+        float hji = descreen_h(r, inv_r, rtj, di.gb.y, di.gb.y * di.gb.y);
         bool ok = r2 >= 1e-18f;
         if (MODEA) ok = ok && (r2 <= a.rc2_ds);
         if (DIAG) ok = ok && (k < 32 || lane < 32);
@@ -118,8 +121,52 @@ __device__ __forceinline__ void descreen_tile(const og_pair_args& a, const LaneD
     acc_j += DIAG ? swap32(gj) : rot(gj);
 }
 
+// Orders a wave's own LDS accesses across lanes for the compiler; a wave's
+// LDS operations are performed in program order, so no wait is needed.
+__device__ __forceinline__ void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Off-diagonal descreening tile with the j data in LDS: P[e] = (x, y, z, rho~)
+// and Q[e] = (S, S^2, -, -) of j atom e, entries 0..63 plus entry 64 = entry
+// 0, so that the pair of steps k, k + 1 reads entries idx and idx + 1 without
+// a wrap (Q follows P in the same array: one address for all four reads).
+// Same arithmetic, in the same order, as the rotating form.
+template <bool MODEA>
+__device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const LaneData& di,
+                                                  const float4* P, const float4* Q, int lane,
+                                                  float& acc_i, float& acc_j) {
+    float gj = 0.0f;
+    const float Si2 = di.gb.y * di.gb.y;
+    auto step = [&](float4 pj, float4 qj) {
+        const float dx = pj.x - di.x, dy = pj.y - di.y, dz = pj.z - di.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float r = r2 * inv_r;
+        const float hij = descreen_h(r, inv_r, di.gb.x, qj.x, qj.y);
+        const float hji = descreen_h(r, inv_r, pj.w, di.gb.y, Si2);
+        bool ok = r2 >= 1e-18f;
+        if (MODEA) ok = ok && (r2 <= a.rc2_ds);
+        acc_i += ok ? hij : 0.0f;
+        gj += ok ? hji : 0.0f;
+        gj = rot(gj);                  // after step 63 this is the way home
+    };
+    for (int k = 0; k < 64; k += 2) {
+        const int idx = (lane + k) & 63;
+        const float4 p0 = P[idx], p1 = P[idx + 1];
+        const float4 q0 = Q[idx], q1 = Q[idx + 1];
+        step(p0, q0);
+        step(p1, q1);
+    }
+    acc_j += gj;
+}
+
 template <bool MODEA>
 __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
+    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
+    float4* P = s_pq[threadIdx.x >> 6];
+    float4* Q = P + (OG_TILE + 1);
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -136,7 +183,14 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
             descreen_tile<MODEA, true>(a, di, di, lane, acc_i, acc_j);
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
-            descreen_tile<MODEA, false>(a, di, dj, lane, acc_i, acc_j);
+            const float4 pj = make_float4(dj.x, dj.y, dj.z, dj.gb.x);
+            const float4 qj = make_float4(dj.gb.y, dj.gb.y * dj.gb.y, 0.0f, 0.0f);
+            wave_lds_order();                       // the previous tile's reads are done
+            P[lane] = pj;
+            Q[lane] = qj;
+            if (lane == 0) { P[OG_TILE] = pj; Q[OG_TILE] = qj; }
+            wave_lds_order();
+            descreen_tile_lds<MODEA>(a, di, P, Q, lane, acc_i, acc_j);
         }
         if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix((double)acc_i));
         if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix((double)acc_j));
@@ -168,6 +222,7 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
     float Rj = dj.gb.z, iRj = dj.gb.w;
     int jid = lane;                         // j index within tile J held by this lane
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const float qki = kGbK * di.q;
     const int k0 = DIAG ? 1 : 0, k1 = DIAG ? 32 : 63;
     auto advance = [&]() {
         xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj);
@@ -207,20 +262,19 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
         const float e = __builtin_amdgcn_exp2f(-r2 * (di.gb.w * iRj));
         const float f2 = fmaf(Rp, e, r2);
         const float inv_f = __builtin_amdgcn_rsqf(f2);
-        const float qgb = kGbK * qq;
+        const float qgb = qki * qj;
         float gbc = qgb * fmaf(-0.25f, e, 1.0f) * (inv_f * inv_f * inv_f);
         if (MODEA) gbc = (r2 <= a.rc2_gb) ? gbc : 0.0f;
         float c = cnb + gbc;
         bool ok = r2 >= 1e-18f;
         if (DIAG) ok = ok && (k < 32 || lane < 32);
         c = ok ? c : 0.0f;
-        const float fx = c * dx, fy = c * dy, fz = c * dz;
-        acc.fx += fx;
-        acc.fy += fy;
-        acc.fz += fz;
-        gx -= fx;
-        gy -= fy;
-        gz -= fz;
+        acc.fx = fmaf(c, dx, acc.fx);
+        acc.fy = fmaf(c, dy, acc.fy);
+        acc.fz = fmaf(c, dz, acc.fz);
+        gx = fmaf(-c, dx, gx);
+        gy = fmaf(-c, dy, gy);
+        gz = fmaf(-c, dz, gz);
         if (k < k1) {
             advance();
             gx = rot(gx); gy = rot(gy); gz = rot(gz);
@@ -237,8 +291,66 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
     acc.gz += gz;
 }
 
+// Off-diagonal force tile without exclusions, j data in LDS: P[e] = (x, y,
+// z, q) and Q[e] = (se, rm, R, 1/R scaled) of j atom e, entries 0..63 plus
+// entry 64 = entry 0 (steps k and k + 1 read entries idx and idx + 1; Q
+// follows P in the same array).  The arithmetic is that of force_tile's
+// unmasked path.
+template <bool MODEA>
+__device__ __forceinline__ void force_tile_lds(const og_pair_args& a, const LaneData& di,
+                                               const float4* P, const float4* Q, int lane,
+                                               ForceAcc& acc) {
+    float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+    const float qki = kGbK * di.q;
+    auto step = [&](float4 pj, float4 qj4) {
+        const float dx = pj.x - di.x, dy = pj.y - di.y, dz = pj.z - di.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float inv_r2 = inv_r * inv_r;
+        const float rmin = di.par.y + qj4.y;
+        const float eij = di.par.x * qj4.x;
+        const float t2 = rmin * rmin * inv_r2;
+        const float t6 = t2 * t2 * t2;
+        const float lj = eij * inv_r2 * (t6 - t6 * t6);
+        const float qq = di.q * pj.w;
+        float coul;
+        if (MODEA) coul = -qq * (inv_r2 * inv_r - a.inv_rc3);
+        else coul = -qq * inv_r2 * inv_r;
+        float cnb = lj + coul;
+        if (MODEA) cnb = (r2 <= a.rc2_nb) ? cnb : 0.0f;
+        const float Rp = di.gb.z * qj4.z;
+        const float e = __builtin_amdgcn_exp2f(-r2 * (di.gb.w * qj4.w));
+        const float f2 = fmaf(Rp, e, r2);
+        const float inv_f = __builtin_amdgcn_rsqf(f2);
+        const float qgb = qki * pj.w;
+        float gbc = qgb * fmaf(-0.25f, e, 1.0f) * (inv_f * inv_f * inv_f);
+        if (MODEA) gbc = (r2 <= a.rc2_gb) ? gbc : 0.0f;
+        float c = cnb + gbc;
+        c = r2 >= 1e-18f ? c : 0.0f;
+        acc.fx = fmaf(c, dx, acc.fx);
+        acc.fy = fmaf(c, dy, acc.fy);
+        acc.fz = fmaf(c, dz, acc.fz);
+        gx = rot(fmaf(-c, dx, gx));     // after step 63 this is the way home
+        gy = rot(fmaf(-c, dy, gy));
+        gz = rot(fmaf(-c, dz, gz));
+    };
+    for (int k = 0; k < 64; k += 2) {
+        const int idx = (lane + k) & 63;
+        const float4 p0 = P[idx], p1 = P[idx + 1];
+        const float4 q0 = Q[idx], q1 = Q[idx + 1];
+        step(p0, q0);
+        step(p1, q1);
+    }
+    acc.gx += gx;
+    acc.gy += gy;
+    acc.gz += gz;
+}
+
 template <bool MODEA>
 __device__ __forceinline__ void force_body(const og_pair_args& a) {
+    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
+    float4* P = s_pq[threadIdx.x >> 6];
+    float4* Q = P + (OG_TILE + 1);
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -259,10 +371,18 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
             force_tile<MODEA, true, true>(a, di, di, lane, xm, m14, acc);
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
-            if (it.mask >= 0)
+            if (it.mask >= 0) {
                 force_tile<MODEA, true, false>(a, di, dj, lane, xm, m14, acc);
-            else
-                force_tile<MODEA, false, false>(a, di, dj, lane, xm, m14, acc);
+            } else {
+                const float4 pj = make_float4(dj.x, dj.y, dj.z, dj.q);
+                const float4 qj = make_float4(dj.par.x, dj.par.y, dj.gb.z, dj.gb.w);
+                wave_lds_order();                   // the previous tile's reads are done
+                P[lane] = pj;
+                Q[lane] = qj;
+                if (lane == 0) { P[OG_TILE] = pj; Q[OG_TILE] = qj; }
+                wave_lds_order();
+                force_tile_lds<MODEA>(a, di, P, Q, lane, acc);
+            }
         }
         if (lane < ci) {
             long long* f = a.facc + 3 * (size_t)(a0i + lane);
