@@ -136,9 +136,12 @@ int og_use_graphs(og_engine* e, int32_t on);
 /* Autotuner (no counterpart in origami): times the candidate launch widths
  * of the all-pairs kernels (2..32 blocks of 256 threads per CU) and of the
  * SASA arrangement kernel (2..8 blocks of 2 waves per CU) at the current
- * positions and keeps the fastest.  choice[2] (optional) receives the pair
- * and SASA block counts, best_ms[2] their times.  Results do not change:
- * accumulation is order-independent fixed point. */
+ * positions and keeps the fastest.  With OG_SASA in flags the widths are
+ * timed on the whole force evaluation (SASA on its own stream beside the
+ * pair kernels), with OG_SURROGATE (surrogate enabled) on the evaluation
+ * with learned radii.  choice[2] (optional) receives the pair and SASA block
+ * counts, best_ms[2] the pair kernel's time alone and the best evaluation's.
+ * Results do not change: accumulation is order-independent fixed point. */
 int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms);
 
 /* Launch widths without timing (no counterpart in origami), e.g. to restore

@@ -909,6 +909,8 @@ extern "C" int og_set_launch(og_engine* e, int32_t pair_blocks, int32_t sasa_blo
 
 extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if ((flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
+        return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     int dev = 0, n_cu = 256;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
@@ -946,9 +948,10 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
         og_sasa_set_passes(&e->sasa, 0);
         // 6 blocks of 2 waves per CU is the arrangement kernel's residency
         // (LDS and VGPRs); fewer leave room for the pair kernels beside it
-        const uint32_t fl = (flags & OG_MODE_A) | OG_SASA;
+        const uint32_t fl = (flags & (OG_MODE_A | OG_SURROGATE)) | OG_SASA;
         int lrc = OG_OK;
-        auto evaluation = [&] { if (!lrc) lrc = launch_forces(e, fl, 1.0, false); };
+        // with the surrogate: its evaluations without the plain-GB check
+        auto evaluation = [&] { if (!lrc) lrc = launch_forces(e, fl, 1.0, false, 0); };
         best_s = 1e30f;
         for (int per_cu : {2, 3, 4, 5, 6, 8}) {
             const int g = std::min(per_cu * n_cu, og_sasa_max_blocks(&e->sasa));

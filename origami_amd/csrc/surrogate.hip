@@ -102,9 +102,12 @@ struct DescArgs {
 // does not depend on the order).  The bin floor(r / 0.375 Å) is estimated
 // from a single-precision square root and then settled exactly on r^2
 // against the bin edges (0.375 k)^2 = 0.140625 k^2, which are exact in FP64.
+// Counts are 16-bit, two bins to a word (25 KB of LDS per block, so that the
+// kernel fits beside the SASA arrangement on a CU).
 #define SUR_DESC_WAVES 4
+#define SUR_HW ((OG_SUR_F + 1) / 2)                     // words per histogram row (81)
 __global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a) {
-    __shared__ unsigned hist[64 * OG_SUR_F];          // stride 161: conflict-free rows
+    __shared__ unsigned hist[64 * SUR_HW];            // odd stride: conflict-free rows
     __shared__ int near_list[1024];
     __shared__ int n_near;
     const int t = blockIdx.x;
@@ -112,7 +115,7 @@ __global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a)
     const int lane = tid & 63, wv = tid >> 6;
     const int sys = a.tile_sys[t];
     if (a.mode && !a.mode[sys]) return;
-    for (int q = tid; q < 64 * OG_SUR_F; q += blockDim.x) hist[q] = 0u;
+    for (int q = tid; q < 64 * SUR_HW; q += blockDim.x) hist[q] = 0u;
     if (tid == 0) n_near = 0;
     __syncthreads();
     const double4 bi = a.tile_bound[t];
@@ -133,7 +136,7 @@ __global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a)
     const int i = a0 + lane;
     double xi = 0.0, yi = 0.0, zi = 0.0;
     if (vi) { xi = a.pos[3 * (size_t)i]; yi = a.pos[3 * (size_t)i + 1]; zi = a.pos[3 * (size_t)i + 2]; }
-    unsigned* row = hist + lane * OG_SUR_F;
+    unsigned* row = hist + lane * SUR_HW;
     for (int q = wv; q < nn; q += SUR_DESC_WAVES) {
         const int J = near_list[q];
         const int b0 = a.tile_atom0[J], cj = a.tile_cnt[J];
@@ -153,7 +156,8 @@ __global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a)
                 k = k > 31 ? 31 : (k < 0 ? 0 : k);
                 while (k < 31 && 0.140625 * (double)((k + 1) * (k + 1)) <= r2) ++k;
                 while (k > 0 && 0.140625 * (double)(k * k) > r2) --k;
-                atomicAdd(row + c * 32 + k, 1u);
+                const int b = c * 32 + k;
+                atomicAdd(row + (b >> 1), 1u << ((b & 1) * 16));
             }
         }
     }
@@ -162,7 +166,7 @@ __global__ void __launch_bounds__(64 * SUR_DESC_WAVES) og_k_sur_desc(DescArgs a)
     for (int q = tid; q < cnt * OG_SUR_K1; q += blockDim.x) {
         const int rr = q / OG_SUR_K1, f = q % OG_SUR_K1;
         unsigned short v = 0;
-        if (f < 160) v = f2bf((float)hist[rr * OG_SUR_F + f] * 0.125f);
+        if (f < 160) v = f2bf((float)((hist[rr * SUR_HW + (f >> 1)] >> ((f & 1) * 16)) & 0xFFFFu) * 0.125f);
         else if (f == 160) v = f2bf((float)((a.rho[a0 + rr] - 1.5) * 4.0));
         a.X[(size_t)(a0 + rr) * OG_SUR_K1 + f] = v;
     }
@@ -242,12 +246,14 @@ __device__ __forceinline__ void mlp_forward(const unsigned short* X, int row0, c
 // fragment load (the weights are read once per 64 rows instead of once per
 // 16).  Each row goes through the 16-row forward's exact operations, so the
 // radii are unchanged.
-__global__ void __launch_bounds__(256) og_k_sur_infer64(const unsigned short* X, int n_valid, int n_rows,
+// Two waves per block (35 KB of LDS, so that the kernel fits beside the SASA
+// arrangement on a CU).
+__global__ void __launch_bounds__(128) og_k_sur_infer64(const unsigned short* X, int n_valid, int n_rows,
                                                         MlpParams P, double* r_out) {
-    __shared__ __attribute__((aligned(16))) unsigned short hs[4][64][136];
+    __shared__ __attribute__((aligned(16))) unsigned short hs[2][64][136];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int r = lane & 15, g = lane >> 4;
-    const int row0 = (blockIdx.x * 4 + w) * 64;
+    const int row0 = (blockIdx.x * 2 + w) * 64;
     if (row0 >= n_valid) return;
     unsigned short (*h)[136] = hs[w];
     og_bf16x8 a1[4][6];
@@ -661,7 +667,7 @@ int og_sur_descriptors(og_sur* s, bool all) {
 int og_sur_infer(og_sur* s) {
     if (s->n_atoms == 0) return 0;
     const int waves64 = (s->n_atoms + 63) / 64;
-    hipLaunchKernelGGL(og_k_sur_infer64, dim3((waves64 + 3) / 4), dim3(256), 0, s->stream, s->X, s->n_atoms,
+    hipLaunchKernelGGL(og_k_sur_infer64, dim3((waves64 + 1) / 2), dim3(128), 0, s->stream, s->X, s->n_atoms,
                        s->n_pad, params_of(s), s->r_sur);
     return hipGetLastError() == hipSuccess ? 0 : fail("kernel launch failed");
 }

@@ -290,10 +290,11 @@ class Engine:
         pass form (0 by LDS fit, 1 one pass of 128 caps, 2 passes 80 + 128)."""
         _check(lib().og_set_launch(self._h, int(pair_blocks), int(sasa_blocks), int(sasa_passes)))
 
-    def autotune(self, mode_a=False, include_sasa=False):
+    def autotune(self, mode_a=False, include_sasa=False, surrogate=False):
         ch = np.zeros(2, np.int32)
         ms = np.zeros(2)
-        _check(lib().og_autotune(self._h, (MODE_A if mode_a else 0) | (SASA if include_sasa else 0),
+        _check(lib().og_autotune(self._h, (MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
+                                 | (SURROGATE if surrogate else 0),
                                  _i(ch), _d(ms)))
         return {"pair_blocks": int(ch[0]), "sasa_blocks": int(ch[1]), "pair_ms": float(ms[0]),
                 "sasa_ms": float(ms[1])}
