@@ -1117,45 +1117,58 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
                 cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
                 tp = arc_theta_primal(pi, pk, pl, pm, ri, rk, rl, rm, pr.y, pr.z);
             }
+            // Roles k, l, m by their tangents; role i as minus their sum: the
+            // arc depends on the positions only through differences from
+            // atom i's (translation invariance).
             double* out = d.arc_terms + 12ll * A;
-            for (int role = 0; role < 4; ++role) {
-                const int xa = role == 0 ? at.x : role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
+            double ti[3] = {0.0, 0.0, 0.0};
+            for (int role = 1; role < 4; ++role) {
+                const int xa = role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
                 for (int axis = 0; axis < 3; ++axis) {
                     double term = 0.0;
                     if (xa >= 0) {
                         const v3 b = unit(axis);
-                        const v3 dri = xa == at.x ? b : zero;
                         const v3 drk = xa == at.y ? b : zero;
-                        const double dcos = cos_alpha_tangent(cp, dri, drk);
+                        const double dcos = cos_alpha_tangent(cp, zero, drk);
                         if (!has_v) {
                             term = pr.x * dcos;
                         } else {
                             const v3 drl = xa == at.z ? b : zero;
                             const v3 drm = xa == at.w ? b : zero;
-                            const double dth = tp.ok ? arc_theta_tangent(tp, dri, drk, drl, drm) : 0.0;
+                            const double dth = tp.ok ? arc_theta_tangent(tp, zero, drk, drl, drm) : 0.0;
                             term = pr.x * dcos + cak * dth;
                         }
                     }
                     out[3 * role + axis] = term;
+                    ti[axis] -= term;
                 }
             }
+            out[0] = ti[0];
+            out[1] = ti[1];
+            out[2] = ti[2];
         } else {
             const int V = it - n_a;
             const int4 at = d.vtx_atoms[V];
             const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y), pl = ldp(pos, at.z);
             const VertexEpsP ep = vertex_eps_primal(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z],
                                                     d.vtx_sign[V]);
+            // roles k, l by their tangents, role i as minus their sum
             double* out = d.vtx_terms + 9ll * V;
-            for (int role = 0; role < 3; ++role) {
-                const int xa = role == 0 ? at.x : role == 1 ? at.y : at.z;
+            double ti[3] = {0.0, 0.0, 0.0};
+            for (int role = 1; role < 3; ++role) {
+                const int xa = role == 1 ? at.y : at.z;
                 for (int axis = 0; axis < 3; ++axis) {
                     const v3 b = unit(axis);
-                    const v3 dri = xa == at.x ? b : zero;
                     const v3 drk = xa == at.y ? b : zero;
                     const v3 drl = xa == at.z ? b : zero;
-                    out[3 * role + axis] = ep.ok ? vertex_eps_tangent(ep, dri, drk, drl) : 0.0;
+                    const double term = ep.ok ? vertex_eps_tangent(ep, zero, drk, drl) : 0.0;
+                    out[3 * role + axis] = term;
+                    ti[axis] -= term;
                 }
             }
+            out[0] = ti[0];
+            out[1] = ti[1];
+            out[2] = ti[2];
         }
     }
 }

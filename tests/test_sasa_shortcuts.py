@@ -305,3 +305,33 @@ def test_jvp_split_is_bit_identical(host):
     assert np.count_nonzero(af) > n and np.count_nonzero(vf) > n
     assert np.array_equal(af.view(np.int64), asp.view(np.int64))
     assert np.array_equal(vf.view(np.int64), vsp.view(np.int64))
+
+
+def test_jvp_translation_invariance(host):
+    # the terms kernel forms role i's tangent (arc: i, k, l, m; vertex: i, k,
+    # l) as minus the sum of the other roles' tangents, since the arc angle
+    # and the vertex angle depend on the positions only through differences;
+    # pinned on the full JVPs, atom i away from the origin
+    host.host_jvp_split.argtypes = [C.c_int] + [C.POINTER(C.c_double)] * 5
+    rng = np.random.default_rng(9)
+    n = 20000
+    pi = rng.uniform(-40.0, 40.0, (n, 3))
+    dirs = [_unit(rng, n) for _ in range(3)]
+    rad = rng.uniform(2.6, 3.2, (n, 4))
+    dist = rng.uniform(1.0, 5.5, (n, 3))
+    pts = [pi] + [pi + dirs[q] * dist[:, q:q + 1] for q in range(3)]
+    rows = np.ascontiguousarray(np.concatenate(pts + [rad, rng.choice([-1.0, 1.0], (n, 2))], axis=1))
+    af, asp = np.zeros((n, 12)), np.zeros((n, 12))
+    vf, vsp = np.zeros((n, 9)), np.zeros((n, 9))
+    host.host_jvp_split(n, _ptr(rows, C.c_double), _ptr(af, C.c_double), _ptr(asp, C.c_double),
+                        _ptr(vf, C.c_double), _ptr(vsp, C.c_double))
+    for t, nr in ((af, 4), (vf, 3)):
+        t = t.reshape(n, nr, 3)
+        live = np.abs(t).max(axis=(1, 2)) > 0
+        assert live.sum() > 2000
+        scale = np.abs(t).max(axis=1)
+        resid = np.abs(t[:, 0] + t[:, 1:].sum(axis=1))
+        assert np.all(resid <= 1e-10 * scale + 1e-14), resid.max()
+        # the negated sum, as the kernel forms it, against the direct tangent
+        direct, formed = t[:, 0], -t[:, 1:].sum(axis=1)
+        assert np.allclose(formed, direct, rtol=1e-9, atol=1e-13)
