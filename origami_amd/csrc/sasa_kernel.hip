@@ -1185,6 +1185,16 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
     const double gam = d.gamma[ia] * gamma_scale;
     const double ri = d.radius[ia];
     const double r_sq = ri * ri;
+    // The role codes of the first 64 arcs and vertices, one per lane, read
+    // together up front (later ones are read where they are used).
+    const unsigned acode = lane < na ? d.arc_code[rec.x + lane] : 0u;
+    const unsigned vcode = lane < nv ? d.vtx_code[rec.y + lane] : 0u;
+    auto arc_code = [&](int A) -> unsigned {
+        return A < 64 ? (unsigned)__builtin_amdgcn_readlane((int)acode, A) : d.arc_code[rec.x + A];
+    };
+    auto vtx_code = [&](int V) -> unsigned {
+        return V < 64 ? (unsigned)__builtin_amdgcn_readlane((int)vcode, V) : d.vtx_code[rec.y + V];
+    };
     for (int x = lane; x <= nc; x += 64) {
         double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
         bool any = false;
@@ -1199,7 +1209,7 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
                 t[q][0] = t[q][1] = t[q][2] = 0.0;
                 if (A0 + q < na) {
                     const int A = rec.x + A0 + q;
-                    const unsigned xr = d.arc_code[A];
+                    const unsigned xr = arc_code(A0 + q);
                     int role = -1;
                     if (x == 0) role = 0;
                     else if (x == (int)(xr & 0xFF)) role = 1;
@@ -1229,7 +1239,7 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
                 t[q][0] = t[q][1] = t[q][2] = 0.0;
                 if (V0 + q < nv) {
                     const int V = rec.y + V0 + q;
-                    const unsigned xr = d.vtx_code[V];
+                    const unsigned xr = vtx_code(V0 + q);
                     int role = -1;
                     if (x == 0) role = 0;
                     else if (x == (int)(xr & 0xFF)) role = 1;
