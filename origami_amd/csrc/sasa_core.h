@@ -116,18 +116,21 @@ OG_HD int intersect_count_fast(const Cap& c1, const Cap& c2) {
 // both ends lie within 2e-6 rad of the exact arc or inside it.
 #define SA_COV_DELTA 1e-6
 #define SA_COV_FULL 100.0f
+// The containment value on the circle is g + SA_COV_DELTA + amp cos(phi -
+// phi_m) with g = B - SA_COV_DELTA; the whole-circle and no-arc cases are
+// settled on amp^2 (no square root), the arc itself from amp.
 OG_HD inline void cover_arc(const Cap& ck, const Cap& cm, v3 u, v3 v, float& a, float& w2) {
-    const double sk = sqrt(fmax(1.0 - ck.ca * ck.ca, 0.0));
     const double ct = dot(ck.axis, cm.axis);
     const double x = dot(cm.axis, u), y = dot(cm.axis, v);
-    const double amp = sk * sqrt(x * x + y * y);
-    const double B = ck.ca * ct - cm.ca;
+    const double amp2 = fmax(1.0 - ck.ca * ck.ca, 0.0) * (x * x + y * y);
+    const double g = (ck.ca * ct - cm.ca) - SA_COV_DELTA;
     a = 0.0f;
     w2 = -1.0f;
-    if (B - amp >= SA_COV_DELTA) {
-        w2 = SA_COV_FULL;
-    } else if (amp >= 1e-3) {
-        const double c = (SA_COV_DELTA - B) / amp;
+    if (g >= 0.0 && g * g >= amp2) {
+        w2 = SA_COV_FULL;                              // g >= amp
+    } else if (amp2 >= 1e-6 && !(g <= 0.0 && g * g >= 0.998001 * amp2)) {
+        // amp >= 1e-3, and not -g >= (1 - 1e-3) amp (no arc)
+        const double c = -g / sqrt(amp2);
         if (c < 1.0 - 1e-3) {
             const float w = c <= -1.0 + 1e-3 ? (float)(3.14159265358979323846 - 0.05) : acosf((float)c);
             a = atan2f((float)y, (float)x) - w;
