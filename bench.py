@@ -10,8 +10,10 @@ LJ + Coulomb + GB (descreening + pair force) plus analytical SASA forces
 chains are built extended and relaxed by a short batched L-BFGS before the
 clock starts (untimed), so that dt = 2 fs is stable.
 
-Prints ONE JSON line.  The timed run replays one captured hipGraph per step,
-SASA on its own stream beside the all-pairs chain.  Kernel timing for the
+Prints ONE JSON line.  The timed run steps with SASA on its own stream beside
+the all-pairs chain; og_run replays captured hipGraphs for batches under
+20 000 atoms (the REMD batch) and launches directly above (the 64-chain
+batch, where direct launches measured 2-3 % faster).  Kernel timing for the
 roofline figure comes from a separate run with HIP events around every
 launch on one stream (kernels one at a time).  The CPU baseline times the
 FP64 oracle on a bounded sample of the same workload, on the host cores this
@@ -308,7 +310,7 @@ def main():
     tune = eng.autotune(include_sasa=include_sasa)
     log("autotune:", tune)
     eng.run(args.warmup)
-    # timed run: captured-graph replay, no per-kernel events
+    # timed run: no per-kernel events (captured graphs below 20 000 atoms)
     t0 = time.perf_counter()
     eng.run(args.steps)
     wall = time.perf_counter() - t0
@@ -386,8 +388,8 @@ def main():
         "pair_interactions_per_s": pair_ips,
         "kernel_ms_per_step": {k: v[0] / max(args.profile_steps, 1) for k, v in prof.items() if v[1] > 0},
         "kernel_timing": "HIP events on the engine stream over a separate %d-step run with direct "
-                         "launches, one kernel at a time (the timed run replays captured graphs "
-                         "with SASA on a second stream beside the pair kernels)" % args.profile_steps,
+                         "launches, one kernel at a time (the timed run has SASA on a second "
+                         "stream beside the pair kernels)" % args.profile_steps,
         "autotune": tune,
         "pair_kernels": pk,
         "roofline": roof,

@@ -129,8 +129,10 @@ int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 
 /* og_run replays one captured hipGraph per BAOAB step (all systems, every
  * launch of the step) when profiling is off and no per-step kinetic energy
- * is requested; on = 0 forces direct launches.  No counterpart in origami
- * (its step loop is langevin.rs:195-367, run per replica by remd.rs:286). */
+ * is requested.  on = 1: always; 0: direct launches; 2 (the default): below
+ * 20 000 atoms, where launch overhead matters (on larger batches direct
+ * launches measured 2-3 % faster).  No counterpart in origami (its step
+ * loop is langevin.rs:195-367, run per replica by remd.rs:286). */
 int og_use_graphs(og_engine* e, int32_t on);
 
 /* Autotuner (no counterpart in origami): times the candidate launch widths

@@ -283,7 +283,9 @@ class Engine:
         return ke
 
     def use_graphs(self, on=True):
-        _check(lib().og_use_graphs(self._h, 1 if on else 0))
+        """True: captured steps; False: direct launches; None: by batch size
+        (the default: captured below 20 000 atoms)."""
+        _check(lib().og_use_graphs(self._h, 2 if on is None else (1 if on else 0)))
 
     def set_launch(self, pair_blocks=0, sasa_blocks=0, sasa_passes=0):
         """Launch widths (0 keeps the current one) and the SASA arrangement's

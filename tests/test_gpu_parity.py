@@ -309,6 +309,7 @@ def test_graph_replay_matches_eager(E, c1_system, c3_system):
         return eng
     eager, graph = make(), make()
     eager.use_graphs(False)
+    graph.use_graphs(True)
     for rep in range(4):
         eager.run(8)
         graph.run(8)
@@ -354,6 +355,7 @@ def test_sasa_side_stream_matches_serial(E, c1_system, c2_system):
         eng.set_velocities(v)
         eng.dynamics_setup([300.0, 330.0], dt_fs=2.0, friction_ps_inv=2.0, seed=5,
                            constrain_h_bonds=True, include_sasa=True, sasa_gamma_scale=0.25)
+    ovl.use_graphs(True)
     for rep in range(3):
         ovl.run(4)                      # captured steps, two streams
         ser.run(4)                      # profiled: direct launches, one stream
