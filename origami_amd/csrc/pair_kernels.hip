@@ -73,14 +73,13 @@ __device__ __forceinline__ float swap32(float v) { return __shfl_xor(v, 32, 64);
 
 // Descreening integral term h(r, rho~_i, S_j) of energy/src/gb.rs:347-369,
 // written with one reciprocal and one log:  1/L - 1/U = (U-L)/(LU),
-// 1/U^2 - 1/L^2 = -(1/L - 1/U)(U+L)/(LU),  ln(L/U) = ln(L^2/(LU)).
+// 1/U^2 - 1/L^2 = -(1/L - 1/U)(U+L)/(LU),  ln(L/U) = ln(L^2/(LU)), with
+// L = max(r - S, rho~) covering both the overlapping and the separate case.
 __device__ __forceinline__ float descreen_h(float r, float inv_r, float rt, float S, float S2) {
     const float U = r + S;
-    const float rmS = r - S;
-    const bool overlap = rmS < rt;
-    const float L = overlap ? rt : rmS;
-    const float UmL = overlap ? U - L : 2.0f * S;
-    const float UpL = overlap ? U + L : 2.0f * r;
+    const float L = fmaxf(r - S, rt);
+    const float UmL = U - L;
+    const float UpL = U + L;
     const float w = __builtin_amdgcn_rcpf(L * U);
     const float t = UmL * w;
     const float s2q = S2 * (0.25f * inv_r);
