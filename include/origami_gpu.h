@@ -99,7 +99,9 @@ int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
 
 /* Replaces total_energy_with_options (dynamics/src/energy_eval.rs:20-34) and
  * remd::potential_energy (dynamics/src/remd.rs:529-539): per-system energy
- * terms; GB pair energy is uncut in both modes, as in gb_energy. */
+ * terms; GB pair energy is uncut in both modes, as in gb_energy.  Computes
+ * no forces: the engine's force array (the forces og_run's next step starts
+ * from) is left as the last og_forces / og_run step made it. */
 int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies);
 
 /* Effective Born radii and descreening integrals after the last force call

@@ -577,8 +577,12 @@ static void invalidate_graphs(og_engine* e) {
 }
 
 // sur_check: -1 = decide from the call counter; 0 / 1 = forced (graph capture)
-static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1) {
+// forces = false (energies only): no pair-force kernel and no SASA terms or
+// sums; the bonded kernel still adds its forces into the accumulator.
+static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1,
+                         bool forces = true) {
     const bool mode_a = flags & OG_MODE_A;
+    if (!forces) sasa_scale = 0.0;
     // SASA on the side stream, overlapping everything up to the join below.
     // Both branches add into the force and energy accumulators with integer
     // atomics only, so the sums do not depend on the interleaving.  The
@@ -638,7 +642,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         if (sur && check && og_sur_check_apply(sur))
             return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
     }
-    {
+    if (forces) {
         ProfScope p(e, OG_K_FORCE);
         og_launch_pair_force(a, mode_a, e->stream);
     }
@@ -679,8 +683,10 @@ static int read_energies(og_engine* e, double* out) {
     return OG_OK;
 }
 
-extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
-                         double sasa_gamma_scale, double* energies) {
+// forces = false: energies only; the engine's force array keeps the forces of
+// the last force evaluation (the next og_run step starts from them).
+static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+                    double sasa_gamma_scale, double* energies, bool forces) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     if ((flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
         return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
@@ -690,8 +696,8 @@ extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_
     if (energy) HIPCHK(hipMemsetAsync(e->eacc, 0, (size_t)e->n_sys * OG_NE * sizeof(long long), e->stream));
     HIPCHK(hipMemsetAsync(e->clamped, 0, (size_t)e->n_sys * sizeof(int), e->stream));
     og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
-    if ((rc = launch_forces(e, flags, sasa_gamma_scale, energy))) return rc;
-    {
+    if ((rc = launch_forces(e, flags, sasa_gamma_scale, energy, -1, forces))) return rc;
+    if (forces) {
         ProfScope p(e, OG_K_BACK);
         og_back_args b{};
         b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
@@ -716,9 +722,14 @@ extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_
     return OG_OK;
 }
 
+extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+                         double sasa_gamma_scale, double* energies) {
+    return evaluate(e, xyz, f_out, flags, sasa_gamma_scale, energies, true);
+}
+
 extern "C" int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies) {
     if (!energies) return fail(OG_ERR_ARG, "energies is NULL");
-    return og_forces(e, xyz, nullptr, flags | OG_ENERGY, 1.0, energies);
+    return evaluate(e, xyz, nullptr, flags | OG_ENERGY, 0.0, energies, false);
 }
 
 extern "C" int og_born_radii(og_engine* e, double* radii, double* integral) {

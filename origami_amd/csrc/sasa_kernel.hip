@@ -1428,9 +1428,11 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
         hipLaunchKernelGGL((og_k_sasa_arrange<false, SA_NC_MAIN, false>), dim3(d->blocks), blk, 0, s, *d, pos);
         hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC, true>), dim3(blocks2), blk, 0, s, *d, pos);
     }
-    hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
-    hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
-                       0, s, *d, facc, gamma_scale);
+    if (gamma_scale != 0.0) {                 // forces (not only areas)
+        hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
+        hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
+                           0, s, *d, facc, gamma_scale);
+    }
     if (hipGetLastError() != hipSuccess) {
         g_sasa_err = "kernel launch failed";
         return -2;

@@ -363,6 +363,37 @@ def test_sasa_side_stream_matches_serial(E, c1_system, c2_system):
         assert np.array_equal(ovl.velocities(), ser.velocities()), rep
 
 
+def test_energies_leave_dynamics_forces(E, c1_system, c2_system):
+    """og_energies between og_run chunks (as REMD does for its swaps, with
+    other flags than the dynamics) must not change the forces the next step
+    starts from: trajectories bitwise equal to a run without the calls, and
+    the energies equal to those of a force-and-energy evaluation."""
+    systems = [c1_system, c2_system]
+
+    def make():
+        eng = E.Engine(systems)
+        eng.set_positions(np.concatenate([jittered(s, 60 + k) for k, s in enumerate(systems)]))
+        v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(80 + k))
+                            for k, s in enumerate(systems)])
+        eng.set_velocities(v)
+        eng.dynamics_setup([300.0, 320.0], dt_fs=2.0, friction_ps_inv=2.0, seed=3,
+                           constrain_h_bonds=True, include_sasa=True, sasa_gamma_scale=0.25)
+        return eng
+    a, b = make(), make()
+    for rep in range(3):
+        a.run(3)
+        b.run(3)
+        ea = a.total_energy(mode_a=True, include_sasa=True)
+        assert np.array_equal(a.positions(), b.positions()), rep
+        assert np.array_equal(a.velocities(), b.velocities()), rep
+    ref = E.Engine(systems)
+    _, e_ref = ref.total_force(a.positions(), mode_a=True, include_sasa=True, energies=True)
+    ea = a.total_energy(mode_a=True, include_sasa=True)
+    for x, y in zip(ea, e_ref):
+        for k in y:
+            assert x[k] == y[k], k
+
+
 def test_sasa_energy_matches_oracle(E, c1_system, c2_system):
     """PowerSasa energy sum_i gamma_i A_i (powersasa_energy, no gamma scale) on
     the GPU against the oracle's per-atom accessible areas, generic geometry,
