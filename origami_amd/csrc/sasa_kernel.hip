@@ -1120,7 +1120,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
             // Roles k, l, m by their tangents; role i as minus their sum: the
             // arc depends on the positions only through differences from
             // atom i's (translation invariance).
-            double* out = d.arc_terms + 12ll * A;
+            double out[12];
             double ti[3] = {0.0, 0.0, 0.0};
             for (int role = 1; role < 4; ++role) {
                 const int xa = role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
@@ -1146,6 +1146,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
             out[0] = ti[0];
             out[1] = ti[1];
             out[2] = ti[2];
+            double* dst = d.arc_terms + 12ll * A;      // one pass, in order
+#pragma unroll
+            for (int q = 0; q < 12; ++q) dst[q] = out[q];
         } else {
             const int V = it - n_a;
             const int4 at = d.vtx_atoms[V];
@@ -1153,7 +1156,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
             const VertexEpsP ep = vertex_eps_primal(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z],
                                                     d.vtx_sign[V]);
             // roles k, l by their tangents, role i as minus their sum
-            double* out = d.vtx_terms + 9ll * V;
+            double out[9];
             double ti[3] = {0.0, 0.0, 0.0};
             for (int role = 1; role < 3; ++role) {
                 const int xa = role == 1 ? at.y : at.z;
@@ -1169,6 +1172,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) o
             out[0] = ti[0];
             out[1] = ti[1];
             out[2] = ti[2];
+            double* dst = d.vtx_terms + 9ll * V;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) dst[q] = out[q];
         }
     }
 }
