@@ -622,8 +622,10 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
             check = sur_check > 0;
         }
         ProfScope p(e, OG_K_SURROGATE);
-        if (og_sur_descriptors(sur, false) || og_sur_infer(sur) || (check && og_sur_check_begin(sur)))
+        if (og_sur_descriptors(sur, false) || og_sur_infer(sur) || (check && og_sur_check_begin(sur))) {
+            if (fork) (void)hipStreamWaitEvent(e->stream, e->join_ev, 0);   // SASA still joins
             return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
+        }
     }
     {
         ProfScope p(e, OG_K_DESCREEN);
@@ -639,8 +641,10 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         r.eacc = e->eacc; r.n_atoms = e->n_atoms;
         if (sur) { r.sur_mode = sur->mode; r.r_sur = sur->r_sur; r.dev = sur->dev; r.check = check ? 1 : 0; }
         og_launch_radii(r, energy, e->stream);
-        if (sur && check && og_sur_check_apply(sur))
+        if (sur && check && og_sur_check_apply(sur)) {
+            if (fork) (void)hipStreamWaitEvent(e->stream, e->join_ev, 0);
             return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());
+        }
     }
     if (forces) {
         ProfScope p(e, OG_K_FORCE);
