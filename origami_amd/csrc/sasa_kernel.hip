@@ -1276,10 +1276,13 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
 }
 
 // Per-tile bounding spheres (center = first atom, radius = max distance).
+// counters != null: block 0 also clears the SASA launch's 8 counters (one
+// launch fewer at the head of the SASA chain).
 __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const int* tile_cnt,
-                                 int n_tiles, double4* out) {
+                                 int n_tiles, double4* out, int* counters) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
+    if (counters && t == 0 && lane < 8) counters[lane] = 0;
     if (t >= n_tiles) return;
     const int a0 = tile_atom0[t];
     const double cx = pos[3 * (size_t)a0], cy = pos[3 * (size_t)a0 + 1], cz = pos[3 * (size_t)a0 + 2];
@@ -1291,10 +1294,6 @@ __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const
     }
     for (int off = 32; off >= 1; off >>= 1) r2 = fmax(r2, __shfl_xor(r2, off, 64));
     if (lane == 0) out[t] = make_double4(cx, cy, cz, sqrt(r2) * (1.0 + 1e-12) + 1e-9);
-}
-
-__global__ void og_k_sasa_reset(int* counters) {
-    if (threadIdx.x < 8) counters[threadIdx.x] = 0;
 }
 
 int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys,
@@ -1416,9 +1415,8 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
     if (d->n_work == 0 || (gamma_scale == 0.0 && !area)) return 0;
     d->eacc = eacc;
     d->atom_sys = atom_sys;
-    hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles), dim3(64), 0, s, pos, d->tile_atom0,
-                       d->tile_cnt, d->n_tiles, d->tile_bound);
-    hipLaunchKernelGGL(og_k_sasa_reset, dim3(1), dim3(64), 0, s, d->counters);
+    hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles > 0 ? d->n_tiles : 1), dim3(64), 0, s, pos,
+                       d->tile_atom0, d->tile_cnt, d->n_tiles, d->tile_bound, d->counters);
     // One pass with MAXC caps when the grid fits its LDS; otherwise the
     // SA_NC_MAIN pass and the second pass over the atoms it deferred (usually
     // few or none), one block per CU.

@@ -37,7 +37,7 @@ const char* og_sur_error() { return g_sur_err.c_str(); }
 
 // defined in sasa_kernel.hip
 __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const int* tile_cnt,
-                                 int n_tiles, double4* out);
+                                 int n_tiles, double4* out, int* counters);
 
 namespace {
 
@@ -657,7 +657,7 @@ void og_sur_free(og_sur* s) {
 int og_sur_descriptors(og_sur* s, bool all) {
     if (s->n_tiles == 0) return 0;
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(s->n_tiles), dim3(64), 0, s->stream, s->pos, s->tile_atom0,
-                       s->tile_cnt, s->n_tiles, s->tile_bound);
+                       s->tile_cnt, s->n_tiles, s->tile_bound, nullptr);
     DescArgs a{s->pos, s->rho, s->cls, s->tile_atom0, s->tile_cnt, s->tile_sys, s->sys_tile_b, s->sys_tile_e,
                s->tile_bound, all ? nullptr : s->mode, s->X};
     hipLaunchKernelGGL(og_k_sur_desc, dim3(s->n_tiles), dim3(64 * SUR_DESC_WAVES), 0, s->stream, a);
