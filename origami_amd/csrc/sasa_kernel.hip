@@ -94,6 +94,11 @@ struct ArrLds {
     unsigned char ulist[SA_TMAX];                    // undecided slots
 };
 
+// the transition pass lists its pairs as 16-bit words across rm and est
+static_assert(offsetof(ArrLds<SA_NC_MAIN>, est) == offsetof(ArrLds<SA_NC_MAIN>, rm) + SA_TMAX &&
+              offsetof(ArrLds<MAXC>, est) == offsetof(ArrLds<MAXC>, rm) + SA_TMAX &&
+              offsetof(ArrLds<SA_NC_MAIN>, rm) % 2 == 0 && offsetof(ArrLds<MAXC>, rm) % 2 == 0,
+              "rm, est must be contiguous and 2-aligned");
 // the union cover test stores its arcs across tx, ty and tz
 static_assert(offsetof(ArrLds<SA_NC_MAIN>, tz) == offsetof(ArrLds<SA_NC_MAIN>, tx) + 2 * SA_TMAX * sizeof(double),
               "tx, ty, tz must be contiguous");
@@ -651,21 +656,53 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         const int nck = a1 - a0;
 
         // transitions in push order (cap, then m ascending, p before q); a cap
-        // without transitions gets one probe entry (tq = 2)
+        // without transitions gets one probe entry (tq = 2).  Segment starts
+        // come from the counts; the pairs that emit entries (the same fast
+        // count decides, exactly as for cnt) are listed first, so that the
+        // intersection points and their angles are formed on full waves.
+        if (lane == 0) {
+            int acc = 0;
+            for (int kl = 0; kl < nck; ++kl) {
+                L.seg[kl] = (unsigned short)acc;
+                const int c = L.cnt[L.act[a0 + kl]];
+                acc += c > 0 ? c : 1;
+            }
+            L.seg[nck] = (unsigned short)acc;
+        }
+        unsigned short* plist = reinterpret_cast<unsigned short*>(L.rm);   // [SA_TMAX], over rm and est
         const int npairs = nck * nper;
-        int T = 0;
+        int npl = 0;
         for (int base = 0; base < npairs; base += 64) {
             const int p = base + lane;
-            int cnt = 0, kl = 0, k = 0, m = 0, first = 0, probe = 0;
-            v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
+            bool emit = false;
+            int code = 0;
             if (p < npairs) {
-                kl = p / nper;
+                const int kl = p / nper, mi = p % nper;
+                const int k = L.act[a0 + kl];
+                if (L.cnt[k] == 0) emit = mi == 0;
+                else if (nc > 1) emit = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1)) != 0;
+                code = (kl << 7) | mi;
+            }
+            const unsigned long long em = __ballot(emit);
+            const int pos = npl + popc(em & lt);
+            if (emit && pos < SA_TMAX) plist[pos] = (unsigned short)code;
+            npl += popc(em);
+        }
+        if (npl > SA_TMAX) { overflow = true; break; }
+        wave_sync();
+        int T = 0;
+        for (int base = 0; base < npl; base += 64) {
+            const int q = base + lane;
+            int cnt = 0, kl = 0, k = 0, m = 0, probe = 0;
+            v3 P = mk(0, 0, 0), Q = mk(0, 0, 0);
+            if (q < npl) {
+                const int code = plist[q];
+                kl = code >> 7;
                 k = L.act[a0 + kl];
-                const int mi = p % nper;
-                first = mi == 0;
+                const int mi = code & 127;
                 if (L.cnt[k] == 0) {
-                    if (first) { probe = 1; cnt = 1; P = sample_circle_point(cap_at(L, k)); }
-                } else if (nc > 1) {
+                    probe = 1; cnt = 1; P = sample_circle_point(cap_at(L, k));
+                } else {
                     m = mi < k ? mi : mi + 1;
                     const int r = intersect_circles(cap_at(L, k), cap_at(L, m), P, Q);
                     cnt = r == 0 ? 2 : (r == 1 ? 1 : 0);
@@ -673,7 +710,6 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             }
             const unsigned long long b1 = __ballot(cnt >= 1), b2 = __ballot(cnt == 2);
             const int off = T + popc(b1 & lt) + popc(b2 & lt);
-            if (first && off <= SA_TMAX) L.seg[kl] = (unsigned short)off;
             if (cnt >= 1 && off < SA_TMAX) {
                 const Cap ck = cap_at(L, k);
                 v3 bu, bv;
@@ -691,14 +727,17 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             T += popc(b1) + popc(b2);
         }
         if (T > SA_TMAX) { overflow = true; break; }
-        if (lane == 0) L.seg[nck] = (unsigned short)T;
         for (int x = lane; x < T; x += 64) L.perm[x] = (unsigned char)x;
         wave_sync();
-        // every cap holds at least one entry (counts come from the same arithmetic)
+        // every entry lies in its cap's segment (the counts and the points
+        // come from the same arithmetic, so this holds; checked all the same)
         {
-            bool empty_seg = false;
-            for (int kl = lane; kl < nck; kl += 64) empty_seg = empty_seg || L.seg[kl] >= L.seg[kl + 1];
-            if (__ballot(empty_seg)) { overflow = true; break; }
+            bool bad = T != L.seg[nck];
+            for (int x = lane; x < T; x += 64) {
+                const int kl = L.tk[x];
+                bad = bad || x < L.seg[kl] || x >= L.seg[kl + 1];
+            }
+            if (__ballot(bad)) { overflow = true; break; }
         }
 
         // stable angular sort inside each cap's segment: rank on (angle, push
