@@ -1128,7 +1128,8 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
 
 // The JVP terms of one arc (12: role i, k, l, m x axis) or one vertex (9:
 // role i, in, out x axis) per thread: the primal geometry once, then one
-// tangent per direction (bit-identical to one full JVP per term).
+// tangent per direction of the roles other than i (bit-identical to one full
+// JVP per term); role i's terms are minus their sum.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) og_k_sasa_terms(og_sasa_data d, const double* pos) {
     const int n_a = min(d.counters[0], d.arc_cap);
     const int n_v = min(d.counters[1], d.vtx_cap);
