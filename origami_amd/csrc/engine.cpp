@@ -90,6 +90,7 @@ struct og_engine {
     og_sasa_data sasa{};
     og_sur* sur = nullptr;                 // learned Born-radius surrogate (optional)
     uint32_t* step_dev = nullptr;          // BAOAB step counter (Philox counter word)
+    double* noise = nullptr;               // [2][3N] the O step's normals, step s in half s & 1
     int graph_mode = 2;                    // og_run: 1 captured steps, 0 direct launches, 2 by size
     bool graph_ok = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
@@ -171,6 +172,7 @@ static void free_all(og_engine* e) {
     for (auto& g : e->gexec)
         if (g) (void)hipGraphExecDestroy(g);
     if (e->step_dev) (void)hipFree(e->step_dev);
+    if (e->noise) (void)hipFree(e->noise);
     if (e->sur) {
         og_sur_free(e->sur);
         delete e->sur;
@@ -504,6 +506,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->diverged, zi));
     HIPCHK(hipMalloc((void**)&e->step_dev, sizeof(uint32_t)));
     HIPCHK(hipMemset(e->step_dev, 0, sizeof(uint32_t)));
+    HIPCHK(hipMalloc((void**)&e->noise, 6 * (size_t)std::max(N, 1) * sizeof(double)));
     HIPCHK(upload(&e->clamped, zi));
     e->mass_h = mass;
     e->sys_tile_off_h = sys_tile_off;
@@ -579,8 +582,12 @@ static void invalidate_graphs(og_engine* e) {
 // sur_check: -1 = decide from the call counter; 0 / 1 = forced (graph capture)
 // forces = false (energies only): no pair-force kernel and no SASA terms or
 // sums; the bonded kernel still adds its forces into the accumulator.
+// next_noise (dynamics steps without SASA): the next step's O-step normals,
+// formed here by one thread per atom so that the cluster step only reads
+// them.  They stay on the engine stream: a second branch in a captured step
+// costs more than the kernel.
 static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1,
-                         bool forces = true) {
+                         bool forces = true, const og_noise_args* next_noise = nullptr) {
     const bool mode_a = flags & OG_MODE_A;
     if (!forces) sasa_scale = 0.0;
     // SASA on the side stream, overlapping everything up to the join below.
@@ -595,6 +602,10 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
                                       energy ? e->eacc : nullptr, e->atom_sys);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
         HIPCHK(hipEventRecord(e->join_ev, e->side));
+    }
+    if (next_noise) {
+        ProfScope ps(e, OG_K_CLUSTER);
+        og_launch_noise(*next_noise, e->stream);
     }
     {
         ProfScope p(e, OG_K_BONDED);
@@ -658,9 +669,8 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         en.rc2_nb = 100.0; en.kappa = 1.0 / 2000.0; en.crf = 3.0 / 20.0;
         og_launch_pair_energy64(a, en, mode_a, e->stream);
     }
-    if (fork) {
-        HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
-    } else if (flags & OG_SASA) {
+    if (fork) HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
+    if (!fork && (flags & OG_SASA)) {
         ProfScope p(e, OG_K_SASA);
         int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream,
                                 energy ? e->eacc : nullptr, e->atom_sys);
@@ -788,13 +798,14 @@ extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
     return OG_OK;
 }
 
-static int record_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check) {
+static int record_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check,
+                       const og_noise_args* next_noise) {
     const og_dynamics_params& p = e->dyn;
     {
         ProfScope ps(e, OG_K_CLUSTER);
         og_launch_cluster_step(c, e->stream);
     }
-    int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false, sur_check);
+    int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false, sur_check, true, next_noise);
     if (rc) return rc;
     {
         ProfScope ps(e, OG_K_BACK);
@@ -804,10 +815,10 @@ static int record_step(og_engine* e, const og_cluster_args& c, const og_back_arg
 }
 
 static int capture_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check,
-                        hipGraphExec_t* out) {
+                        const og_noise_args* next_noise, hipGraphExec_t* out) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-    const int rc = record_step(e, c, b, sur_check);
+    const int rc = record_step(e, c, b, sur_check, next_noise);
     const hipError_t er = hipStreamEndCapture(e->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -848,6 +859,25 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     b.step_dev = e->step_dev;
     // the device step counter restarts from the host's count each call
     og_launch_set_u32(e->step_dev, e->step, e->stream);
+    // Without SASA the O-step normals come from a buffer: the first step's
+    // here, each later step's during the force evaluation before it.  With
+    // SASA the cluster step forms its own; the extra launch would share the
+    // chip with the SASA branch, which sets the step time there.
+    og_noise_args nz{};
+    const og_noise_args* nzp = nullptr;
+    if (!(p.flags & OG_SASA)) {
+        nz.key0 = c.key0; nz.key1 = c.key1; nz.atom_sys = e->atom_sys; nz.atom_local = e->atom_local;
+        nz.n_atoms = e->n_atoms; nz.step_dev = e->step_dev; nz.noise = e->noise;
+        nz.ahead = 0;
+        {
+            ProfScope ps(e, OG_K_CLUSTER);
+            og_launch_noise(nz, e->stream);
+        }
+        nz.ahead = 1;
+        nzp = &nz;
+        c.noise = e->noise;
+        c.n_atoms = e->n_atoms;
+    }
     og_sur* sur = (p.flags & OG_SURROGATE) ? e->sur : nullptr;
     // Captured steps: every launch of one BAOAB step (cluster B-A-SHAKE-O-A,
     // forces, final kick) is recorded once into a hipGraph and replayed; the
@@ -859,9 +889,9 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     if (graphs) {
         if (!e->graph_ok) {
             invalidate_graphs(e);
-            if (int rc = capture_step(e, c, b, 0, &e->gexec[0])) return rc;
+            if (int rc = capture_step(e, c, b, 0, nzp, &e->gexec[0])) return rc;
             if (sur && sur->check_every > 0)
-                if (int rc = capture_step(e, c, b, 1, &e->gexec[1])) return rc;
+                if (int rc = capture_step(e, c, b, 1, nzp, &e->gexec[1])) return rc;
             e->graph_ok = true;
         }
         for (int k = 0; k < n_steps; ++k) {
@@ -875,7 +905,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     } else {
         for (int k = 0; k < n_steps; ++k) {
             b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
-            if (int rc = record_step(e, c, b, -1)) {
+            if (int rc = record_step(e, c, b, -1, nzp)) {
                 if (ke_dev) (void)hipFreeAsync(ke_dev, e->stream);
                 return rc;
             }

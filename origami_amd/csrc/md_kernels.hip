@@ -383,7 +383,10 @@ __device__ __forceinline__ bool shake_cluster(double (&xs)[3 * MC], const double
 }
 
 // One thread per cluster of at most MC atoms: clusters [c_begin, c_end).
-template <int MC>
+// PRE: the O step reads its normals from a.noise (formed by og_k_noise)
+// rather than forming them; a separate instance keeps the other one's
+// registers as they are.
+template <int MC, bool PRE>
 __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a, int c_begin, int c_end) {
     const int c = c_begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= c_end) return;
@@ -435,7 +438,12 @@ __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a, int 
                     const int g = at[t];
                     const double sigma = sqrt(om * kbt * dofc * OG_ACCEL / a.mass[g]);
                     double g0, g1, g2;
-                    noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], step, g0, g1, g2);
+                    if (PRE) {
+                        const double* nz = a.noise + (size_t)(step & 1u) * 3 * a.n_atoms + 3 * (size_t)g;
+                        g0 = nz[0]; g1 = nz[1]; g2 = nz[2];
+                    } else {
+                        noise3(a.key0, a.key1, (uint32_t)sys, (uint32_t)a.atom_local[g], step, g0, g1, g2);
+                    }
                     vs[3 * t] = a.alpha * vs[3 * t] + sigma * g0;
                     vs[3 * t + 1] = a.alpha * vs[3 * t + 1] + sigma * g1;
                     vs[3 * t + 2] = a.alpha * vs[3 * t + 2] + sigma * g2;
@@ -466,14 +474,39 @@ __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a, int 
     if (fails) atomicAdd(a.shake_fail + sys, fails);
 }
 
+__global__ void __launch_bounds__(256) og_k_noise(og_noise_args a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_atoms) return;
+    const uint32_t step = *a.step_dev + a.ahead;
+    double g0, g1, g2;
+    noise3(a.key0, a.key1, (uint32_t)a.atom_sys[i], (uint32_t)a.atom_local[i], step, g0, g1, g2);
+    double* o = a.noise + (size_t)(step & 1u) * 3 * a.n_atoms + 3 * (size_t)i;
+    o[0] = g0;
+    o[1] = g1;
+    o[2] = g2;
+}
+
+void og_launch_noise(const og_noise_args& a, hipStream_t s) {
+    if (a.n_atoms <= 0) return;
+    hipLaunchKernelGGL(og_k_noise, dim3((a.n_atoms + 255) / 256), dim3(256), 0, s, a);
+}
+
 void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s) {
     if (a.n_clusters <= 0) return;
-    if (a.n_small > 0)
-        hipLaunchKernelGGL(og_k_cluster_step<OG_SMALL_CL>, dim3((a.n_small + 255) / 256), dim3(256), 0, s, a,
-                           0, a.n_small);
-    if (a.n_clusters > a.n_small)
-        hipLaunchKernelGGL(og_k_cluster_step<OG_MAX_CL>, dim3((a.n_clusters - a.n_small + 255) / 256), dim3(256),
-                           0, s, a, a.n_small, a.n_clusters);
+    const dim3 g_small((a.n_small + 255) / 256), g_large((a.n_clusters - a.n_small + 255) / 256);
+    if (a.noise) {
+        if (a.n_small > 0)
+            hipLaunchKernelGGL((og_k_cluster_step<OG_SMALL_CL, true>), g_small, dim3(256), 0, s, a, 0, a.n_small);
+        if (a.n_clusters > a.n_small)
+            hipLaunchKernelGGL((og_k_cluster_step<OG_MAX_CL, true>), g_large, dim3(256), 0, s, a, a.n_small,
+                               a.n_clusters);
+    } else {
+        if (a.n_small > 0)
+            hipLaunchKernelGGL((og_k_cluster_step<OG_SMALL_CL, false>), g_small, dim3(256), 0, s, a, 0, a.n_small);
+        if (a.n_clusters > a.n_small)
+            hipLaunchKernelGGL((og_k_cluster_step<OG_MAX_CL, false>), g_large, dim3(256), 0, s, a, a.n_small,
+                               a.n_clusters);
+    }
 }
 
 __global__ void __launch_bounds__(256) og_k_zero_acc(long long* facc, long long* iacc, int n) {

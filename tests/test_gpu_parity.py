@@ -412,3 +412,51 @@ def test_sasa_energy_matches_oracle(E, c1_system, c2_system):
     f0 = eng.total_force(np.concatenate(pos), include_sasa=True, sasa_gamma_scale=0.25)
     f1, _ = eng.total_force(np.concatenate(pos), include_sasa=True, sasa_gamma_scale=0.25, energies=True)
     assert np.array_equal(f0, f1)
+
+
+@pytest.mark.parametrize("graphs", [False, True])
+def test_noise_ahead_matches_single_steps(E, c1_system, c3_system, graphs):
+    """Without SASA each step's O-step normals are formed during the force
+    evaluation of the step before (one og_run call of several steps); a run of
+    single-step calls forms every step's at the start of its call, the form
+    that test_baoab_step_matches_oracle pins to the oracle's Philox normals.
+    Both bitwise equal over steps that cover both halves of the buffer."""
+    systems = [c1_system, c3_system]
+
+    def make():
+        eng = E.Engine(systems)
+        v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(31 + k))
+                            for k, s in enumerate(systems)])
+        eng.set_velocities(v)
+        eng.dynamics_setup([300.0, 325.0], dt_fs=2.0, friction_ps_inv=2.0, seed=99,
+                           constrain_h_bonds=True)
+        return eng
+    many, single = make(), make()
+    many.use_graphs(graphs)
+    for rep in range(2):
+        many.run(5)
+        for _ in range(5):
+            single.run(1)
+        assert np.array_equal(many.positions(), single.positions()), rep
+        assert np.array_equal(many.velocities(), single.velocities()), rep
+
+
+def test_noise_paths_agree(E, c1_system, c3_system):
+    """Dynamics with SASA switched on at zero surface tension (no SASA force;
+    the cluster step forms its own normals) against dynamics without SASA (the
+    normals read from the precomputed buffer): the same trajectory up to the
+    rounding of the O-step's multiply-add."""
+    systems = [c1_system, c3_system]
+    engs = []
+    for sasa in (False, True):
+        eng = E.Engine(systems)
+        v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(41 + k))
+                            for k, s in enumerate(systems)])
+        eng.set_velocities(v)
+        eng.dynamics_setup([300.0, 325.0], dt_fs=2.0, friction_ps_inv=2.0, seed=123,
+                           constrain_h_bonds=True, include_sasa=sasa, sasa_gamma_scale=0.0)
+        eng.run(6)
+        engs.append(eng)
+    a, b = engs
+    assert np.abs(a.positions() - b.positions()).max() < 1e-9
+    assert np.abs(a.velocities() - b.velocities()).max() < 1e-9
