@@ -199,16 +199,36 @@ __global__ void __launch_bounds__(256) og_k_bonded_e(og_bonded_args a) { bonded_
 
 // Per atom: sum its (term, atom) force slots in slot order (bonds, angles,
 // dihedrals, impropers, each in list order, as origami's term loops add
-// them) in FP64, then one fixed-point add per component.
+// them) in FP64, then one fixed-point add per component.  The slots are
+// fetched eight at a time (indices, then their forces) so that the loads of
+// a batch are in flight together; the adds keep slot order.
 __global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
+    constexpr int B = 8;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n_atoms) return;
+    const int q1 = a.csr_start[i + 1];
     double fx = 0.0, fy = 0.0, fz = 0.0;
-    for (int q = a.csr_start[i]; q < a.csr_start[i + 1]; ++q) {
-        const long long s = a.csr_slot[q];
-        fx += a.tf[3 * s];
-        fy += a.tf[3 * s + 1];
-        fz += a.tf[3 * s + 2];
+    for (int q = a.csr_start[i]; q < q1; q += B) {
+        long long s[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) s[u] = q + u < q1 ? a.csr_slot[q + u] : -1;
+        double t[3 * B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (s[u] >= 0) {
+                t[3 * u] = a.tf[3 * s[u]];
+                t[3 * u + 1] = a.tf[3 * s[u] + 1];
+                t[3 * u + 2] = a.tf[3 * s[u] + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+            if (s[u] >= 0) {
+                fx += t[3 * u];
+                fy += t[3 * u + 1];
+                fz += t[3 * u + 2];
+            }
+        }
     }
     og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix(fx));
     og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix(fy));
