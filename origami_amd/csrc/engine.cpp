@@ -583,9 +583,9 @@ static void invalidate_graphs(og_engine* e) {
 // forces = false (energies only): no pair-force kernel and no SASA terms or
 // sums; the bonded kernel still adds its forces into the accumulator.
 // next_noise (dynamics steps without SASA): the next step's O-step normals,
-// formed here by one thread per atom so that the cluster step only reads
-// them.  They stay on the engine stream: a second branch in a captured step
-// costs more than the kernel.
+// formed one thread per atom by the bonded gather kernel, so that the cluster
+// step only reads them.  They stay on the engine stream: a second branch in a
+// captured step costs more than the work.
 static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1,
                          bool forces = true, const og_noise_args* next_noise = nullptr) {
     const bool mode_a = flags & OG_MODE_A;
@@ -603,10 +603,6 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
         HIPCHK(hipEventRecord(e->join_ev, e->side));
     }
-    if (next_noise) {
-        ProfScope ps(e, OG_K_CLUSTER);
-        og_launch_noise(*next_noise, e->stream);
-    }
     {
         ProfScope p(e, OG_K_BONDED);
         og_bonded_args b{};
@@ -616,6 +612,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         b.dih = e->dih; b.dih_p = e->dih_p; b.imp = e->imp; b.imp_p = e->imp_p;
         b.tf = e->bonded_tf; b.csr_start = e->bonded_csr_start; b.csr_slot = e->bonded_csr_slot;
         b.n_atoms = e->n_atoms;
+        if (next_noise) b.noise = *next_noise;
         og_launch_bonded(b, energy, e->stream);
     }
     og_pair_args a = pair_args(e);

@@ -197,15 +197,33 @@ __device__ void bonded_body(const og_bonded_args& a) {
 __global__ void __launch_bounds__(256) og_k_bonded(og_bonded_args a) { bonded_body<false>(a); }
 __global__ void __launch_bounds__(256) og_k_bonded_e(og_bonded_args a) { bonded_body<true>(a); }
 
+__device__ void noise3(uint32_t k0, uint32_t k1, uint32_t sys, uint32_t atom, uint32_t step,
+                       double& g0, double& g1, double& g2);
+
+// The O-step normals of atom i for step *step_dev + ahead.
+__device__ __forceinline__ void noise_atom(const og_noise_args& a, int i) {
+    const uint32_t step = *a.step_dev + a.ahead;
+    double g0, g1, g2;
+    noise3(a.key0, a.key1, (uint32_t)a.atom_sys[i], (uint32_t)a.atom_local[i], step, g0, g1, g2);
+    double* o = a.noise + (size_t)(step & 1u) * 3 * a.n_atoms + 3 * (size_t)i;
+    o[0] = g0;
+    o[1] = g1;
+    o[2] = g2;
+}
+
 // Per atom: sum its (term, atom) force slots in slot order (bonds, angles,
 // dihedrals, impropers, each in list order, as origami's term loops add
 // them) in FP64, then one fixed-point add per component.  The slots are
 // fetched eight at a time (indices, then their forces) so that the loads of
-// a batch are in flight together; the adds keep slot order.
+// a batch are in flight together; the adds keep slot order.  NOISE: the
+// thread also forms its atom's O-step normals for the next step (ALU work
+// beside the gather's load latency).
+template <bool NOISE>
 __global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
     constexpr int B = 8;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n_atoms) return;
+    if (NOISE) noise_atom(a.noise, i);
     const int q1 = a.csr_start[i + 1];
     double fx = 0.0, fy = 0.0, fz = 0.0;
     for (int q = a.csr_start[i]; q < q1; q += B) {
@@ -237,13 +255,19 @@ __global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
 
 void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
     const int total = a.nb + a.na + a.nd + a.ni;
-    if (total <= 0) return;
+    if (total <= 0) {
+        if (a.noise.noise) og_launch_noise(a.noise, s);
+        return;
+    }
     int g = (total + 255) / 256;
     if (g > 4096) g = 4096;
     if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_bonded, dim3(g), dim3(256), 0, s, a);
-    if (a.n_atoms > 0)
-        hipLaunchKernelGGL(og_k_bonded_gather, dim3((a.n_atoms + 255) / 256), dim3(256), 0, s, a);
+    if (a.n_atoms > 0) {
+        const dim3 gg((a.n_atoms + 255) / 256);
+        if (a.noise.noise) hipLaunchKernelGGL(og_k_bonded_gather<true>, gg, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(og_k_bonded_gather<false>, gg, dim3(256), 0, s, a);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -496,14 +520,7 @@ __global__ void __launch_bounds__(256) og_k_cluster_step(og_cluster_args a, int 
 
 __global__ void __launch_bounds__(256) og_k_noise(og_noise_args a) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n_atoms) return;
-    const uint32_t step = *a.step_dev + a.ahead;
-    double g0, g1, g2;
-    noise3(a.key0, a.key1, (uint32_t)a.atom_sys[i], (uint32_t)a.atom_local[i], step, g0, g1, g2);
-    double* o = a.noise + (size_t)(step & 1u) * 3 * a.n_atoms + 3 * (size_t)i;
-    o[0] = g0;
-    o[1] = g1;
-    o[2] = g2;
+    if (i < a.n_atoms) noise_atom(a, i);
 }
 
 void og_launch_noise(const og_noise_args& a, hipStream_t s) {

@@ -31,6 +31,21 @@ enum {
     OG_E_GB_SELF, OG_E_GB_PAIR, OG_E_SASA, OG_E_KIN, OG_NE
 };
 
+// The O step's three normals per atom for the step *step_dev + ahead, written
+// into half (step & 1) of noise [2][3N]; they depend only on the key, the
+// system, the atom and the step, so the next step's are formed while this
+// step's forces are computed.
+struct og_noise_args {
+    uint32_t key0, key1;
+    const int* atom_sys;
+    const int* atom_local;
+    int n_atoms;
+    const uint32_t* step_dev;
+    uint32_t ahead;
+    double* noise;
+};
+void og_launch_noise(const og_noise_args& a, hipStream_t s);
+
 struct og_bonded_args {
     const double* pos;
     long long* facc;
@@ -47,6 +62,7 @@ struct og_bonded_args {
     const int* csr_start;                         // [N+1]
     const int* csr_slot;                          // [slots]
     int n_atoms;
+    og_noise_args noise;                          // if noise.noise is set, formed beside the gather
 };
 
 struct og_radii_args {
@@ -93,21 +109,6 @@ struct og_cluster_args {
     const double* noise;             // if set, the O step's normals: [2][3N], step s in half s & 1
     int n_atoms;
 };
-
-// The O step's three normals per atom for the step *step_dev + ahead, written
-// into half (step & 1) of noise [2][3N]; they depend only on the key, the
-// system, the atom and the step, so the next step's are formed while this
-// step's forces are computed.
-struct og_noise_args {
-    uint32_t key0, key1;
-    const int* atom_sys;
-    const int* atom_local;
-    int n_atoms;
-    const uint32_t* step_dev;
-    uint32_t ahead;
-    double* noise;
-};
-void og_launch_noise(const og_noise_args& a, hipStream_t s);
 
 struct og_back_args {
     double* vel; double* force;
