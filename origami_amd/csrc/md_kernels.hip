@@ -213,7 +213,77 @@ __device__ __forceinline__ void noise_atom(const og_noise_args& a, int i) {
 
 // Per atom: sum its (term, atom) force slots in slot order (bonds, angles,
 // dihedrals, impropers, each in list order, as origami's term loops add
-// them) in FP64, then one fixed-point add per component.  The slots are
+// them) in FP64, then one fixed-point add per component.  A block takes 64
+// consecutive atoms, whose slots are one contiguous CSR range: all 256
+// threads first copy the slot forces of that range into LDS (independent
+// loads, all in flight together, where a thread walking its own atom's
+// slots would wait on each one), then wave 0 sums each atom's slots from
+// LDS in slot order.  Ranges longer than the LDS copy go in chunks.
+// NOISE: wave 1 meanwhile forms the block's atoms' O-step normals for the
+// next step.
+#define OG_GATHER_LDS_MAX_ATOMS 20000       // batches below this size use the LDS form
+#define OG_GATHER_ATOMS 64
+#define OG_GATHER_CHUNK 2048                // 8 entries per thread of a 256-thread block
+template <bool NOISE>
+__global__ void __launch_bounds__(256) og_k_bonded_gather_lds(og_bonded_args a) {
+    __shared__ double s_t[3 * OG_GATHER_CHUNK];
+    const int a0 = blockIdx.x * OG_GATHER_ATOMS;
+    const int a1 = min(a0 + OG_GATHER_ATOMS, a.n_atoms);
+    const int t = threadIdx.x;
+    const int i = a0 + t;
+    const bool own = t < OG_GATHER_ATOMS && i < a1;
+    const int q_begin = a.csr_start[a0], q_end = a.csr_start[a1];
+    int q = own ? a.csr_start[i] : 0;
+    const int qe = own ? a.csr_start[i + 1] : 0;
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    for (int c0 = q_begin; c0 < q_end; c0 += OG_GATHER_CHUNK) {
+        const int c1 = min(c0 + OG_GATHER_CHUNK, q_end);
+        __syncthreads();                                  // the previous chunk is summed
+        // OG_GATHER_CHUNK / 256 = 8 entries per thread, every load issued
+        // before the first LDS store
+        size_t sl[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = c0 + t + 256 * u;
+            sl[u] = k < c1 ? 3 * (size_t)a.csr_slot[k] : 0;
+        }
+        double v[24];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            if (c0 + t + 256 * u < c1) {
+                v[3 * u] = a.tf[sl[u]];
+                v[3 * u + 1] = a.tf[sl[u] + 1];
+                v[3 * u + 2] = a.tf[sl[u] + 2];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = t + 256 * u;
+            if (c0 + e < c1) {
+                s_t[3 * e] = v[3 * u];
+                s_t[3 * e + 1] = v[3 * u + 1];
+                s_t[3 * e + 2] = v[3 * u + 2];
+            }
+        }
+        __syncthreads();
+        for (; q < qe && q < c1; ++q) {
+            const double* d = s_t + 3 * (q - c0);
+            fx += d[0];
+            fy += d[1];
+            fz += d[2];
+        }
+    }
+    if (NOISE && t >= OG_GATHER_ATOMS && t < 2 * OG_GATHER_ATOMS && a0 + t - OG_GATHER_ATOMS < a1)
+        noise_atom(a.noise, a0 + t - OG_GATHER_ATOMS);
+    if (own) {
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix(fx));
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix(fy));
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
+    }
+}
+
+// The same sums, a thread per atom for large batches (where the LDS copy
+// would take LDS from the SASA kernels running beside it): the slots are
 // fetched eight at a time (indices, then their forces) so that the loads of
 // a batch are in flight together; the adds keep slot order.  NOISE: the
 // thread also forms its atom's O-step normals for the next step (ALU work
@@ -263,7 +333,12 @@ void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
     if (g > 4096) g = 4096;
     if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_bonded, dim3(g), dim3(256), 0, s, a);
-    if (a.n_atoms > 0) {
+    if (a.n_atoms <= 0) return;
+    if (a.n_atoms < OG_GATHER_LDS_MAX_ATOMS) {
+        const dim3 gg((a.n_atoms + OG_GATHER_ATOMS - 1) / OG_GATHER_ATOMS);
+        if (a.noise.noise) hipLaunchKernelGGL(og_k_bonded_gather_lds<true>, gg, dim3(256), 0, s, a);
+        else hipLaunchKernelGGL(og_k_bonded_gather_lds<false>, gg, dim3(256), 0, s, a);
+    } else {
         const dim3 gg((a.n_atoms + 255) / 256);
         if (a.noise.noise) hipLaunchKernelGGL(og_k_bonded_gather<true>, gg, dim3(256), 0, s, a);
         else hipLaunchKernelGGL(og_k_bonded_gather<false>, gg, dim3(256), 0, s, a);

@@ -460,3 +460,18 @@ def test_noise_paths_agree(E, c1_system, c3_system):
     a, b = engs
     assert np.abs(a.positions() - b.positions()).max() < 1e-9
     assert np.abs(a.velocities() - b.velocities()).max() < 1e-9
+
+
+def test_gather_forms_agree(E, c2_system):
+    """The bonded gather sums a thread per atom on batches of 20 000 atoms and
+    more and from an LDS copy of each block's slots below that; both add in
+    slot order, so a system's forces in a 16-copy batch (21 008 atoms) equal
+    its forces alone bit for bit (tile pairs never straddle systems and every
+    sum is fixed point)."""
+    pos = [jittered(c2_system, 60 + k) for k in range(16)]
+    big = E.Engine([c2_system] * 16)
+    assert big.n_atoms >= 20000
+    f = big.total_force(np.concatenate(pos))
+    for k in (0, 7, 15):
+        one = E.Engine([c2_system])
+        assert np.array_equal(big.split(f)[k], one.total_force(pos[k])), k
