@@ -23,6 +23,7 @@
 
 // og_run's default: captured steps below this many atoms
 #define OG_GRAPH_MAX_ATOMS 20000
+#define OG_SPLIT_MAX_TILES 4           // systems up to 256 atoms: off-diagonal tile pairs in halves
 
 static thread_local std::string g_err;
 
@@ -55,6 +56,7 @@ static hipError_t upload(T** dst, const std::vector<T>& src) {
 
 struct og_engine {
     int n_sys = 0, n_atoms = 0, n_tiles = 0, n_work = 0, n_masked = 0, n_clusters = 0, n_clusters_small = 0;
+    int n_tile_pairs = 0;                  // n_work counts a split tile pair twice
     long long n_pairs = 0;
     int n_sasa_work = 0;
     std::vector<int> atom_off, sys_n;
@@ -333,7 +335,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
         std::map<std::pair<int, int>, int> mask_of;
         const int w0 = (int)work.size();
         for (int I = 0; I < nt; ++I)
-            for (int J = I; J < nt; ++J) work.push_back({t0 + I, t0 + J, -1, s});
+            for (int J = I; J < nt; ++J) work.push_back({t0 + I, t0 + J, -1, s, -1});
         auto tile_pair_index = [&](int I, int J) {  // I <= J local
             return w0 + I * nt - I * (I - 1) / 2 + (J - I);
         };
@@ -348,6 +350,28 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
         };
         for (int k = 0; k < d.n_excl; ++k) set_bit(d.excl[2 * k], d.excl[2 * k + 1], 0);
         for (int k = 0; k < d.n_14; ++k) set_bit(d.pair14[2 * k], d.pair14[2 * k + 1], 1);
+        e->n_tile_pairs += (int)work.size() - w0;
+        // Systems of at most OG_SPLIT_MAX_TILES tiles: every off-diagonal
+        // tile pair becomes two work items of 32 steps each (the diagonal
+        // ones already take 32), so that a batch of small systems, whose
+        // pair kernels are bound by one wave's 64-step latency, runs on twice
+        // the waves.  Decided per system, so a system's forces do not depend
+        // on what else is in the batch.
+        if (nt <= OG_SPLIT_MAX_TILES) {
+            std::vector<og_work> seg(work.begin() + w0, work.end());
+            work.resize(w0);
+            for (const og_work& w : seg) {
+                if (w.ti == w.tj) {
+                    work.push_back(w);
+                } else {
+                    og_work h = w;
+                    h.part = 0;
+                    work.push_back(h);
+                    h.part = 1;
+                    work.push_back(h);
+                }
+            }
+        }
         // Bonded terms (global atom indices).
         for (int k = 0; k < d.n_bonds; ++k) {
             bonds.push_back(off + d.bonds[2 * k]); bonds.push_back(off + d.bonds[2 * k + 1]);
@@ -523,7 +547,7 @@ extern "C" int og_engine_info(const og_engine* e, int64_t* info) {
     info[0] = e->n_atoms;
     info[1] = e->n_sys;
     info[2] = e->n_tiles;
-    info[3] = e->n_work;
+    info[3] = e->n_tile_pairs;
     info[4] = e->n_masked;
     info[5] = e->n_clusters;
     info[6] = e->n_pairs;

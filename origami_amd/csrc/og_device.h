@@ -37,6 +37,7 @@ struct og_work {              // one tile pair (I <= J) of one system
     int32_t ti, tj;           // global tile indices
     int32_t mask;             // index into the exclusion-mask table, -1 if none
     int32_t sys;              // system index
+    int32_t part;             // -1: all 64 steps; 0 / 1: steps 0-31 / 32-63 (I < J only)
 };
 
 __device__ __forceinline__ long long og_to_fix(double x) {

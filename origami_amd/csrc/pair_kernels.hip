@@ -132,7 +132,9 @@ __device__ __forceinline__ void wave_lds_order() {
 // 0, so that the pair of steps k, k + 1 reads entries idx and idx + 1 without
 // a wrap (Q follows P in the same array: one address for all four reads).
 // Same arithmetic, in the same order, as the rotating form.
-template <bool MODEA>
+// PART (og_work::part): -1 all 64 steps, 0 steps 0-31, 1 steps 32-63.  After
+// steps 0-31 the j-side sums sit 32 lanes from home, hence the swap.
+template <bool MODEA, int PART>
 __device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const LaneData& di,
                                                   const float4* P, const float4* Q, int lane,
                                                   float& acc_i, float& acc_j) {
@@ -151,13 +153,15 @@ __device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const L
         gj += ok ? hji : 0.0f;
         gj = rot(gj);                  // after step 63 this is the way home
     };
-    for (int k = 0; k < 64; k += 2) {
+    constexpr int kb = PART == 1 ? 32 : 0, ke = PART == 0 ? 32 : 64;
+    for (int k = kb; k < ke; k += 2) {
         const int idx = (lane + k) & 63;
         const float4 p0 = P[idx], p1 = P[idx + 1];
         const float4 q0 = Q[idx], q1 = Q[idx + 1];
         step(p0, q0);
         step(p1, q1);
     }
+    if (PART == 0) gj = swap32(gj);
     acc_j += gj;
 }
 
@@ -189,7 +193,9 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
             Q[lane] = qj;
             if (lane == 0) { P[OG_TILE] = pj; Q[OG_TILE] = qj; }
             wave_lds_order();
-            descreen_tile_lds<MODEA>(a, di, P, Q, lane, acc_i, acc_j);
+            if (it.part < 0) descreen_tile_lds<MODEA, -1>(a, di, P, Q, lane, acc_i, acc_j);
+            else if (it.part == 0) descreen_tile_lds<MODEA, 0>(a, di, P, Q, lane, acc_i, acc_j);
+            else descreen_tile_lds<MODEA, 1>(a, di, P, Q, lane, acc_i, acc_j);
         }
         if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix((double)acc_i));
         if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix((double)acc_j));
@@ -211,7 +217,10 @@ struct ForceAcc {
     float gx, gy, gz;       // j-side (lane j)
 };
 
-template <bool MODEA, bool MASKED, bool DIAG>
+// PART (og_work::part, off-diagonal tiles): -1 all 64 steps, 0 steps 0-31,
+// 1 steps 32-63 (the j data start half a turn round); after steps 0-31 the
+// j-side sums sit 32 lanes from home, hence the swap.
+template <bool MODEA, bool MASKED, bool DIAG, int PART>
 __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData& di,
                                            const LaneData& dj, int lane,
                                            unsigned long long xmask, unsigned long long m14,
@@ -222,13 +231,18 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
     int jid = lane;                         // j index within tile J held by this lane
     float gx = 0.0f, gy = 0.0f, gz = 0.0f;
     const float qki = kGbK * di.q;
-    const int k0 = DIAG ? 1 : 0, k1 = DIAG ? 32 : 63;
+    constexpr int k0 = DIAG ? 1 : (PART == 1 ? 32 : 0), k1 = DIAG ? 32 : (PART == 0 ? 31 : 63);
     auto advance = [&]() {
         xj = rot(xj); yj = rot(yj); zj = rot(zj); qj = rot(qj);
         sej = rot(sej); rmj = rot(rmj); Rj = rot(Rj); iRj = rot(iRj);
         if (MASKED) { se14j = rot(se14j); rm14j = rot(rm14j); jid = roti(jid); }
     };
     if (DIAG) advance();
+    if (!DIAG && PART == 1) {
+        xj = swap32(xj); yj = swap32(yj); zj = swap32(zj); qj = swap32(qj);
+        sej = swap32(sej); rmj = swap32(rmj); Rj = swap32(Rj); iRj = swap32(iRj);
+        if (MASKED) { se14j = swap32(se14j); rm14j = swap32(rm14j); jid = lane ^ 32; }
+    }
 #pragma unroll 2
     for (int k = k0; k <= k1; ++k) {
         const float dx = xj - di.x, dy = yj - di.y, dz = zj - di.z;
@@ -284,6 +298,7 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
         gx = swap32(gx); gy = swap32(gy); gz = swap32(gz);
     } else {
         gx = rot(gx); gy = rot(gy); gz = rot(gz);
+        if (PART == 0) { gx = swap32(gx); gy = swap32(gy); gz = swap32(gz); }
     }
     acc.gx += gx;
     acc.gy += gy;
@@ -295,7 +310,7 @@ __device__ __forceinline__ void force_tile(const og_pair_args& a, const LaneData
 // entry 64 = entry 0 (steps k and k + 1 read entries idx and idx + 1; Q
 // follows P in the same array).  The arithmetic is that of force_tile's
 // unmasked path.
-template <bool MODEA>
+template <bool MODEA, int PART>
 __device__ __forceinline__ void force_tile_lds(const og_pair_args& a, const LaneData& di,
                                                const float4* P, const float4* Q, int lane,
                                                ForceAcc& acc) {
@@ -333,13 +348,15 @@ __device__ __forceinline__ void force_tile_lds(const og_pair_args& a, const Lane
         gy = rot(fmaf(-c, dy, gy));
         gz = rot(fmaf(-c, dz, gz));
     };
-    for (int k = 0; k < 64; k += 2) {
+    constexpr int kb = PART == 1 ? 32 : 0, ke = PART == 0 ? 32 : 64;
+    for (int k = kb; k < ke; k += 2) {
         const int idx = (lane + k) & 63;
         const float4 p0 = P[idx], p1 = P[idx + 1];
         const float4 q0 = Q[idx], q1 = Q[idx + 1];
         step(p0, q0);
         step(p1, q1);
     }
+    if (PART == 0) { gx = swap32(gx); gy = swap32(gy); gz = swap32(gz); }
     acc.gx += gx;
     acc.gy += gy;
     acc.gz += gz;
@@ -367,11 +384,13 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
             m14 = a.masks[(size_t)it.mask * 128 + 64 + lane];
         }
         if (it.ti == it.tj) {
-            force_tile<MODEA, true, true>(a, di, di, lane, xm, m14, acc);
+            force_tile<MODEA, true, true, -1>(a, di, di, lane, xm, m14, acc);
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
             if (it.mask >= 0) {
-                force_tile<MODEA, true, false>(a, di, dj, lane, xm, m14, acc);
+                if (it.part < 0) force_tile<MODEA, true, false, -1>(a, di, dj, lane, xm, m14, acc);
+                else if (it.part == 0) force_tile<MODEA, true, false, 0>(a, di, dj, lane, xm, m14, acc);
+                else force_tile<MODEA, true, false, 1>(a, di, dj, lane, xm, m14, acc);
             } else {
                 const float4 pj = make_float4(dj.x, dj.y, dj.z, dj.q);
                 const float4 qj = make_float4(dj.par.x, dj.par.y, dj.gb.z, dj.gb.w);
@@ -380,7 +399,9 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
                 Q[lane] = qj;
                 if (lane == 0) { P[OG_TILE] = pj; Q[OG_TILE] = qj; }
                 wave_lds_order();
-                force_tile_lds<MODEA>(a, di, P, Q, lane, acc);
+                if (it.part < 0) force_tile_lds<MODEA, -1>(a, di, P, Q, lane, acc);
+                else if (it.part == 0) force_tile_lds<MODEA, 0>(a, di, P, Q, lane, acc);
+                else force_tile_lds<MODEA, 1>(a, di, P, Q, lane, acc);
             }
         }
         if (lane < ci) {
@@ -450,6 +471,7 @@ __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
     const double pref = -(1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
     for (int w = wave; w < a.n_work; w += n_waves) {
         const og_work it = a.work[w];
+        if (it.part == 1) continue;                 // a split tile pair is evaluated whole once
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const bool vi = lane < ci, vj = lane < cj;

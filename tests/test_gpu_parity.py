@@ -475,3 +475,32 @@ def test_gather_forms_agree(E, c2_system):
     for k in (0, 7, 15):
         one = E.Engine([c2_system])
         assert np.array_equal(big.split(f)[k], one.total_force(pos[k])), k
+
+
+@pytest.mark.parametrize("mode_a", [False, True])
+def test_split_tile_pairs_small_system(E, c3_system, mode_a):
+    """Systems of at most four tiles (here 171 atoms, three tiles) run each
+    off-diagonal tile pair as two 32-step work items: forces, Born radii and
+    energies against the oracle, at generic geometry, alone and in a batch
+    beside a larger (unsplit) system."""
+    s = c3_system
+    pos = jittered(s, 70)
+    orc = O.Oracle(s)
+    cut = O.MODE_A if mode_a else O.MODE_B
+    fref = orc.total_force(pos, **cut)
+    eng = E.Engine([s])
+    assert eng.info()["tile_pairs"] == 6
+    f = eng.total_force(pos, mode_a=mode_a)
+    err = rel_rms(f, fref)
+    print("171 atoms mode_a=%s rel force RMS %.3e" % (mode_a, err))
+    assert err < FORCE_RMS_TOL
+    ref = orc.total_energy(pos, cut_nb=cut["cut_nb"], cut_descreen=cut["cut_descreen"])
+    got = eng.total_energy(pos, mode_a=mode_a)[0]
+    for k in ("lj", "coulomb"):
+        assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
+    assert abs(got["gb_self"] - ref["gb_self"]) <= 1e-9 * abs(ref["gb_self"])
+    assert abs(got["gb_pair"] - ref["gb_pair"]) <= 1e-4 * abs(ref["gb_pair"])
+    big = S.seeded_chain(1, 20)                     # 300+ atoms: whole tile pairs
+    both = E.Engine([big, s])
+    fb = both.total_force(np.concatenate([big.pos, pos]), mode_a=mode_a)
+    assert np.array_equal(both.split(fb)[1], f)
