@@ -627,9 +627,13 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
         HIPCHK(hipEventRecord(e->join_ev, e->side));
     }
+    og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
+    // Small batches (the LDS gather's range): the bonded gather shares one
+    // launch with the Born radii after the descreening pass.
+    const bool fuse = !energy && !sur && !e->prof && e->n_atoms < OG_GATHER_LDS_MAX_ATOMS;
+    og_bonded_args b{};
     {
         ProfScope p(e, OG_K_BONDED);
-        og_bonded_args b{};
         b.pos = e->pos; b.facc = e->facc; b.eacc = e->eacc; b.atom_sys = e->atom_sys;
         b.nb = e->nb; b.na = e->na; b.nd = e->nd; b.ni = e->ni;
         b.bonds = e->bonds; b.bond_p = e->bond_p; b.angles = e->angles; b.angle_p = e->angle_p;
@@ -637,14 +641,14 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         b.tf = e->bonded_tf; b.csr_start = e->bonded_csr_start; b.csr_slot = e->bonded_csr_slot;
         b.n_atoms = e->n_atoms;
         if (next_noise) b.noise = *next_noise;
-        og_launch_bonded(b, energy, e->stream);
+        if (fuse) og_launch_bonded_terms(b, e->stream);
+        else og_launch_bonded(b, energy, e->stream);
     }
     og_pair_args a = pair_args(e);
     // Surrogate radii: descriptors + MLP for the systems still in surrogate
     // mode; their tile pairs skip the descreening pass.  Every check_every
     // calls the plain pass runs for every system too and systems whose radii
     // deviate by more than the tolerance go back to plain GB.
-    og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
     bool check = false;
     if (sur) {
         if (sur_check < 0) {
@@ -672,7 +676,8 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
         r.eacc = e->eacc; r.n_atoms = e->n_atoms;
         if (sur) { r.sur_mode = sur->mode; r.r_sur = sur->r_sur; r.dev = sur->dev; r.check = check ? 1 : 0; }
-        og_launch_radii(r, energy, e->stream);
+        if (fuse) og_launch_radii_gather(r, b, e->stream);
+        else og_launch_radii(r, energy, e->stream);
         if (sur && check && og_sur_check_apply(sur)) {
             if (fork) (void)hipStreamWaitEvent(e->stream, e->join_ev, 0);
             return fail(OG_ERR_HIP, "surrogate launch failed: %s", og_sur_error());

@@ -221,13 +221,12 @@ __device__ __forceinline__ void noise_atom(const og_noise_args& a, int i) {
 // LDS in slot order.  Ranges longer than the LDS copy go in chunks.
 // NOISE: wave 1 meanwhile forms the block's atoms' O-step normals for the
 // next step.
-#define OG_GATHER_LDS_MAX_ATOMS 20000       // batches below this size use the LDS form
 #define OG_GATHER_ATOMS 64
 #define OG_GATHER_CHUNK 2048                // 8 entries per thread of a 256-thread block
 template <bool NOISE>
-__global__ void __launch_bounds__(256) og_k_bonded_gather_lds(og_bonded_args a) {
+__device__ __forceinline__ void gather_lds_body(const og_bonded_args& a, int blk) {
     __shared__ double s_t[3 * OG_GATHER_CHUNK];
-    const int a0 = blockIdx.x * OG_GATHER_ATOMS;
+    const int a0 = blk * OG_GATHER_ATOMS;
     const int a1 = min(a0 + OG_GATHER_ATOMS, a.n_atoms);
     const int t = threadIdx.x;
     const int i = a0 + t;
@@ -281,6 +280,10 @@ __global__ void __launch_bounds__(256) og_k_bonded_gather_lds(og_bonded_args a) 
         og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
     }
 }
+template <bool NOISE>
+__global__ void __launch_bounds__(256) og_k_bonded_gather_lds(og_bonded_args a) {
+    gather_lds_body<NOISE>(a, blockIdx.x);
+}
 
 // The same sums, a thread per atom for large batches (where the LDS copy
 // would take LDS from the SASA kernels running beside it): the slots are
@@ -323,16 +326,24 @@ __global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
     og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
 }
 
+static int bonded_grid(int total) {
+    const int g = (total + 255) / 256;
+    return g > 4096 ? 4096 : g;
+}
+
+void og_launch_bonded_terms(const og_bonded_args& a, hipStream_t s) {
+    const int total = a.nb + a.na + a.nd + a.ni;
+    if (total > 0) hipLaunchKernelGGL(og_k_bonded, dim3(bonded_grid(total)), dim3(256), 0, s, a);
+}
+
 void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
     const int total = a.nb + a.na + a.nd + a.ni;
     if (total <= 0) {
         if (a.noise.noise) og_launch_noise(a.noise, s);
         return;
     }
-    int g = (total + 255) / 256;
-    if (g > 4096) g = 4096;
-    if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(g), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(og_k_bonded, dim3(g), dim3(256), 0, s, a);
+    if (energy) hipLaunchKernelGGL(og_k_bonded_e, dim3(bonded_grid(total)), dim3(256), 0, s, a);
+    else og_launch_bonded_terms(a, s);
     if (a.n_atoms <= 0) return;
     if (a.n_atoms < OG_GATHER_LDS_MAX_ATOMS) {
         const dim3 gg((a.n_atoms + OG_GATHER_ATOMS - 1) / OG_GATHER_ATOMS);
@@ -349,8 +360,7 @@ void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s) {
 // Born radii from the descreening integral (gb.rs:327-341), GB self energy.
 // ---------------------------------------------------------------------------
 template <bool ENERGY>
-__global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void radii_body(const og_radii_args& a, int i) {
     int sys = 0;
     long long e_fix = 0;
     bool has_e = false;
@@ -398,6 +408,30 @@ __global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
     }
     // GB self energy: one atomic per system in the wave
     if (ENERGY) og_wave_add_fix_keyed(a.eacc, sys, OG_NE, OG_E_GB_SELF, e_fix, has_e);
+}
+
+template <bool ENERGY>
+__global__ void __launch_bounds__(256) og_k_radii(og_radii_args a) {
+    radii_body<ENERGY>(a, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// Born radii and the bonded gather in one launch, for small batches whose
+// force evaluation is a chain of latency-bound launches: the two are
+// independent (the gather needs only the bonded terms, the radii only the
+// descreening), so blocks [0, nr) take the radii and the rest the gather's
+// LDS form, side by side.
+template <bool NOISE>
+__global__ void __launch_bounds__(256) og_k_radii_gather(og_radii_args r, og_bonded_args b, int nr) {
+    if ((int)blockIdx.x < nr) radii_body<false>(r, blockIdx.x * blockDim.x + threadIdx.x);
+    else gather_lds_body<NOISE>(b, blockIdx.x - nr);
+}
+
+void og_launch_radii_gather(const og_radii_args& r, const og_bonded_args& b, hipStream_t s) {
+    const int nr = (r.n_atoms + 255) / 256;
+    const int ng = b.n_atoms > 0 ? (b.n_atoms + OG_GATHER_ATOMS - 1) / OG_GATHER_ATOMS : 0;
+    if (nr + ng <= 0) return;
+    if (b.noise.noise) hipLaunchKernelGGL(og_k_radii_gather<true>, dim3(nr + ng), dim3(256), 0, s, r, b, nr);
+    else hipLaunchKernelGGL(og_k_radii_gather<false>, dim3(nr + ng), dim3(256), 0, s, r, b, nr);
 }
 
 void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s) {

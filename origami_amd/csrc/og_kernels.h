@@ -136,6 +136,12 @@ struct og_energy64_args {            // FP64 per-atom data for the energy pass
 void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, bool mode_a,
                              hipStream_t s);
 void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s);
+#define OG_GATHER_LDS_MAX_ATOMS 20000   // batches below this size gather through LDS
+// the bonded terms alone (forces), the gather left to og_launch_radii_gather
+void og_launch_bonded_terms(const og_bonded_args& a, hipStream_t s);
+// Born radii (forces only, no surrogate) and the bonded gather's LDS form
+// in one launch; batches below 20 000 atoms
+void og_launch_radii_gather(const og_radii_args& r, const og_bonded_args& b, hipStream_t s);
 void og_launch_radii(const og_radii_args& a, bool energy, hipStream_t s);
 void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s);
 void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream_t s);
