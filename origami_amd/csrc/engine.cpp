@@ -628,8 +628,9 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         HIPCHK(hipEventRecord(e->join_ev, e->side));
     }
     og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
-    // Small batches (the LDS gather's range): the bonded gather shares one
-    // launch with the Born radii after the descreening pass.
+    // Small batches (the LDS gather's range): the bonded terms share one
+    // launch with the descreening pass, the bonded gather one with the Born
+    // radii after it.
     const bool fuse = !energy && !sur && !e->prof && e->n_atoms < OG_GATHER_LDS_MAX_ATOMS;
     og_bonded_args b{};
     {
@@ -641,8 +642,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         b.tf = e->bonded_tf; b.csr_start = e->bonded_csr_start; b.csr_slot = e->bonded_csr_slot;
         b.n_atoms = e->n_atoms;
         if (next_noise) b.noise = *next_noise;
-        if (fuse) og_launch_bonded_terms(b, e->stream);
-        else og_launch_bonded(b, energy, e->stream);
+        if (!fuse) og_launch_bonded(b, energy, e->stream);
     }
     og_pair_args a = pair_args(e);
     // Surrogate radii: descriptors + MLP for the systems still in surrogate
@@ -667,7 +667,8 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         ProfScope p(e, OG_K_DESCREEN);
         og_pair_args ad = a;
         ad.skip_sys = (sur && !check) ? sur->mode : nullptr;
-        og_launch_pair_descreen(ad, mode_a, e->stream);
+        if (fuse) og_launch_descreen_bonded(ad, b, mode_a, e->stream);
+        else og_launch_pair_descreen(ad, mode_a, e->stream);
     }
     {
         ProfScope p(e, OG_K_RADII);

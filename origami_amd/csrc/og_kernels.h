@@ -137,6 +137,9 @@ void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, b
                              hipStream_t s);
 void og_launch_bonded(const og_bonded_args& a, bool energy, hipStream_t s);
 #define OG_GATHER_LDS_MAX_ATOMS 20000   // batches below this size gather through LDS
+// the descreening pass and the bonded terms (forces) in one launch; batches
+// below OG_GATHER_LDS_MAX_ATOMS
+void og_launch_descreen_bonded(const og_pair_args& a, const og_bonded_args& b, bool mode_a, hipStream_t s);
 // the bonded terms alone (forces), the gather left to og_launch_radii_gather
 void og_launch_bonded_terms(const og_bonded_args& a, hipStream_t s);
 // Born radii (forces only, no surrogate) and the bonded gather's LDS form

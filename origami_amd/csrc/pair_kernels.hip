@@ -25,6 +25,7 @@
 // cutoffs (10 Å LJ + reaction-field Coulomb, 10 Å GB force, 20 Å descreen).
 #include "og_device.h"
 #include "og_kernels.h"
+#include "bonded_core.h"
 
 namespace {
 
@@ -165,14 +166,15 @@ __device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const L
     acc_j += gj;
 }
 
+// blk / nblk: this block among the nblk blocks sharing the tile pairs.
 template <bool MODEA>
-__device__ __forceinline__ void descreen_body(const og_pair_args& a) {
+__device__ __forceinline__ void descreen_body(const og_pair_args& a, int blk, int nblk) {
     __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
     float4* P = s_pq[threadIdx.x >> 6];
     float4* Q = P + (OG_TILE + 1);
     const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    const int wave = (blk * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (nblk * blockDim.x) >> 6;
     for (int w = wave; w < a.n_work; w += n_waves) {
         const og_work it = a.work[w];
         if (a.skip_sys && a.skip_sys[it.sys]) continue;     // radii from the surrogate
@@ -204,8 +206,21 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a) {
 
 }  // namespace
 
-__global__ void __launch_bounds__(256) og_k_descreen_b(og_pair_args a) { descreen_body<false>(a); }
-__global__ void __launch_bounds__(256) og_k_descreen_a(og_pair_args a) { descreen_body<true>(a); }
+__global__ void __launch_bounds__(256) og_k_descreen_b(og_pair_args a) {
+    descreen_body<false>(a, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(256) og_k_descreen_a(og_pair_args a) {
+    descreen_body<true>(a, blockIdx.x, gridDim.x);
+}
+
+// The descreening pass and the bonded terms (forces) in one launch, for small
+// batches whose force evaluation is a chain of latency-bound launches: blocks
+// [0, nd) take the tile pairs, the rest the bonded terms, side by side.
+template <bool MODEA>
+__global__ void __launch_bounds__(256) og_k_descreen_bonded(og_pair_args a, og_bonded_args b, int nd) {
+    if ((int)blockIdx.x < nd) descreen_body<MODEA>(a, blockIdx.x, nd);
+    else bonded_body<false>(b, blockIdx.x - nd, gridDim.x - nd);
+}
 
 namespace {
 
@@ -436,6 +451,16 @@ void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s) 
     const int g = a.grid > 0 ? a.grid : pair_grid(a.n_work);
     if (mode_a) hipLaunchKernelGGL(og_k_descreen_a, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_descreen_b, dim3(g), dim3(256), 0, s, a);
+}
+
+void og_launch_descreen_bonded(const og_pair_args& a, const og_bonded_args& b, bool mode_a, hipStream_t s) {
+    const int nd = a.n_work > 0 ? (a.grid > 0 ? a.grid : pair_grid(a.n_work)) : 0;
+    const int total = b.nb + b.na + b.nd + b.ni;
+    int nb = (total + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    if (nd + nb <= 0) return;
+    if (mode_a) hipLaunchKernelGGL(og_k_descreen_bonded<true>, dim3(nd + nb), dim3(256), 0, s, a, b, nd);
+    else hipLaunchKernelGGL(og_k_descreen_bonded<false>, dim3(nd + nb), dim3(256), 0, s, a, b, nd);
 }
 
 void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
