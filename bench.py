@@ -262,8 +262,8 @@ def main():
                     help="steps of the separate profiled run that gives the kernel breakdown")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(chains and replicas would shard across GPUs as described in DESIGN.md)"
+        print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
+              "(whole chains and replicas would go to one engine per GPU)"
               % args.gpus, file=sys.stderr)
         return 2
 

@@ -3,7 +3,7 @@
 //
 // There is no counterpart in origami: the surrogate stands in for the
 // descreening pass of compute_born_radii_into_scratch (energy/src/gb.rs:203-342)
-// and is trained on the radii that pass produces.  Plain GB stays the judge:
+// and is trained on the radii that pass produces.  Plain GB stays the arbiter:
 // systems whose surrogate radii drift more than a tolerance from the plain
 // radii at a periodic check fall back to plain GB (engine.cpp).
 //

@@ -1,5 +1,5 @@
 """GPU: the learned Born-radius surrogate against its CPU oracle and against
-plain GB (the judge), through the C ABI.
+plain GB (the arbiter), through the C ABI.
 
 Tolerances: descriptors are exact (integer counts, exact bin edges);
 forward radii within 1e-3 relative and gradients within 2e-2 relative L2 of
