@@ -332,7 +332,7 @@ def main():
     dom = max(totals, key=totals.get)
     flops = {"descreen": FLOPS_DESCREEN, "force": FLOPS_FORCE}
     ach = flops["force"] * pairs / (k_ms["force"] * 1e-3) / 1e12
-    roof = {"kernel": "og_k_force_b (all-pairs LJ + Coulomb + GB pair force)", "bound": "mfma",
+    roof = {"kernel": "og_k_force_b (all-pairs LJ + Coulomb + GB pair force)", "bound": "valu",
             "roof": "FP32 vector ALU, 157.3 TFLOP/s (MI355X FP32 peak)", "achieved": ach,
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
             "traffic": None, "flops_per_pair": FLOPS_FORCE, "pairs_per_launch": pairs,

@@ -28,13 +28,20 @@ enum og_status {
     OG_ERR_ARG = -1,        /* invalid argument / size */
     OG_ERR_HIP = -2,        /* HIP runtime failure */
     OG_ERR_STATE = -3,      /* call sequence (e.g. run before setup) */
-    OG_ERR_NONFINITE = -4,  /* a system diverged (non-finite velocities/forces) */
+    OG_ERR_NONFINITE = -4,  /* a system diverged or was stopped (see og_get_status) */
 };
 
+/* Per-system status bits (og_get_status's diverged[], og_run's stop reasons). */
+#define OG_STATUS_NONFINITE 1u  /* non-finite velocities after a step (langevin.rs:316-319) */
+#define OG_STATUS_RANGE 2u      /* a force/energy contribution NaN or >= 2^26 kJ/mol(/Å): outside
+                                   the int64 fixed-point accumulators' range */
+#define OG_STATUS_SASA 4u       /* a SASA atom exceeded the arrangement's capacity
+                                   (128 caps, 256 arcs, 256 vertices) */
+
 /* Force flags. */
-#define OG_MODE_A 1u        /* origami's integrator cutoffs: LJ + reaction-field Coulomb
-                               and GB pair force at 10 Å, descreening at 20 Å.  Without
-                               it: all pairs, no cutoff (bare Coulomb). */
+#define OG_MODE_A 1u        /* origami's integrator cutoffs: LJ + reaction-field Coulomb at
+                               cutoff_a, GB pair force at 10 Å, descreening at 20 Å.
+                               Without it: all pairs, no cutoff (bare Coulomb). */
 #define OG_SASA 2u          /* add analytical (PowerSasa) hydrophobic SASA forces */
 #define OG_ENERGY 4u        /* also accumulate per-system energy terms */
 #define OG_SURROGATE 8u     /* Born radii from the learned surrogate (og_surrogate_*) */
@@ -92,9 +99,16 @@ int og_get_forces(og_engine* e, double* f);
  * overwrites f_out (if not NULL) with bonded + LJ + Coulomb + GB (+ SASA)
  * forces at xyz (if NULL, at the device-resident positions).  energies
  * (if not NULL, needs OG_ENERGY) receives n_systems x OG_N_ENERGY terms.
- * sasa_gamma_scale replaces the ORIGAMI_SASA_GAMMA_SCALE environment
- * variable (energy/src/powersasa/analytical.rs:477-480). */
-int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+ * cutoff_a is origami's cutoff_a argument (forces.rs:82): the LJ and
+ * reaction-field Coulomb cutoff under OG_MODE_A (finite, > 0; origami's
+ * callers pass DEFAULT_CUTOFF_A = 10); without OG_MODE_A every pair counts
+ * and cutoff_a is not used.  sasa_gamma_scale replaces the
+ * ORIGAMI_SASA_GAMMA_SCALE environment variable
+ * (energy/src/powersasa/analytical.rs:477-480).  Returns OG_ERR_NONFINITE
+ * (energies of the system set to NaN) when a system's contributions leave the
+ * fixed-point range (OG_STATUS_RANGE), OG_ERR_ARG when a SASA atom exceeds
+ * the arrangement's capacity. */
+int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags, double cutoff_a,
               double sasa_gamma_scale, double* energies);
 
 /* Replaces total_energy_with_options (dynamics/src/energy_eval.rs:20-34) and
@@ -102,7 +116,7 @@ int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
  * terms; GB pair energy is uncut in both modes, as in gb_energy.  Computes
  * no forces: the engine's force array (the forces og_run's next step starts
  * from) is left as the last og_forces / og_run step made it. */
-int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies);
+int og_energies(og_engine* e, const double* xyz, uint32_t flags, double cutoff_a, double* energies);
 
 /* Effective Born radii and descreening integrals after the last force call
  * (gb.rs:203-342); either pointer may be NULL. */
@@ -115,6 +129,8 @@ typedef struct og_dynamics_params {
     int32_t constrain_h_bonds;      /* SHAKE X-H (tol 1e-6 Å^2, 64 iterations) */
     uint32_t flags;                 /* force flags (OG_MODE_A, OG_SASA, OG_SURROGATE) */
     double sasa_gamma_scale;
+    double cutoff_a;                /* LJ / Coulomb cutoff under OG_MODE_A (langevin.rs passes
+                                       DEFAULT_CUTOFF_A = 10 Å); not used without it */
 } og_dynamics_params;
 
 /* Replaces run_langevin's set-up (dynamics/src/langevin.rs:138-199):
@@ -125,8 +141,11 @@ int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, const double* t
 
 /* n_steps BAOAB steps for every system (langevin.rs:223-342 /
  * remd.rs:286-405).  ke_per_step (optional, n_steps x n_systems) receives
- * each step's kinetic energy in kJ/mol.  Returns OG_ERR_NONFINITE if any
- * system diverged; diverged systems stop moving. */
+ * each step's kinetic energy in kJ/mol.  A system stops at the step where
+ * its velocities turn non-finite (remd.rs:382-389), a force leaves the
+ * fixed-point range or a SASA atom exceeds capacity; the others run all
+ * n_steps.  Returns OG_ERR_NONFINITE if any system has stopped;
+ * og_get_status gives each system's OG_STATUS_* bits. */
 int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 
 /* og_run replays one captured hipGraph per BAOAB step (all systems, every
@@ -161,11 +180,13 @@ int og_set_launch(og_engine* e, int32_t pair_blocks, int32_t sasa_blocks, int32_
 int og_set_temperature(og_engine* e, int32_t system, double temperature_k);
 
 /* REMD exchange (remd.rs:447-466): swap positions, velocities and forces of
- * systems a and b (same topology) and rescale velocities by sqrt(T_a/T_b)
- * and sqrt(T_b/T_a). */
+ * systems a and b and rescale velocities by sqrt(T_a/T_b) and sqrt(T_b/T_a).
+ * OG_ERR_ARG unless the two systems have identical parameters and term
+ * lists (checked against a hash taken at og_engine_create). */
 int og_swap_systems(og_engine* e, int32_t a, int32_t b);
 
-/* Per-system diverged flags, SHAKE failure counts, Born-radius clamp counts. */
+/* Per-system status (OG_STATUS_* bits; 0 = running), SHAKE failure counts,
+ * Born-radius clamp counts. */
 int og_get_status(og_engine* e, int32_t* diverged, int32_t* shake_failures, int32_t* clamped);
 
 /* Kernel timing: when enabled, every launch is bracketed by HIP events on

@@ -90,7 +90,7 @@ __device__ __forceinline__ void bonded_term(const og_bonded_args& a, int k, int&
             const double dr = r - r0;
             e_slot = OG_E_BOND;
             e_sys = a.atom_sys[ia];
-            e_fix = og_to_fix(K * dr * dr * OG_KCAL_TO_KJ);
+            e_fix = og_to_fix_chk(K * dr * dr * OG_KCAL_TO_KJ, a.sflag ? a.sflag + e_sys : nullptr);
         }
         return;
     }
@@ -117,7 +117,7 @@ __device__ __forceinline__ void bonded_term(const og_bonded_args& a, int k, int&
                 const double dt = acos(fmin(fmax(dot(uh, vh), -1.0), 1.0)) - th0;
                 e_slot = OG_E_ANGLE;
                 e_sys = a.atom_sys[ia];
-                e_fix = og_to_fix(K * dt * dt * OG_KCAL_TO_KJ);
+                e_fix = og_to_fix_chk(K * dt * dt * OG_KCAL_TO_KJ, a.sflag ? a.sflag + e_sys : nullptr);
             }
         }
         put(a.tf, s0, fa);
@@ -166,7 +166,7 @@ __device__ __forceinline__ void bonded_term(const og_bonded_args& a, int k, int&
         }
         e_slot = slot;
         e_sys = a.atom_sys[q[0]];
-        e_fix = og_to_fix(e * OG_KCAL_TO_KJ);
+        e_fix = og_to_fix_chk(e * OG_KCAL_TO_KJ, a.sflag ? a.sflag + e_sys : nullptr);
     }
 }
 

@@ -89,6 +89,8 @@ struct og_engine {
     double* cn_len = nullptr;
     double *sys_temp = nullptr, *sys_dofc = nullptr;
     int *shake_fail = nullptr, *diverged = nullptr, *clamped = nullptr;
+    int* sflag = nullptr;                  // [S] status bits of the current evaluation / run (OG_ST_*)
+    std::vector<uint64_t> topo_hash;       // per system: parameters and term lists (swap check)
     og_sasa_data sasa{};
     og_sur* sur = nullptr;                 // learned Born-radius surrogate (optional)
     uint32_t* step_dev = nullptr;          // BAOAB step counter (Philox counter word)
@@ -168,7 +170,8 @@ static void free_all(og_engine* e) {
                     e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
                     e->bonded_tf, e->bonded_csr_start, e->bonded_csr_slot,
                     e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start,
-                    e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped};
+                    e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped,
+                    e->sflag};
     for (void* p : ptrs)
         if (p) hipFree(p);
     for (auto& g : e->gexec)
@@ -260,6 +263,39 @@ extern "C" void og_engine_destroy(og_engine* e) {
     delete e;
 }
 
+// FNV-1a over everything that defines a system's energy function except the
+// positions: per-atom parameters, exclusion and 1-4 lists, bonded terms with
+// their parameters and the constraints.  Two systems may exchange
+// configurations (REMD) only when these agree.
+static uint64_t topology_hash(const og_system_desc& d) {
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t k = 0; k < bytes; ++k) {
+            h ^= c[k];
+            h *= 1099511628211ull;
+        }
+    };
+    const size_t n = (size_t)d.n_atoms;
+    mix(&d.n_atoms, sizeof d.n_atoms);
+    const double* per_atom[] = {d.charge, d.rmin_half, d.epsilon, d.rmin_half_14, d.epsilon_14, d.gb_rho,
+                                d.gb_scale, d.sasa_radius, d.sasa_gamma, d.mass};
+    for (const double* p : per_atom) mix(p, n * sizeof(double));
+    auto lst = [&](int32_t cnt, const int32_t* idx, int w, const double* par, int pw) {
+        mix(&cnt, sizeof cnt);
+        if (cnt > 0 && idx) mix(idx, (size_t)cnt * w * sizeof(int32_t));
+        if (cnt > 0 && par) mix(par, (size_t)cnt * pw * sizeof(double));
+    };
+    lst(d.n_excl, d.excl, 2, nullptr, 0);
+    lst(d.n_14, d.pair14, 2, nullptr, 0);
+    lst(d.n_bonds, d.bonds, 2, d.bond_params, 2);
+    lst(d.n_angles, d.angles, 3, d.angle_params, 2);
+    lst(d.n_dihedrals, d.dihedrals, 4, d.dihedral_params, 3);
+    lst(d.n_impropers, d.impropers, 4, d.improper_params, 2);
+    lst(d.n_constraints, d.constraints, 2, d.constraint_len, 1);
+    return h;
+}
+
 static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&e->side, hipStreamNonBlocking));
@@ -288,12 +324,14 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<int> cl_start{0}, cl_atoms, cn_start{0};
     std::vector<double> cn_len;
     e->n_cons_sys.resize(n_sys);
+    e->topo_hash.assign(n_sys, 0);
 
     const double sqrt_lj = std::sqrt(12.0 * OG_KCAL_TO_KJ);
     const double sqrt_kc = std::sqrt(OG_COULOMB_KCAL * OG_KCAL_TO_KJ);
     for (int s = 0; s < n_sys; ++s) {
         const og_system_desc& d = sys[s];
         const int n = d.n_atoms, off = e->atom_off[s];
+        e->topo_hash[s] = topology_hash(d);
         const int t0 = (int)tile_atom0.size();
         const int nt = (n + OG_TILE - 1) / OG_TILE;
         e->n_pairs += (long long)n * (n - 1) / 2;
@@ -339,6 +377,11 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
         auto tile_pair_index = [&](int I, int J) {  // I <= J local
             return w0 + I * nt - I * (I - 1) / 2 + (J - I);
         };
+        // Lane l of a masked tile pair holds i atom l; bit m of its word marks
+        // j atom m.  On a diagonal tile (I == J) the force kernel evaluates a
+        // pair a < b on lane a when b - a <= 32 and on lane b (with j = a)
+        // when b - a > 32, so both lanes carry the bit; the FP64 energy kernel
+        // evaluates every pair on its lower lane and never reads the mirror.
         auto set_bit = [&](int a, int b, int which) {
             const int I = a / OG_TILE, J = b / OG_TILE;
             const int wi = tile_pair_index(I, J);
@@ -346,7 +389,9 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
                 work[wi].mask = (int)(masks.size() / 128);
                 masks.resize(masks.size() + 128, 0ull);
             }
-            masks[(size_t)work[wi].mask * 128 + which * 64 + (a % OG_TILE)] |= 1ull << (b % OG_TILE);
+            unsigned long long* m = &masks[(size_t)work[wi].mask * 128 + which * 64];
+            m[a % OG_TILE] |= 1ull << (b % OG_TILE);
+            if (I == J) m[b % OG_TILE] |= 1ull << (a % OG_TILE);
         };
         for (int k = 0; k < d.n_excl; ++k) set_bit(d.excl[2 * k], d.excl[2 * k + 1], 0);
         for (int k = 0; k < d.n_14; ++k) set_bit(d.pair14[2 * k], d.pair14[2 * k + 1], 1);
@@ -528,6 +573,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<int> zi(n_sys, 0);
     HIPCHK(upload(&e->shake_fail, zi));
     HIPCHK(upload(&e->diverged, zi));
+    HIPCHK(upload(&e->sflag, zi));
     HIPCHK(hipMalloc((void**)&e->step_dev, sizeof(uint32_t)));
     HIPCHK(hipMemset(e->step_dev, 0, sizeof(uint32_t)));
     HIPCHK(hipMalloc((void**)&e->noise, 6 * (size_t)std::max(N, 1) * sizeof(double)));
@@ -578,21 +624,37 @@ extern "C" int og_get_forces(og_engine* e, double* f) { return copy_state(e, e ?
 // ---------------------------------------------------------------------------
 // Force pipeline (accumulators must be zero on entry).
 // ---------------------------------------------------------------------------
-static og_pair_args pair_args(og_engine* e) {
+// origami's integrator cutoffs under OG_MODE_A: the caller's cutoff_a for LJ
+// and reaction-field Coulomb (forces_nonbonded.rs:125, nonbonded.rs:41-84),
+// 10 Å for the GB pair force (GB_DEFAULT_CUTOFF_A_PUB, forces_gb.rs:33-36,
+// forces.rs:115-120) and 20 Å for the descreening sum (gb.rs:37-40).
+#define OG_GB_CUTOFF_A 10.0
+#define OG_DESCREEN_CUTOFF_A 20.0
+
+static og_pair_args pair_args(og_engine* e, double cutoff_a) {
     og_pair_args a{};
     a.pos = e->pos; a.par = e->par; a.gb = e->gb; a.qs = e->qs;
     a.tile_atom0 = e->tile_atom0; a.tile_cnt = e->tile_cnt;
     a.work = e->work; a.n_work = e->n_work; a.masks = e->masks;
     a.facc = e->facc; a.iacc = e->iacc; a.eacc = e->eacc;
     a.grid = e->pair_grid;
-    const double rc = 10.0;
+    a.sflag = e->sflag;
+    const double rc = cutoff_a;
     a.rc2_nb = (float)(rc * rc);
-    a.rc2_gb = (float)(rc * rc);
-    a.rc2_ds = (float)(20.0 * 20.0);
+    a.rc2_gb = (float)(OG_GB_CUTOFF_A * OG_GB_CUTOFF_A);
+    a.rc2_ds = (float)(OG_DESCREEN_CUTOFF_A * OG_DESCREEN_CUTOFF_A);
     a.inv_rc3 = (float)(1.0 / (rc * rc * rc));
     a.kappa = (float)(1.0 / (2.0 * rc * rc * rc));
     a.crf = (float)(3.0 / (2.0 * rc));
     return a;
+}
+
+// The LJ / Coulomb cutoff of a call: cutoff_a under OG_MODE_A (finite and
+// positive), none otherwise (the value is then not used).
+static int check_cutoff(uint32_t flags, double cutoff_a) {
+    if ((flags & OG_MODE_A) && !(cutoff_a > 0.0 && std::isfinite(cutoff_a)))
+        return fail(OG_ERR_ARG, "cutoff_a must be finite and > 0 under OG_MODE_A (origami uses 10 Å)");
+    return OG_OK;
 }
 
 static void invalidate_graphs(og_engine* e) {
@@ -610,9 +672,10 @@ static void invalidate_graphs(og_engine* e) {
 // formed one thread per atom by the bonded gather kernel, so that the cluster
 // step only reads them.  They stay on the engine stream: a second branch in a
 // captured step costs more than the work.
-static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool energy, int sur_check = -1,
-                         bool forces = true, const og_noise_args* next_noise = nullptr) {
+static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double sasa_scale, bool energy,
+                         int sur_check = -1, bool forces = true, const og_noise_args* next_noise = nullptr) {
     const bool mode_a = flags & OG_MODE_A;
+    if (!mode_a) cutoff_a = OG_GB_CUTOFF_A;         // not used by the all-pairs kernels
     if (!forces) sasa_scale = 0.0;
     // SASA on the side stream, overlapping everything up to the join below.
     // Both branches add into the force and energy accumulators with integer
@@ -623,7 +686,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         HIPCHK(hipEventRecord(e->fork_ev, e->stream));
         HIPCHK(hipStreamWaitEvent(e->side, e->fork_ev, 0));
         const int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->side,
-                                      energy ? e->eacc : nullptr, e->atom_sys);
+                                      energy ? e->eacc : nullptr, e->atom_sys, e->sflag);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
         HIPCHK(hipEventRecord(e->join_ev, e->side));
     }
@@ -641,10 +704,11 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         b.dih = e->dih; b.dih_p = e->dih_p; b.imp = e->imp; b.imp_p = e->imp_p;
         b.tf = e->bonded_tf; b.csr_start = e->bonded_csr_start; b.csr_slot = e->bonded_csr_slot;
         b.n_atoms = e->n_atoms;
+        b.sflag = e->sflag;
         if (next_noise) b.noise = *next_noise;
         if (!fuse) og_launch_bonded(b, energy, e->stream);
     }
-    og_pair_args a = pair_args(e);
+    og_pair_args a = pair_args(e, cutoff_a);
     // Surrogate radii: descriptors + MLP for the systems still in surrogate
     // mode; their tile pairs skip the descreening pass.  Every check_every
     // calls the plain pass runs for every system too and systems whose radii
@@ -676,6 +740,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         r.iacc = e->iacc; r.atom_sys = e->atom_sys; r.atom_slot = e->atom_slot; r.rho = e->rho;
         r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
         r.eacc = e->eacc; r.n_atoms = e->n_atoms;
+        r.sflag = e->sflag;
         if (sur) { r.sur_mode = sur->mode; r.r_sur = sur->r_sur; r.dev = sur->dev; r.check = check ? 1 : 0; }
         if (fuse) og_launch_radii_gather(r, b, e->stream);
         else og_launch_radii(r, energy, e->stream);
@@ -693,14 +758,16 @@ static int launch_forces(og_engine* e, uint32_t flags, double sasa_scale, bool e
         og_energy64_args en{};
         en.charge = e->charge; en.eps = e->eps; en.rmin = e->rmin; en.eps14 = e->eps14;
         en.rmin14 = e->rmin14; en.radii = e->radii;
-        en.rc2_nb = 100.0; en.kappa = 1.0 / 2000.0; en.crf = 3.0 / 20.0;
+        en.rc2_nb = cutoff_a * cutoff_a;
+        en.kappa = 1.0 / (2.0 * cutoff_a * cutoff_a * cutoff_a);
+        en.crf = 3.0 / (2.0 * cutoff_a);
         og_launch_pair_energy64(a, en, mode_a, e->stream);
     }
     if (fork) HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
     if (!fork && (flags & OG_SASA)) {
         ProfScope p(e, OG_K_SASA);
         int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream,
-                                energy ? e->eacc : nullptr, e->atom_sys);
+                                energy ? e->eacc : nullptr, e->atom_sys, e->sflag);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
     }
     HIPCHK(hipGetLastError());
@@ -716,6 +783,12 @@ static int check_sasa_overflow(og_engine* e, uint32_t flags) {
     return OG_OK;
 }
 
+static const char* status_reason(int bits) {
+    if (bits & OG_ST_SASA) return "a SASA atom exceeded the arrangement's capacity (128 caps / 256 arcs / 256 vertices)";
+    if (bits & OG_ST_RANGE) return "a force or energy contribution was NaN or beyond the fixed-point range (|x| >= 2^26)";
+    return "non-finite velocities";
+}
+
 static int read_energies(og_engine* e, double* out) {
     std::vector<long long> acc((size_t)e->n_sys * OG_NE);
     HIPCHK(hipMemcpyAsync(acc.data(), e->eacc, acc.size() * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
@@ -726,18 +799,20 @@ static int read_energies(og_engine* e, double* out) {
 
 // forces = false: energies only; the engine's force array keeps the forces of
 // the last force evaluation (the next og_run step starts from them).
-static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t flags, double cutoff_a,
                     double sasa_gamma_scale, double* energies, bool forces) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     if ((flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
         return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     int rc;
+    if ((rc = check_cutoff(flags, cutoff_a))) return rc;
     if (xyz && (rc = og_set_positions(e, xyz))) return rc;
     const bool energy = (flags & OG_ENERGY) || energies;
     if (energy) HIPCHK(hipMemsetAsync(e->eacc, 0, (size_t)e->n_sys * OG_NE * sizeof(long long), e->stream));
     HIPCHK(hipMemsetAsync(e->clamped, 0, (size_t)e->n_sys * sizeof(int), e->stream));
+    HIPCHK(hipMemsetAsync(e->sflag, 0, (size_t)e->n_sys * sizeof(int), e->stream));
     og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
-    if ((rc = launch_forces(e, flags, sasa_gamma_scale, energy, -1, forces))) return rc;
+    if ((rc = launch_forces(e, flags, cutoff_a, sasa_gamma_scale, energy, -1, forces))) return rc;
     if (forces) {
         ProfScope p(e, OG_K_BACK);
         og_back_args b{};
@@ -752,10 +827,17 @@ static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t fla
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) prof_collect(e);
-    if ((rc = check_sasa_overflow(e, flags))) return rc;
+    std::vector<int> st(e->n_sys);
+    HIPCHK(hipMemcpy(st.data(), e->sflag, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
     if (energies) {
         if ((rc = read_energies(e, energies))) return rc;
+        for (int s = 0; s < e->n_sys; ++s)     // a flagged system's sums are not its energies
+            if (st[s])
+                for (int k = 0; k < OG_NE; ++k) energies[(size_t)s * OG_NE + k] = NAN;
     }
+    if ((rc = check_sasa_overflow(e, flags))) return rc;
+    for (int s = 0; s < e->n_sys; ++s)
+        if (st[s]) return fail(OG_ERR_NONFINITE, "system %d: %s", s, status_reason(st[s]));
     if (f_out) {
         for (size_t k = 0; k < 3 * (size_t)e->n_atoms; ++k)
             if (!std::isfinite(f_out[k])) return fail(OG_ERR_NONFINITE, "non-finite force at component %zu", k);
@@ -763,14 +845,14 @@ static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t fla
     return OG_OK;
 }
 
-extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags,
+extern "C" int og_forces(og_engine* e, const double* xyz, double* f_out, uint32_t flags, double cutoff_a,
                          double sasa_gamma_scale, double* energies) {
-    return evaluate(e, xyz, f_out, flags, sasa_gamma_scale, energies, true);
+    return evaluate(e, xyz, f_out, flags, cutoff_a, sasa_gamma_scale, energies, true);
 }
 
-extern "C" int og_energies(og_engine* e, const double* xyz, uint32_t flags, double* energies) {
+extern "C" int og_energies(og_engine* e, const double* xyz, uint32_t flags, double cutoff_a, double* energies) {
     if (!energies) return fail(OG_ERR_ARG, "energies is NULL");
-    return evaluate(e, xyz, nullptr, flags | OG_ENERGY, 0.0, energies, false);
+    return evaluate(e, xyz, nullptr, flags | OG_ENERGY, cutoff_a, 0.0, energies, false);
 }
 
 extern "C" int og_born_radii(og_engine* e, double* radii, double* integral) {
@@ -794,6 +876,7 @@ extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, cons
     if (!e || !p || !temps) return fail(OG_ERR_ARG, "NULL argument");
     if (!(p->dt_fs > 0.0) || !(p->friction_ps_inv >= 0.0))
         return fail(OG_ERR_ARG, "dt must be > 0 and friction >= 0");
+    if (int rc = check_cutoff(p->flags, p->cutoff_a)) return rc;
     if ((p->flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
         return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     for (int s = 0; s < e->n_sys; ++s)
@@ -814,7 +897,7 @@ extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, cons
     e->dyn_ready = true;
     invalidate_graphs(e);
     // initial forces at the current positions
-    return og_forces(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->sasa_gamma_scale, nullptr);
+    return og_forces(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->cutoff_a, p->sasa_gamma_scale, nullptr);
 }
 
 extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
@@ -832,7 +915,8 @@ static int record_step(og_engine* e, const og_cluster_args& c, const og_back_arg
         ProfScope ps(e, OG_K_CLUSTER);
         og_launch_cluster_step(c, e->stream);
     }
-    int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.sasa_gamma_scale, false, sur_check, true, next_noise);
+    int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.cutoff_a, p.sasa_gamma_scale, false, sur_check, true,
+                           next_noise);
     if (rc) return rc;
     {
         ProfScope ps(e, OG_K_BACK);
@@ -884,8 +968,12 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = p.dt_fs; b.do_kick = 1;
     b.diverged = e->diverged;
     b.step_dev = e->step_dev;
+    b.sflag = e->sflag;
     // the device step counter restarts from the host's count each call
     og_launch_set_u32(e->step_dev, e->step, e->stream);
+    // status bits of this call's force evaluations (a system already stopped
+    // keeps its diverged bits)
+    HIPCHK(hipMemsetAsync(e->sflag, 0, (size_t)e->n_sys * sizeof(int), e->stream));
     // Without SASA the O-step normals come from a buffer: the first step's
     // here, each later step's during the force evaluation before it.  With
     // SASA the cluster step forms its own; the extra launch would share the
@@ -953,11 +1041,20 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     }
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) prof_collect(e);
-    if (int rc = check_sasa_overflow(e, p.flags)) return rc;
+    int ovf = 0;
+    if ((p.flags & OG_SASA) && og_sasa_overflow(&e->sasa, &ovf) != 0)
+        return fail(OG_ERR_HIP, "reading SASA status failed");
     std::vector<int> div(e->n_sys);
     HIPCHK(hipMemcpy(div.data(), e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
+    int n_div = 0, first = -1;
     for (int s = 0; s < e->n_sys; ++s)
-        if (div[s]) return fail(OG_ERR_NONFINITE, "system %d diverged (non-finite velocities)", s);
+        if (div[s]) {
+            n_div++;
+            if (first < 0) first = s;
+        }
+    if (n_div)
+        return fail(OG_ERR_NONFINITE, "%d system(s) stopped; system %d: %s (the others ran all steps)", n_div,
+                    first, status_reason(div[first]));
     return OG_OK;
 }
 
@@ -1013,7 +1110,7 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     float best_pair = 1e30f;
     for (int per_cu : {2, 4, 8, 16, 32}) {
         const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
-        og_pair_args a = pair_args(e);
+        og_pair_args a = pair_args(e, 10.0);
         a.grid = g;
         float ms = 0.0f;
         if (int rc = timed([&] { og_launch_pair_force(a, mode_a, e->stream); }, ms)) return rc;
@@ -1030,7 +1127,7 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
         const uint32_t fl = (flags & (OG_MODE_A | OG_SURROGATE)) | OG_SASA;
         int lrc = OG_OK;
         // with the surrogate: its evaluations without the plain-GB check
-        auto evaluation = [&] { if (!lrc) lrc = launch_forces(e, fl, 1.0, false, 0); };
+        auto evaluation = [&] { if (!lrc) lrc = launch_forces(e, fl, 10.0, 1.0, false, 0); };
         best_s = 1e30f;
         for (int per_cu : {2, 3, 4, 5, 6, 8}) {
             const int g = std::min(per_cu * n_cu, og_sasa_max_blocks(&e->sasa));
@@ -1067,6 +1164,8 @@ extern "C" int og_swap_systems(og_engine* e, int32_t a, int32_t b) {
     if (!e || a < 0 || b < 0 || a >= e->n_sys || b >= e->n_sys || a == b)
         return fail(OG_ERR_ARG, "bad system indices");
     if (e->sys_n[a] != e->sys_n[b]) return fail(OG_ERR_ARG, "systems %d and %d differ in size", a, b);
+    if (e->topo_hash[a] != e->topo_hash[b])
+        return fail(OG_ERR_ARG, "systems %d and %d differ in parameters or topology", a, b);
     if (!(e->temps[a] > 0.0) || !(e->temps[b] > 0.0)) return fail(OG_ERR_STATE, "temperatures not set");
     const double sa = std::sqrt(e->temps[a] / e->temps[b]);
     const double sb = std::sqrt(e->temps[b] / e->temps[a]);
@@ -1142,12 +1241,13 @@ extern "C" int og_surrogate_setup(og_engine* e, int64_t capacity_rows, uint64_t 
 // plain Born radii (descreening pass + radii) for every system at the current positions
 static int plain_radii(og_engine* e, uint32_t flags) {
     og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
-    og_pair_args a = pair_args(e);
+    og_pair_args a = pair_args(e, 10.0);
     og_launch_pair_descreen(a, (flags & OG_MODE_A) != 0, e->stream);
     og_radii_args r{};
     r.iacc = e->iacc; r.atom_sys = e->atom_sys; r.atom_slot = e->atom_slot; r.rho = e->rho;
     r.charge = e->charge; r.gb = e->gb; r.radii = e->radii; r.clamped = e->clamped;
     r.eacc = e->eacc; r.n_atoms = e->n_atoms;
+    r.sflag = e->sflag;
     og_launch_radii(r, false, e->stream);
     HIPCHK(hipGetLastError());
     return OG_OK;

@@ -100,9 +100,10 @@ __device__ __forceinline__ void gather_lds_body(const og_bonded_args& a, int blk
     if (NOISE && t >= OG_GATHER_ATOMS && t < 2 * OG_GATHER_ATOMS && a0 + t - OG_GATHER_ATOMS < a1)
         noise_atom(a.noise, a0 + t - OG_GATHER_ATOMS);
     if (own) {
-        og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix(fx));
-        og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix(fy));
-        og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
+        int* fl = a.sflag ? a.sflag + a.atom_sys[i] : nullptr;
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix_chk(fx, fl));
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix_chk(fy, fl));
+        og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix_chk(fz, fl));
     }
 }
 template <bool NOISE>
@@ -146,9 +147,10 @@ __global__ void __launch_bounds__(256) og_k_bonded_gather(og_bonded_args a) {
             }
         }
     }
-    og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix(fx));
-    og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix(fy));
-    og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix(fz));
+    int* fl = a.sflag ? a.sflag + a.atom_sys[i] : nullptr;
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 0, og_to_fix_chk(fx, fl));
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 1, og_to_fix_chk(fy, fl));
+    og_atomic_add_fix(a.facc + 3 * (size_t)i + 2, og_to_fix_chk(fz, fl));
 }
 
 static int bonded_grid(int total) {
@@ -226,7 +228,7 @@ __device__ __forceinline__ void radii_body(const og_radii_args& a, int i) {
             const double q = a.charge[i];
             if (q != 0.0) {
                 const double pref = -0.5 * (1.0 - 1.0 / OG_EPS_WATER) * OG_COULOMB_KCAL * OG_KCAL_TO_KJ;
-                e_fix = og_to_fix(pref * q * q / R);
+                e_fix = og_to_fix_chk(pref * q * q / R, a.sflag ? a.sflag + sys : nullptr);
                 has_e = true;
             }
         }
@@ -508,6 +510,13 @@ __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
     if (!a.do_kick) return;
     const int sys = a.atom_sys[i];
     if (a.diverged[sys]) return;
+    // a contribution out of range (or a SASA atom over capacity) in this
+    // step's forces: the system stops here, as on non-finite velocities
+    const int st = a.sflag ? a.sflag[sys] : 0;
+    if (st) {
+        atomicOr(a.diverged + sys, st);
+        return;
+    }
     const double m = a.mass[i];
     const double k = OG_ACCEL / m * (0.5 * a.dt);
     double v2 = 0.0;
@@ -518,8 +527,8 @@ __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
         v2 += v * v;
         finite = finite && isfinite(v);
     }
-    if (!finite) atomicMax(a.diverged + sys, 1);
-    if (a.ke_acc) og_atomic_add_fix(a.ke_acc + sys, og_to_fix(0.5 * m * v2 / OG_ACCEL));
+    if (!finite) atomicOr(a.diverged + sys, OG_ST_NONFINITE);
+    if (a.ke_acc) og_atomic_add_fix(a.ke_acc + sys, og_to_fix_chk(0.5 * m * v2 / OG_ACCEL, nullptr));
 }
 
 __global__ void og_k_set_u32(uint32_t* p, uint32_t v) {

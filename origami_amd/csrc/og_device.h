@@ -40,7 +40,30 @@ struct og_work {              // one tile pair (I <= J) of one system
     int32_t part;             // -1: all 64 steps; 0 / 1: steps 0-31 / 32-63 (I < J only)
 };
 
+// Per-system status bits (the engine's diverged flags and the per-evaluation
+// range flags share them).
+#define OG_ST_NONFINITE 1     // non-finite velocities after the closing half-kick
+#define OG_ST_RANGE 2         // a contribution outside the fixed-point range (or NaN)
+#define OG_ST_SASA 4          // a SASA atom exceeded the arrangement's capacity
+
+// Largest magnitude one contribution may have (kJ/mol/Å, kJ/mol or Å^-1):
+// 2^26.  An int64 accumulator at 2^32 units holds 2^31 in all, so 32
+// contributions at the limit still cannot wrap it.
+#define OG_FIX_LIMIT 67108864.0
+
 __device__ __forceinline__ long long og_to_fix(double x) {
+    return __double2ll_rn(x * OG_FIX_SCALE);
+}
+
+// og_to_fix for values that depend on the state: a NaN or a magnitude at or
+// beyond OG_FIX_LIMIT sets OG_ST_RANGE in *flag (when flag is set) and
+// converts as 0 (NaN) or the signed limit, so the integer sums stay defined
+// and the system is reported instead of carrying a wrapped value.
+__device__ __forceinline__ long long og_to_fix_chk(double x, int* flag) {
+    if (!(fabs(x) < OG_FIX_LIMIT)) {
+        if (flag) atomicOr(flag, OG_ST_RANGE);
+        x = (x != x) ? 0.0 : (x > 0.0 ? OG_FIX_LIMIT : -OG_FIX_LIMIT);
+    }
     return __double2ll_rn(x * OG_FIX_SCALE);
 }
 

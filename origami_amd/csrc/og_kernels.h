@@ -23,6 +23,7 @@ struct og_pair_args {
     float inv_rc3, kappa, crf;       // reaction-field constants (Mode A)
     const int* skip_sys;             // [S] descreening skipped where nonzero (may be null)
     int grid;                        // blocks of 256 threads (0: default)
+    int* sflag;                      // [S] status bits (OG_ST_RANGE on a contribution out of range)
 };
 
 // energy slots per system
@@ -63,6 +64,7 @@ struct og_bonded_args {
     const int* csr_slot;                          // [slots]
     int n_atoms;
     og_noise_args noise;                          // if noise.noise is set, formed beside the gather
+    int* sflag;                                   // [S] status bits
 };
 
 struct og_radii_args {
@@ -80,6 +82,7 @@ struct og_radii_args {
     const double* r_sur;             // [N] surrogate radii
     unsigned* dev;                   // [S] max |R_sur - R_plain| / R_plain (float bits), when check
     int check;                       // plain radii were computed for every system this call
+    int* sflag;                      // [S] status bits
 };
 
 #define OG_MAX_CL 8                  // atoms per SHAKE cluster (X + up to 7 H)
@@ -121,6 +124,8 @@ struct og_back_args {
     long long* ke_acc;               // [S] or null
     int* diverged;                   // [S]
     uint32_t* step_dev;              // if set (and do_kick), advanced by one per step
+    int* sflag;                      // [S] status bits of this evaluation: with do_kick, a flagged
+                                     //   system gets them in diverged and is not kicked
 };
 
 void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s);

@@ -52,7 +52,10 @@ struct og_sasa_data {
     int* padj_start = nullptr;    // [OG_SASA_PROBES + 1] probes within 2.5 spacings
     int* padj = nullptr;
     long long* eacc = nullptr;    // engine energy accumulators (set per launch)
-    const int* atom_sys = nullptr;
+    const int* atom_sys = nullptr;    // [N] system of each atom (engine-owned, set per launch)
+    int* sflag = nullptr;         // [S] per-system status bits (engine-owned, set per launch):
+                                  //     OG_ST_SASA on an atom over capacity, OG_ST_RANGE on a
+                                  //     force or energy out of the fixed-point range
 };
 
 // sys_tile_off[s] .. sys_tile_off[s+1] are system s's tiles (64-atom, never
@@ -64,8 +67,9 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
 void og_sasa_free(og_sasa_data* d);
 // eacc != null: also accumulate gamma_i * A_i (unscaled gamma) into each
 // system's SASA energy slot (powersasa_energy, mod.rs:55-111).
+// atom_sys and sflag (engine-owned) are required.
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
-                   hipStream_t s, long long* eacc = nullptr, const int* atom_sys = nullptr);
+                   hipStream_t s, long long* eacc, const int* atom_sys, int* sflag);
 // Arrangement-kernel width (blocks of 2 waves); at most the staging slots allocated.
 int og_sasa_max_blocks(const og_sasa_data* d);
 void og_sasa_set_blocks(og_sasa_data* d, int blocks);

@@ -199,8 +199,9 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a, int blk, in
             else if (it.part == 0) descreen_tile_lds<MODEA, 0>(a, di, P, Q, lane, acc_i, acc_j);
             else descreen_tile_lds<MODEA, 1>(a, di, P, Q, lane, acc_i, acc_j);
         }
-        if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix((double)acc_i));
-        if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix((double)acc_j));
+        int* fl = a.sflag ? a.sflag + it.sys : nullptr;
+        if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix_chk((double)acc_i, fl));
+        if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix_chk((double)acc_j, fl));
     }
 }
 
@@ -419,17 +420,18 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
                 else force_tile_lds<MODEA, 1>(a, di, P, Q, lane, acc);
             }
         }
+        int* fl = a.sflag ? a.sflag + it.sys : nullptr;
         if (lane < ci) {
             long long* f = a.facc + 3 * (size_t)(a0i + lane);
-            og_atomic_add_fix(f + 0, og_to_fix((double)acc.fx));
-            og_atomic_add_fix(f + 1, og_to_fix((double)acc.fy));
-            og_atomic_add_fix(f + 2, og_to_fix((double)acc.fz));
+            og_atomic_add_fix(f + 0, og_to_fix_chk((double)acc.fx, fl));
+            og_atomic_add_fix(f + 1, og_to_fix_chk((double)acc.fy, fl));
+            og_atomic_add_fix(f + 2, og_to_fix_chk((double)acc.fz, fl));
         }
         if (lane < cj) {
             long long* f = a.facc + 3 * (size_t)(a0j + lane);
-            og_atomic_add_fix(f + 0, og_to_fix((double)acc.gx));
-            og_atomic_add_fix(f + 1, og_to_fix((double)acc.gy));
-            og_atomic_add_fix(f + 2, og_to_fix((double)acc.gz));
+            og_atomic_add_fix(f + 0, og_to_fix_chk((double)acc.gx, fl));
+            og_atomic_add_fix(f + 1, og_to_fix_chk((double)acc.gy, fl));
+            og_atomic_add_fix(f + 2, og_to_fix_chk((double)acc.gz, fl));
         }
     }
 }
@@ -548,9 +550,10 @@ __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
         eg = og_wave_sum_d(eg * pref);
         if (lane == 0) {
             long long* acc = a.eacc + (size_t)it.sys * OG_NE;
-            og_atomic_add_fix(acc + OG_E_LJ, og_to_fix(elj));
-            og_atomic_add_fix(acc + OG_E_COUL, og_to_fix(ec));
-            og_atomic_add_fix(acc + OG_E_GB_PAIR, og_to_fix(eg));
+            int* fl = a.sflag ? a.sflag + it.sys : nullptr;
+            og_atomic_add_fix(acc + OG_E_LJ, og_to_fix_chk(elj, fl));
+            og_atomic_add_fix(acc + OG_E_COUL, og_to_fix_chk(ec, fl));
+            og_atomic_add_fix(acc + OG_E_GB_PAIR, og_to_fix_chk(eg, fl));
         }
     }
 }

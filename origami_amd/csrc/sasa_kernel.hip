@@ -200,8 +200,16 @@ __device__ __forceinline__ float cover_ext(float p, float2 e) {
 
 __device__ __forceinline__ void add_sasa_energy(const og_sasa_data& d, int ia, double area) {
     // powersasa_energy (mod.rs:55-111): gamma * A, no gamma scale
-    og_atomic_add_fix(d.eacc + (size_t)d.atom_sys[ia] * OG_NE + OG_E_SASA,
-                      og_to_fix(d.gamma[ia] * area));
+    const int sys = d.atom_sys[ia];
+    og_atomic_add_fix(d.eacc + (size_t)sys * OG_NE + OG_E_SASA,
+                      og_to_fix_chk(d.gamma[ia] * area, d.sflag + sys));
+}
+
+// An atom whose arrangement does not fit the kernel's capacity: counted, and
+// its system flagged so that dynamics stop it at this step.
+__device__ __forceinline__ void sasa_capacity_exceeded(const og_sasa_data& d, int ia) {
+    atomicAdd(d.overflow, 1);
+    atomicOr(d.sflag + d.atom_sys[ia], OG_ST_SASA);
 }
 
 __device__ __forceinline__ double wave_min_d(double v) {
@@ -419,7 +427,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         // more caps than this pass holds: the MAXC pass takes the atom
         if (lane == 0) {
             if (NC < MAXC) d.deferred[atomicAdd(d.counters + 3, 1)] = w;
-            else atomicAdd(d.overflow, 1);
+            else sasa_capacity_exceeded(d, ia);
         }
         return empty;
     }
@@ -980,7 +988,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         a0 = a1;
     }
     if (overflow) {
-        if (lane == 0) atomicAdd(d.overflow, 1);
+        if (lane == 0) sasa_capacity_exceeded(d, ia);
         return empty;
     }
     // FullyBuried and the other arc-less outcomes contribute no gradient
@@ -1018,7 +1026,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     vbase = __shfl(vbase, 0, 64);
     obase = __shfl(obase, 0, 64);
     if (abase + na > d.arc_cap || vbase + nv > d.vtx_cap || obase + nc > d.own_cap) {
-        if (lane == 0) atomicAdd(d.overflow, 1);
+        if (lane == 0) sasa_capacity_exceeded(d, ia);
         return empty;
     }
     for (int c = lane; c < nc; c += 64) d.owners[obase + c] = L.own[c];
@@ -1309,9 +1317,10 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
         const int atom = x == 0 ? ia : d.owners[rec.z + x - 1];
         const double g0 = (-sa0 - sv0) * r_sq, g1 = (-sa1 - sv1) * r_sq, g2 = (-sa2 - sv2) * r_sq;
         const double f0 = -(g0 * gam), f1 = -(g1 * gam), f2 = -(g2 * gam);
-        if (f0 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 0, og_to_fix(f0));
-        if (f1 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 1, og_to_fix(f1));
-        if (f2 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 2, og_to_fix(f2));
+        int* fl = d.sflag + d.atom_sys[ia];
+        if (f0 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 0, og_to_fix_chk(f0, fl));
+        if (f1 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 1, og_to_fix_chk(f1, fl));
+        if (f2 != 0.0) og_atomic_add_fix(facc + 3 * (size_t)atom + 2, og_to_fix_chk(f2, fl));
     }
 }
 
@@ -1450,11 +1459,16 @@ void og_sasa_free(og_sasa_data* d) {
 }
 
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
-                   hipStream_t s, long long* eacc, const int* atom_sys) {
+                   hipStream_t s, long long* eacc, const int* atom_sys, int* sflag) {
     const bool area = eacc != nullptr;
     if (d->n_work == 0 || (gamma_scale == 0.0 && !area)) return 0;
+    if (!atom_sys || !sflag) {
+        g_sasa_err = "atom_sys and sflag are required";
+        return -1;
+    }
     d->eacc = eacc;
     d->atom_sys = atom_sys;
+    d->sflag = sflag;
     hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles > 0 ? d->n_tiles : 1), dim3(64), 0, s, pos,
                        d->tile_atom0, d->tile_cnt, d->n_tiles, d->tile_bound, d->counters);
     // One pass with MAXC caps when the grid fits its LDS; otherwise the

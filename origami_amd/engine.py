@@ -59,7 +59,17 @@ class SystemDesc(C.Structure):
 class DynamicsParams(C.Structure):
     _fields_ = [("dt_fs", C.c_double), ("friction_ps_inv", C.c_double), ("seed", C.c_uint64),
                 ("constrain_h_bonds", C.c_int32), ("flags", C.c_uint32),
-                ("sasa_gamma_scale", C.c_double)]
+                ("sasa_gamma_scale", C.c_double), ("cutoff_a", C.c_double)]
+
+
+# origami's DEFAULT_CUTOFF_A (energy/src/nonbonded.rs:37): the LJ / Coulomb
+# cutoff its integrator passes; used here under mode_a
+DEFAULT_CUTOFF_A = 10.0
+
+# per-system status bits (og_get_status)
+STATUS_NONFINITE = 1
+STATUS_RANGE = 2
+STATUS_SASA = 4
 
 
 class EngineError(RuntimeError):
@@ -86,8 +96,8 @@ def lib():
         for nm in ("og_set_positions", "og_get_positions", "og_set_velocities",
                    "og_get_velocities", "og_get_forces"):
             getattr(L, nm).argtypes = [vp, _dp]
-        L.og_forces.argtypes = [vp, _dp, _dp, C.c_uint32, C.c_double, _dp]
-        L.og_energies.argtypes = [vp, _dp, C.c_uint32, _dp]
+        L.og_forces.argtypes = [vp, _dp, _dp, C.c_uint32, C.c_double, C.c_double, _dp]
+        L.og_energies.argtypes = [vp, _dp, C.c_uint32, C.c_double, _dp]
         L.og_born_radii.argtypes = [vp, _dp, _dp]
         L.og_dynamics_setup.argtypes = [vp, C.POINTER(DynamicsParams), _dp]
         L.og_run.argtypes = [vp, C.c_int32, _dp]
@@ -222,7 +232,9 @@ class Engine:
 
     # -- forces / energies ----------------------------------------------------
     def total_force(self, xyz=None, mode_a=False, include_sasa=False, sasa_gamma_scale=1.0,
-                    energies=False, surrogate=False):
+                    energies=False, surrogate=False, cutoff_a=DEFAULT_CUTOFF_A):
+        """mode_a: origami's integrator cutoffs with cutoff_a for LJ / Coulomb
+        (total_force_with_scratch's cutoff_a, forces.rs:82); otherwise all pairs."""
         flags = ((MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
                  | (SURROGATE if surrogate else 0))
         f = np.zeros((self.n_atoms, 3))
@@ -231,19 +243,25 @@ class Engine:
         if xyz is not None:
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
         _check(lib().og_forces(self._h, _d(x) if x is not None else None, _d(f), flags,
-                               float(sasa_gamma_scale), _d(e) if e is not None else None))
+                               float(cutoff_a), float(sasa_gamma_scale),
+                               _d(e) if e is not None else None))
         if energies:
             return f, [dict(zip(ENERGY_TERMS, row)) for row in e]
         return f
 
-    def total_energy(self, xyz=None, mode_a=False, surrogate=False, include_sasa=False):
+    def total_energy(self, xyz=None, mode_a=False, surrogate=False, include_sasa=False,
+                     cutoff_a=DEFAULT_CUTOFF_A, allow_nonfinite=False):
+        """Per-system energy terms.  allow_nonfinite: a system whose sums left
+        the fixed-point range gets NaN terms instead of an exception."""
         e = np.zeros((self.n_systems, len(ENERGY_TERMS)))
         x = None
         if xyz is not None:
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
-        _check(lib().og_energies(self._h, _d(x) if x is not None else None,
-                                 (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0)
-                                 | (SASA if include_sasa else 0), _d(e)))
+        rc = lib().og_energies(self._h, _d(x) if x is not None else None,
+                               (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0)
+                               | (SASA if include_sasa else 0), float(cutoff_a), _d(e))
+        if not (allow_nonfinite and rc == -4):
+            _check(rc)
         out = []
         for row in e:
             dct = dict(zip(ENERGY_TERMS, row.tolist()))
@@ -262,7 +280,7 @@ class Engine:
     # -- dynamics ---------------------------------------------------------------
     def dynamics_setup(self, temperatures, dt_fs=1.0, friction_ps_inv=1.0, seed=0,
                        constrain_h_bonds=False, mode_a=False, include_sasa=False,
-                       sasa_gamma_scale=1.0, surrogate=False):
+                       sasa_gamma_scale=1.0, surrogate=False, cutoff_a=DEFAULT_CUTOFF_A):
         t = np.ascontiguousarray(np.broadcast_to(np.asarray(temperatures, dtype=np.float64),
                                                  (self.n_systems,)))
         self.temperatures = t.copy()
@@ -270,14 +288,21 @@ class Engine:
                            1 if constrain_h_bonds else 0,
                            (MODE_A if mode_a else 0) | (SASA if include_sasa else 0)
                            | (SURROGATE if surrogate else 0),
-                           sasa_gamma_scale)
+                           sasa_gamma_scale, cutoff_a)
         self.dt_fs = dt_fs
         _check(lib().og_dynamics_setup(self._h, C.byref(p), _d(t)))
 
-    def run(self, n_steps, record_ke=False):
+    def run(self, n_steps, record_ke=False, allow_stopped=False):
+        """n_steps BAOAB steps of every system.  A system whose velocities turn
+        non-finite (or whose forces leave the fixed-point range, or whose SASA
+        exceeds capacity) stops at that step while the others keep running
+        (remd.rs:382-389); that raises unless allow_stopped, and status()
+        names the stopped systems."""
         ke = np.zeros((n_steps, self.n_systems)) if record_ke and n_steps > 0 else None
         rc = lib().og_run(self._h, int(n_steps), _d(ke) if ke is not None else None)
         if rc == -4:
+            if allow_stopped:
+                return ke
             raise EngineError("diverged: " + lib().og_last_error().decode())
         _check(rc)
         return ke
@@ -455,6 +480,7 @@ class RemdSummary:
     swap_accepts: List[int]
     temperatures: List[float]
     final_potential_energy: List[float] = field(default_factory=list)
+    diverged: List[bool] = field(default_factory=list)
 
     def acceptance_ratios(self):
         return [a / t if t else 0.0 for a, t in zip(self.swap_accepts, self.swap_attempts)]
@@ -469,22 +495,31 @@ def geometric_ladder(t_min, t_max, n):
 def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs: float,
              swap_interval_fs: float, dt_fs=1.0, friction_ps_inv=2.0, seed=0,
              constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0, mode_a=False,
-             engine: Optional[Engine] = None):
+             engine: Optional[Engine] = None, velocities=None):
     """Parallel tempering with every replica in one device batch (remd.rs:136-498).
+
+    velocities: initial velocities of all replicas, (n_replicas * N, 3) in
+    ladder order (a restart); by default Maxwell-Boltzmann draws from each
+    replica's seed without COM removal (remd.rs:204-206).
 
     Replicas advance together in the batched step; swap decisions use the
     replicas' potential energies, origami's swap RNG (seed + 0xDEADBEEF) and
     Metropolis rule on alternating adjacent pairs, then exchange positions,
-    velocities and forces with the sqrt(T_a/T_b) velocity rescale.
+    velocities and forces with the sqrt(T_a/T_b) velocity rescale.  A replica
+    that diverges stops while the others keep running, and pairs with a
+    stopped replica are not attempted (remd.rs:382-389, :430).  With
+    include_sasa the swap energies carry the SASA term (remd.rs:533-537).
     """
     temps = sorted(float(t) for t in temperatures)
     n_rep = len(temps)
     eng = engine or Engine([sys] * n_rep)
-    v0 = []
-    for i, t in enumerate(temps):
-        rng = S.Xoshiro256pp((seed + i * 0xC0FFEE) & ((1 << 64) - 1))
-        v0.append(maxwell_boltzmann(sys, t, rng, remove_com=False))
-    eng.set_velocities(np.concatenate(v0))
+    if velocities is None:
+        v0 = []
+        for i, t in enumerate(temps):
+            rng = S.Xoshiro256pp((seed + i * 0xC0FFEE) & ((1 << 64) - 1))
+            v0.append(maxwell_boltzmann(sys, t, rng, remove_com=False))
+        velocities = np.concatenate(v0)
+    eng.set_velocities(velocities)
     eng.dynamics_setup(temps, dt_fs=dt_fs, friction_ps_inv=friction_ps_inv, seed=seed,
                        constrain_h_bonds=constrain_h_bonds, mode_a=mode_a,
                        include_sasa=include_sasa, sasa_gamma_scale=sasa_gamma_scale)
@@ -498,10 +533,14 @@ def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs
         n = min(steps_per_swap, total_steps - rnd * steps_per_swap)
         if n <= 0:
             break
-        eng.run(n)
-        energies = [e["total"] for e in eng.total_energy(mode_a=mode_a)]
+        eng.run(n, allow_stopped=True)
+        stopped = eng.status()["diverged"] != 0
+        energies = [e["total"] for e in eng.total_energy(mode_a=mode_a, include_sasa=include_sasa,
+                                                          allow_nonfinite=True)]
         for pair in range(rnd % 2, n_rep - 1, 2):
             a, b = pair, pair + 1
+            if stopped[a] or stopped[b]:
+                continue
             attempts[pair] += 1
             beta_a = 1.0 / (S.BOLTZMANN_KJ * temps[a])
             beta_b = 1.0 / (S.BOLTZMANN_KJ * temps[b])
@@ -511,5 +550,7 @@ def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs
                 accepts[pair] += 1
                 eng.swap(a, b)
                 energies[a], energies[b] = energies[b], energies[a]
-    final = [e["total"] for e in eng.total_energy(mode_a=mode_a)]
-    return eng, RemdSummary(n_rep, sys.n_atoms, attempts, accepts, temps, final)
+    final = [e["total"] for e in eng.total_energy(mode_a=mode_a, include_sasa=include_sasa,
+                                                   allow_nonfinite=True)]
+    stopped = [bool(x) for x in eng.status()["diverged"]]
+    return eng, RemdSummary(n_rep, sys.n_atoms, attempts, accepts, temps, final, stopped)

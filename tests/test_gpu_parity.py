@@ -4,6 +4,7 @@ Bar (from the engine's contract): FP32 pair arithmetic with int64
 fixed-point accumulation must stay within 1e-5 relative force RMS of the
 FP64 oracle; FP64 paths (bonded, SHAKE, integrator, SASA) within 1e-9.
 """
+import copy
 import math
 
 import numpy as np
@@ -19,6 +20,12 @@ FORCE_RMS_TOL = 1e-5
 
 def rel_rms(a, b):
     return float(np.sqrt(((a - b) ** 2).sum() / (b ** 2).sum()))
+
+
+def energy_close(got, ref, rel=1e-9, floor=1e-6):
+    """FP64 energy terms: relative 1e-9 with an absolute floor of 1e-6 kJ/mol
+    (fixed-point units are 2.3e-10 kJ/mol per contribution)."""
+    return abs(got - ref) <= rel * abs(ref) + floor
 
 
 @pytest.fixture(scope="module")
@@ -166,14 +173,10 @@ def test_energies_c2(E, c2_system):
         ref = orc.total_energy(s.pos, cut_nb=cut["cut_nb"], cut_descreen=cut["cut_descreen"])
         eng = E.Engine([s])
         got = eng.total_energy(mode_a=mode_a)[0]
-        for k in ("bond", "angle", "dihedral", "improper"):
-            assert abs(got[k] - ref[k]) <= 1e-8 * max(1.0, abs(ref[k])), k
         print("mode_a=%s" % mode_a, {k: (got[k], ref[k]) for k in ref if k in got})
-        for k in ("lj", "coulomb"):
-            assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
-        assert abs(got["gb_self"] - ref["gb_self"]) <= 1e-9 * abs(ref["gb_self"])
-        assert abs(got["gb_pair"] - ref["gb_pair"]) <= 1e-4 * abs(ref["gb_pair"])
-        assert abs(got["total"] - ref["total"]) <= 1e-4 * max(1.0, abs(ref["total"]))
+        for k in ("bond", "angle", "dihedral", "improper", "lj", "coulomb", "gb_self", "gb_pair",
+                  "total"):
+            assert energy_close(got[k], ref[k]), (k, got[k], ref[k])
 
 
 def test_uneven_batch_matches_single(E):
@@ -496,11 +499,149 @@ def test_split_tile_pairs_small_system(E, c3_system, mode_a):
     assert err < FORCE_RMS_TOL
     ref = orc.total_energy(pos, cut_nb=cut["cut_nb"], cut_descreen=cut["cut_descreen"])
     got = eng.total_energy(pos, mode_a=mode_a)[0]
-    for k in ("lj", "coulomb"):
-        assert abs(got[k] - ref[k]) <= 1e-4 * max(1.0, abs(ref[k])), (k, got[k], ref[k])
-    assert abs(got["gb_self"] - ref["gb_self"]) <= 1e-9 * abs(ref["gb_self"])
-    assert abs(got["gb_pair"] - ref["gb_pair"]) <= 1e-4 * abs(ref["gb_pair"])
+    for k in ("lj", "coulomb", "gb_self", "gb_pair"):
+        assert energy_close(got[k], ref[k]), (k, got[k], ref[k])
     big = S.seeded_chain(1, 20)                     # 300+ atoms: whole tile pairs
     both = E.Engine([big, s])
     fb = both.total_force(np.concatenate([big.pos, pos]), mode_a=mode_a)
     assert np.array_equal(both.split(fb)[1], f)
+
+
+@pytest.mark.parametrize("mode_a", [False, True])
+def test_diagonal_tile_far_exclusions(E, mode_a):
+    """Exclusion and 1-4 pairs more than 32 atoms apart inside one 64-atom
+    tile: the diagonal-tile force kernel evaluates such a pair on the lane of
+    its higher atom, which must carry the pair's mask bit too (a C(i)-CD(i+1)
+    1-3 pair across a large residue followed by Pro spans 33)."""
+    s = S.build_system(["Trp", "Pro", "Trp", "Pro", "Trp", "Pro"])
+    far = lambda p: [(int(a), int(b)) for a, b in np.asarray(p).reshape(-1, 2)
+                     if a // 64 == b // 64 and b - a > 32]
+    assert far(s.excl_pairs) and far(s.pair14)
+    pos = jittered(s, 12)
+    orc = O.Oracle(s)
+    cut = O.MODE_A if mode_a else O.MODE_B
+    fref = orc.total_force(pos, **cut)
+    eng = E.Engine([s])
+    f = eng.total_force(pos, mode_a=mode_a)
+    err = rel_rms(f, fref)
+    print("Trp-Pro x3 mode_a=%s rel force RMS %.3e, far pairs %s / %s"
+          % (mode_a, err, far(s.excl_pairs), far(s.pair14)))
+    assert err < FORCE_RMS_TOL
+    ref = orc.total_energy(pos, cut_nb=cut["cut_nb"], cut_descreen=cut["cut_descreen"])
+    got = eng.total_energy(pos, mode_a=mode_a)[0]
+    for k in ("lj", "coulomb", "gb_pair", "total"):
+        assert energy_close(got[k], ref[k]), (k, got[k], ref[k])
+
+
+def test_cutoff_a_sets_lj_coulomb_cutoff(E, c2_system):
+    """og_forces' cutoff_a (total_force_with_scratch's cutoff_a, forces.rs:82):
+    under OG_MODE_A it is the LJ / reaction-field Coulomb cutoff, while the GB
+    pair force stays at 10 A and the descreening sum at 20 A."""
+    s = c2_system
+    pos = jittered(s, 31)
+    orc = O.Oracle(s)
+    eng = E.Engine([s])
+    for rc in (8.0, 12.5):
+        fref = orc.total_force(pos, cut_nb=rc, cut_gb=10.0, cut_descreen=20.0)
+        f = eng.total_force(pos, mode_a=True, cutoff_a=rc)
+        assert rel_rms(f, fref) < FORCE_RMS_TOL, rc
+        ref = orc.total_energy(pos, cut_nb=rc, cut_descreen=20.0)
+        got = eng.total_energy(pos, mode_a=True, cutoff_a=rc)[0]
+        for k in ("lj", "coulomb"):
+            assert energy_close(got[k], ref[k]), (rc, k, got[k], ref[k])
+    for bad in (0.0, -1.0, float("inf"), float("nan")):
+        with pytest.raises(E.EngineError, match="cutoff_a"):
+            eng.total_force(pos, mode_a=True, cutoff_a=bad)
+    # without OG_MODE_A every pair counts and cutoff_a is not read
+    assert np.array_equal(eng.total_force(pos, cutoff_a=float("nan")), eng.total_force(pos))
+
+
+def test_fixed_point_range_is_reported(E, c3_system):
+    """Two atoms 0.3 A apart give pair forces far beyond the int64 fixed-point
+    range: the evaluation reports OG_ERR_NONFINITE with the system's range bit
+    instead of returning wrapped sums, every kernel exits normally, and the
+    engine evaluates normally again afterwards."""
+    s = c3_system
+    eng = E.Engine([s])
+    f_ok = eng.total_force()
+    pos = s.pos.copy()
+    heavy = np.flatnonzero(s.mass > 2.0)
+    a, b = int(heavy[0]), int(heavy[-1])
+    pos[b] = pos[a] + np.array([0.3, 0.0, 0.0])
+    with pytest.raises(E.EngineError, match="fixed-point range"):
+        eng.total_force(pos)
+    e = eng.total_energy(pos, allow_nonfinite=True)[0]
+    assert np.isnan(e["lj"])
+    assert np.array_equal(eng.total_force(s.pos), f_ok)
+
+
+def test_sasa_capacity_stops_the_system(E, c1_system):
+    """A packing where SASA atoms overlap more than 128 neighbours exceeds the
+    arrangement's capacity: the evaluation reports it, and in dynamics that
+    system stops at the step where it happened while the others run on."""
+    s = c1_system
+    pos = dense_packing(s.n_atoms, 9, density=0.16, min_sep=1.2)
+    rad = s.sasa_radius
+    d = np.linalg.norm(pos[:, None] - pos[None], axis=2)
+    over = ((d < rad[:, None] + rad[None]) & (d > 0))[s.sasa_gamma != 0].sum(1)
+    assert (over > 128).sum() >= 1
+    single = E.Engine([s])
+    with pytest.raises(E.EngineError, match="capacity"):
+        single.total_force(pos, include_sasa=True, sasa_gamma_scale=0.25)
+
+    def make():
+        eng = E.Engine([s, s])
+        eng.set_velocities(np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(3 + k))
+                                           for k in range(2)]))
+        eng.dynamics_setup([300.0, 300.0], dt_fs=1.0, friction_ps_inv=2.0, seed=8,
+                           include_sasa=True, sasa_gamma_scale=0.25)
+        eng.set_positions(np.concatenate([pos, s.pos]))
+        return eng
+    one, five = make(), make()
+    one.run(1, allow_stopped=True)
+    with pytest.raises(E.EngineError, match="stopped"):
+        five.run(5)
+    st = five.status()["diverged"]
+    print("status bits after 5 steps:", st)
+    assert st[0] & E.STATUS_SASA and st[1] == 0
+    x1, x5 = one.split(one.positions()), five.split(five.positions())
+    assert np.array_equal(x1[0], x5[0])             # stopped at the first step
+    assert not np.array_equal(x1[1], x5[1])         # the other system ran on
+    assert np.isfinite(x5[1]).all()
+
+
+def test_swap_refuses_other_topology(E, c3_system):
+    s = c3_system
+    t = copy.deepcopy(s)
+    t.charge = t.charge.copy()
+    t.charge[0] += 0.01
+    eng = E.Engine([s, t, s])
+    eng.dynamics_setup([300.0, 310.0, 320.0], dt_fs=1.0)
+    with pytest.raises(E.EngineError, match="topology"):
+        eng.swap(0, 1)
+    eng.swap(0, 2)
+
+
+def test_remd_stopped_replica_is_skipped(E, c3_system):
+    """remd.rs:382-389, :430: a replica whose velocities blow up stops, pairs
+    with it are not attempted, and the other replicas run on."""
+    import copy as cp
+    from origami_amd.minimize import minimize
+    e0 = E.Engine([c3_system])
+    minimize(e0, max_steps=200, mode_a=False)
+    s = cp.copy(c3_system)
+    s.pos = e0.positions()
+    temps = E.geometric_ladder(280.0, 395.0, 6)
+    v = [E.maxwell_boltzmann(s, t, S.Xoshiro256pp(50 + k), remove_com=False)
+         for k, t in enumerate(temps)]
+    v[2] = v[2] * 1e150
+    eng, summ = E.run_remd(s, temps, total_time_fs=400.0, swap_interval_fs=50.0, dt_fs=2.0,
+                           friction_ps_inv=2.0, seed=4, constrain_h_bonds=True,
+                           velocities=np.concatenate(v))
+    print(summ)
+    assert summ.diverged == [False, False, True, False, False, False]
+    assert summ.swap_attempts[1] == 0 and summ.swap_attempts[2] == 0
+    assert summ.swap_attempts[0] > 0 and summ.swap_attempts[3] > 0 and summ.swap_attempts[4] > 0
+    xs = eng.split(eng.positions())
+    for k in (0, 1, 3, 4, 5):
+        assert np.isfinite(xs[k]).all()
