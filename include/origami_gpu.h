@@ -33,7 +33,7 @@ enum og_status {
 
 /* Per-system status bits (og_get_status's diverged[], og_run's stop reasons). */
 #define OG_STATUS_NONFINITE 1u  /* non-finite velocities after a step (langevin.rs:316-319) */
-#define OG_STATUS_RANGE 2u      /* a force/energy contribution NaN or >= 2^26 kJ/mol(/Å): outside
+#define OG_STATUS_RANGE 2u      /* a force/energy contribution NaN or >= 2^30 kJ/mol(/Å), or a summed force >= 2^30: outside
                                    the int64 fixed-point accumulators' range */
 #define OG_STATUS_SASA 4u       /* a SASA atom exceeded the arrangement's capacity
                                    (128 caps, 256 arcs, 256 vertices) */

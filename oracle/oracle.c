@@ -622,15 +622,21 @@ double orc_kinetic_energy(const orc_sys *s, const double *vel) {
 /* ------------------------------------------------------------------------ */
 static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
 
-void orc_xoshiro_u64(uint64_t seed, int count, uint64_t *out) {
-    uint64_t st[4], sm = seed;
-    for (int k = 0; k < 4; ++k) {
+/* SplitMix64::next, rng.rs:87-97: the seeding generator. */
+void orc_splitmix64(uint64_t seed, int count, uint64_t *out) {
+    uint64_t sm = seed;
+    for (int k = 0; k < count; ++k) {
         sm += 0x9e3779b97f4a7c15ULL;
         uint64_t z = sm;
         z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
         z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-        st[k] = z ^ (z >> 31);
+        out[k] = z ^ (z >> 31);
     }
+}
+
+/* Xoshiro256pp::next_u64, rng.rs:40-55, from an explicit state st[4]
+ * (advanced in place). */
+void orc_xoshiro_from_state(uint64_t *st, int count, uint64_t *out) {
     for (int k = 0; k < count; ++k) {
         uint64_t result = rotl64(st[0] + st[3], 23) + st[0];
         uint64_t t = st[1] << 17;
@@ -642,6 +648,13 @@ void orc_xoshiro_u64(uint64_t seed, int count, uint64_t *out) {
         st[3] = rotl64(st[3], 45);
         out[k] = result;
     }
+}
+
+/* Xoshiro256pp::from_seed (rng.rs:27-37): state = four SplitMix64 outputs. */
+void orc_xoshiro_u64(uint64_t seed, int count, uint64_t *out) {
+    uint64_t st[4];
+    orc_splitmix64(seed, 4, st);
+    orc_xoshiro_from_state(st, count, out);
 }
 
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11), the counter-based

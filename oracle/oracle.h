@@ -39,6 +39,8 @@ int orc_baoab_step(const orc_sys *s, double *pos, double *vel, double *forces, c
                    double cut_nb, double cut_gb, double cut_descreen);
 double orc_kinetic_energy(const orc_sys *s, const double *vel);
 void orc_xoshiro_u64(uint64_t seed, int count, uint64_t *out);
+void orc_splitmix64(uint64_t seed, int count, uint64_t *out);
+void orc_xoshiro_from_state(uint64_t *st, int count, uint64_t *out);
 void orc_philox4x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t c2,
                     uint32_t c3, uint32_t *out);
 void orc_philox_normals(uint32_t k0, uint32_t k1, uint32_t sys, int n_atoms, uint32_t step,

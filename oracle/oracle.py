@@ -57,6 +57,8 @@ def lib():
         L.orc_kinetic_energy.restype = C.c_double
         L.orc_kinetic_energy.argtypes = [C.c_void_p, dp]
         L.orc_xoshiro_u64.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_splitmix64.argtypes = [C.c_uint64, C.c_int, C.POINTER(C.c_uint64)]
+        L.orc_xoshiro_from_state.argtypes = [C.POINTER(C.c_uint64), C.c_int, C.POINTER(C.c_uint64)]
         L.orc_philox4x32.argtypes = [C.c_uint32] * 6 + [C.POINTER(C.c_uint32)]
         L.orc_philox_normals.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, C.c_uint32,
                                          C.POINTER(C.c_double)]
@@ -229,6 +231,19 @@ def pairwise_descreening(r, rho_i_tilde, s_rho_j_tilde):
 def xoshiro_u64(seed, count):
     out = (C.c_uint64 * count)()
     lib().orc_xoshiro_u64(seed, count, out)
+    return [int(x) for x in out]
+
+
+def splitmix64(seed, count):
+    out = (C.c_uint64 * count)()
+    lib().orc_splitmix64(seed, count, out)
+    return [int(x) for x in out]
+
+
+def xoshiro_from_state(state, count):
+    st = (C.c_uint64 * 4)(*state)
+    out = (C.c_uint64 * count)()
+    lib().orc_xoshiro_from_state(st, count, out)
     return [int(x) for x in out]
 
 

@@ -785,7 +785,7 @@ static int check_sasa_overflow(og_engine* e, uint32_t flags) {
 
 static const char* status_reason(int bits) {
     if (bits & OG_ST_SASA) return "a SASA atom exceeded the arrangement's capacity (128 caps / 256 arcs / 256 vertices)";
-    if (bits & OG_ST_RANGE) return "a force or energy contribution was NaN or beyond the fixed-point range (|x| >= 2^26)";
+    if (bits & OG_ST_RANGE) return "a force or energy contribution was NaN or beyond the fixed-point range (|x| >= 2^30)";
     return "non-finite velocities";
 }
 
@@ -818,7 +818,7 @@ static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t fla
         og_back_args b{};
         b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
         b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = 0.0; b.do_kick = 0;
-        b.ke_acc = nullptr; b.diverged = e->diverged;
+        b.ke_acc = nullptr; b.diverged = e->diverged; b.sflag = e->sflag;
         og_launch_back(b, e->stream);
     }
     HIPCHK(hipGetLastError());

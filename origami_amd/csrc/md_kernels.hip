@@ -503,12 +503,17 @@ __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
     if (i >= a.n_atoms) return;
     if (i == 0 && a.do_kick && a.step_dev) *a.step_dev += 1u;
     double f[3];
+    bool big = false;
     for (int d = 0; d < 3; ++d) {
         f[d] = (double)a.facc[3 * (size_t)i + d] * OG_FIX_INV;
         a.force[3 * (size_t)i + d] = f[d];
+        big = big || !(fabs(f[d]) < OG_FIX_LIMIT);
     }
-    if (!a.do_kick) return;
     const int sys = a.atom_sys[i];
+    // a summed force at or beyond the contribution limit: the int64 sum may
+    // have wrapped, so the system is reported as out of range
+    if (big && a.sflag) atomicOr(a.sflag + sys, OG_ST_RANGE);
+    if (!a.do_kick) return;
     if (a.diverged[sys]) return;
     // a contribution out of range (or a SASA atom over capacity) in this
     // step's forces: the system stops here, as on non-finite velocities

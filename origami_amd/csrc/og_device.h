@@ -47,9 +47,12 @@ struct og_work {              // one tile pair (I <= J) of one system
 #define OG_ST_SASA 4          // a SASA atom exceeded the arrangement's capacity
 
 // Largest magnitude one contribution may have (kJ/mol/Å, kJ/mol or Å^-1):
-// 2^26.  An int64 accumulator at 2^32 units holds 2^31 in all, so 32
-// contributions at the limit still cannot wrap it.
-#define OG_FIX_LIMIT 67108864.0
+// 2^30.  An int64 accumulator at 2^32 units holds sums below 2^31; a sum
+// whose magnitude reaches 2^30 is reported as well when it is read (the
+// force finalisation), so a wrapped sum goes unreported only when the
+// contributions to one atom component add up to more than 3 * 2^30 in
+// magnitude (four or more of them, each below 2^30).
+#define OG_FIX_LIMIT 1073741824.0
 
 __device__ __forceinline__ long long og_to_fix(double x) {
     return __double2ll_rn(x * OG_FIX_SCALE);

@@ -84,6 +84,22 @@ def test_total_force_c2(E, c2_system, mode_a):
     assert err < FORCE_RMS_TOL
 
 
+@pytest.fixture(scope="module")
+def relaxed(E, c1_system, c2_system, c3_system):
+    """The seeded chains after 300 L-BFGS steps (all pairs).  Dynamics straight
+    from the extended builder output runs away within a few steps at dt = 2 fs
+    (it leaves ~1e4 kJ/mol/A on strained rings), on the oracle as well."""
+    from origami_amd.minimize import minimize
+    out = {}
+    for name, s in (("c1", c1_system), ("c2", c2_system), ("c3", c3_system)):
+        eng = E.Engine([s])
+        minimize(eng, max_steps=300, mode_a=False)
+        r = copy.copy(s)
+        r.pos = eng.positions()
+        out[name] = r
+    return out
+
+
 def jittered(s, seed, scale=0.02):
     """Generic geometry: the builder's exact symmetries removed."""
     return s.pos + np.random.default_rng(seed).normal(scale=scale, size=s.pos.shape)
@@ -286,22 +302,27 @@ def test_remd_runs_and_swaps(E, c3_system):
 
 
 def test_trp_cage_minimise_acceptance(E):
-    # dynamics/tests/trp_cage_minimise.rs: extended Trp-cage, origami's
-    # minimiser settings (10 Å cutoffs) -> below 1000 kJ/mol within 500 steps
-    # with the bond term under 200 kJ/mol.
+    # dynamics/tests/trp_cage_minimise.rs:12-55: extended Trp-cage, L-BFGS with
+    # max_steps 500, gradient_tol 5.0, max_step_a 0.1 and origami's minimiser
+    # energy (10 Å cutoffs) -> below 1000 kJ/mol in fewer than 500 steps with
+    # the bond term under 200 kJ/mol.
     from origami_amd.minimize import minimize
-    s = S.build_system(list("NLYIQWLKDGGPSSGRPPPS"))
+    s = S.build_system([S.FROM_ONE_LETTER[c] for c in "NLYIQWLKDGGPSSGRPPPS"])
     eng = E.Engine([s])
-    res = minimize(eng, max_steps=500, mode_a=True)[0]
+    res = minimize(eng, max_steps=500, gradient_tol=5.0, max_step_a=0.1, mode_a=True)[0]
     e = eng.total_energy(mode_a=True)[0]
     print(res, e)
+    assert res.initial_energy > 40000.0          # +47 506 kJ/mol extended (README.md:34-36)
     assert res.final_energy < 1000.0
+    assert res.steps < 500
     assert e["bond"] < 200.0
 
 
-def test_graph_replay_matches_eager(E, c1_system, c3_system):
+def test_graph_replay_matches_eager(E, relaxed):
     """Captured-step replay (og_run with graphs) against direct launches, with
     eager work and synchronisations between replays, every replay checked."""
+    c1_system = relaxed["c1"]
+    c3_system = relaxed["c3"]
     def make():
         eng = E.Engine([c1_system, c3_system, c3_system])
         v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(5 + k))
@@ -337,10 +358,12 @@ def test_autotune_keeps_results(E, c2_system):
     assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0
 
 
-def test_sasa_side_stream_matches_serial(E, c1_system, c2_system):
+def test_sasa_side_stream_matches_serial(E, relaxed):
     """SASA on its own stream beside the pair kernels (the default) against the
     one-stream order that the per-kernel profile uses: forces, energies and
     captured SASA dynamics bitwise equal."""
+    c1_system = relaxed["c1"]
+    c2_system = relaxed["c2"]
     systems = [c1_system, c2_system]
     ovl, ser = E.Engine(systems), E.Engine(systems)
     ser.profile(True)
@@ -366,11 +389,13 @@ def test_sasa_side_stream_matches_serial(E, c1_system, c2_system):
         assert np.array_equal(ovl.velocities(), ser.velocities()), rep
 
 
-def test_energies_leave_dynamics_forces(E, c1_system, c2_system):
+def test_energies_leave_dynamics_forces(E, relaxed):
     """og_energies between og_run chunks (as REMD does for its swaps, with
     other flags than the dynamics) must not change the forces the next step
     starts from: trajectories bitwise equal to a run without the calls, and
     the energies equal to those of a force-and-energy evaluation."""
+    c1_system = relaxed["c1"]
+    c2_system = relaxed["c2"]
     systems = [c1_system, c2_system]
 
     def make():
@@ -418,12 +443,14 @@ def test_sasa_energy_matches_oracle(E, c1_system, c2_system):
 
 
 @pytest.mark.parametrize("graphs", [False, True])
-def test_noise_ahead_matches_single_steps(E, c1_system, c3_system, graphs):
+def test_noise_ahead_matches_single_steps(E, relaxed, graphs):
     """Without SASA each step's O-step normals are formed during the force
     evaluation of the step before (one og_run call of several steps); a run of
     single-step calls forms every step's at the start of its call, the form
     that test_baoab_step_matches_oracle pins to the oracle's Philox normals.
     Both bitwise equal over steps that cover both halves of the buffer."""
+    c1_system = relaxed["c1"]
+    c3_system = relaxed["c3"]
     systems = [c1_system, c3_system]
 
     def make():
@@ -444,11 +471,13 @@ def test_noise_ahead_matches_single_steps(E, c1_system, c3_system, graphs):
         assert np.array_equal(many.velocities(), single.velocities()), rep
 
 
-def test_noise_paths_agree(E, c1_system, c3_system):
+def test_noise_paths_agree(E, relaxed):
     """Dynamics with SASA switched on at zero surface tension (no SASA force;
     the cluster step forms its own normals) against dynamics without SASA (the
     normals read from the precomputed buffer): the same trajectory up to the
     rounding of the O-step's multiply-add."""
+    c1_system = relaxed["c1"]
+    c3_system = relaxed["c3"]
     systems = [c1_system, c3_system]
     engs = []
     for sasa in (False, True):

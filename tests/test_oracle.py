@@ -14,7 +14,9 @@ KC = 332.0637
 
 
 def make_sys(pos, q=None, eps=None, rmin=None, rho=None, scale=None, mass=None, sasa_r=None,
-             sasa_g=None, excl=(), p14=(), bonds=(), bond_p=(), cons=(), cons_len=()):
+             sasa_g=None, excl=(), p14=(), bonds=(), bond_p=(), cons=(), cons_len=(),
+             eps14=None, rmin14=None, angles=(), angle_p=(), dih=(), dih_p=(), imps=(),
+             imp_p=()):
     pos = np.asarray(pos, dtype=np.float64).reshape(-1, 3)
     n = len(pos)
     z = np.zeros(n)
@@ -27,14 +29,18 @@ def make_sys(pos, q=None, eps=None, rmin=None, rho=None, scale=None, mass=None, 
         n_atoms=n, sequence=[], names=[""] * n, residue_index=np.zeros(n, np.int32),
         element=np.zeros(n, np.int32), atom_type=np.zeros(n, np.int32), pos=pos,
         charge=arr(q, 0.0), rmin_half=arr(rmin, 1.0), epsilon=arr(eps, 0.0),
-        rmin_half_14=arr(rmin, 1.0), epsilon_14=arr(eps, 0.0), gb_rho=arr(rho, 1.7),
+        rmin_half_14=arr(rmin if rmin14 is None else rmin14, 1.0),
+        epsilon_14=arr(eps if eps14 is None else eps14, 0.0), gb_rho=arr(rho, 1.7),
         gb_scale=arr(scale, 0.72), sasa_radius=arr(sasa_r, 3.1), sasa_gamma=arr(sasa_g, 0.0),
         mass=arr(mass, 12.011), excl_pairs=i2(excl), pair14=i2(p14), bonds=i2(bonds),
         bond_params=np.asarray(bond_p, dtype=np.float64).reshape(-1, 2),
-        angles=np.zeros((0, 3), np.int32), angle_params=np.zeros((0, 2)),
-        dihedrals=np.zeros((0, 4), np.int32), dihedral_params=np.zeros((0, 3)),
-        dihedral_first=np.zeros(0, np.int32), impropers=np.zeros((0, 4), np.int32),
-        improper_params=np.zeros((0, 2)), constraints=i2(cons),
+        angles=np.asarray(angles, np.int32).reshape(-1, 3),
+        angle_params=np.asarray(angle_p, np.float64).reshape(-1, 2),
+        dihedrals=np.asarray(dih, np.int32).reshape(-1, 4),
+        dihedral_params=np.asarray(dih_p, np.float64).reshape(-1, 3),
+        dihedral_first=np.ones(len(dih), np.int32),
+        impropers=np.asarray(imps, np.int32).reshape(-1, 4),
+        improper_params=np.asarray(imp_p, np.float64).reshape(-1, 2), constraints=i2(cons),
         constraint_len=np.asarray(cons_len, dtype=np.float64))
 
 
@@ -55,9 +61,40 @@ def test_philox_normals_moments():
     assert not np.array_equal(O.philox_normals(1, 0, 10, 0), O.philox_normals(1, 1, 10, 0))
 
 
-def test_xoshiro_matches_host_stream():
-    r = S.Xoshiro256pp(987654321)
-    assert O.xoshiro_u64(987654321, 64) == [r.next_u64() for _ in range(64)]
+def test_splitmix64_published_vector():
+    # SplitMix64 (Steele, Lea, Flood 2014; Vigna's splitmix64.c), the seeding
+    # generator of rng.rs:87-97: the published sequence for seed 1234567
+    # (Rosetta Code, "Pseudo-random numbers/Splitmix64").
+    assert O.splitmix64(1234567, 5) == [6457827717110365317, 3203168211198807973,
+                                        9817491932198370423, 4593380528125082431,
+                                        16408922859458223821]
+
+
+def test_xoshiro256pp_from_state_known_answers():
+    # xoshiro256++ (Blackman & Vigna, xoshiro256plusplus.c), rng.rs:40-55.
+    # By hand from state (1, 2, 3, 4):
+    #   out0 = rotl(1 + 4, 23) + 1 = 5 * 2^23 + 1 = 41943041;
+    #   then t = 2 << 17, s2 = 3^1 = 2, s3 = 4^2 = 6, s1 = 2^2 = 0, s0 = 1^6 = 7,
+    #   s2 = 2 ^ 2^18, s3 = rotl(6, 45) = 3 * 2^46;
+    #   out1 = rotl(7 + 3 * 2^46, 23) + 7 = (7 << 23) + (2^46 + 2^47 rotated to
+    #   bits 5, 6 = 96) + 7 = 58720359.
+    # The rest is the published reference sequence for this state (the test
+    # vector of the rand_xoshiro crate's Xoshiro256PlusPlus).
+    assert O.xoshiro_from_state([1, 2, 3, 4], 10) == [
+        41943041, 58720359, 3588806011781223, 3591011842654386, 9228616714210784205,
+        9973669472204895162, 14011001112246962877, 12406186145184390807,
+        15849039046786891736, 10450023813501588000]
+
+
+def test_xoshiro_seeding_and_host_stream():
+    # from_seed (rng.rs:27-37): the state is the first four SplitMix64 outputs
+    seed = 987654321
+    assert O.xoshiro_u64(seed, 64) == O.xoshiro_from_state(O.splitmix64(seed, 4), 64)
+    r = S.Xoshiro256pp(seed)
+    assert O.xoshiro_u64(seed, 64) == [r.next_u64() for _ in range(64)]
+    # next_f64 = (u64 >> 11) * 2^-53 (rng.rs:58-61)
+    r1, r2 = S.Xoshiro256pp(5), O.xoshiro_u64(5, 3)
+    assert [r1.next_f64() for _ in range(3)] == [(u >> 11) * 2.0 ** -53 for u in r2]
 
 
 # ------------------------------------------------------ LJ and Coulomb --
@@ -127,6 +164,30 @@ def test_descreening_closed_form_and_sign():
         for S_ in (0.94, 1.04, 1.17, 1.31, 1.64):
             for r in np.linspace(0.01, 30.0, 600):
                 assert O.pairwise_descreening(float(r), rho, S_) <= 0.0
+
+
+def test_descreening_far_field_and_branch_continuity():
+    # Far field (x = S/r << 1): h -> -S^3 / (3 r^4), minus the volume integral
+    # of 1 / (4 pi r'^4) over sphere j (4/3 pi S^3 / (4 pi r^4)).  The series
+    # x/(1-x^2) - x^3/(1-x^2)^2 - atanh x = -x^3/3 - 6/5 x^5 - 15/7 x^7 ... gives
+    # h / lead = 1 + 3.6 x^2 + O(x^4).
+    for rho, S_ in ((1.61, 1.2), (1.11, 0.94), (1.46, 1.64)):
+        for r in (20.0, 40.0, 80.0):
+            h = O.pairwise_descreening(r, rho, S_)
+            lead = -S_ ** 3 / (3 * r ** 4)
+            x = S_ / r
+            assert abs(h / lead - 1.0 - 3.6 * x * x) < 10.0 * x ** 4
+    # the overlapping branch (L = rho~_i) meets the separated one (L = r - S)
+    # at r = rho~_i + S, and the buried branch (0) at r = rho~_i - S
+    eps = 1e-9
+    for rho, S_ in ((1.61, 1.2), (1.46, 0.94), (1.71, 1.04)):
+        r0 = rho + S_
+        a, b = O.pairwise_descreening(r0 - eps, rho, S_), O.pairwise_descreening(r0 + eps, rho, S_)
+        assert abs(a - b) < 1e-7 * abs(a)
+        if rho > S_:
+            r1 = rho - S_
+            assert O.pairwise_descreening(r1 - eps, rho, S_) == 0.0
+            assert abs(O.pairwise_descreening(r1 + eps, rho, S_)) < 1e-7
 
 
 def test_born_radii_equal_rho_tilde(c1_system, c2_system):
@@ -200,6 +261,140 @@ def test_bonded_forces_finite_difference_and_zero_at_r0():
             assert abs(-(ep - E(p)) / (2 * h) - f[i, ax]) < 1e-3 * max(1.0, abs(f[i, ax]))
     b = make_sys([[0, 0, 0], [1.5, 0, 0]], bonds=[(0, 1)], bond_p=[(300.0, 1.5)])
     assert np.abs(O.Oracle(b).bonded_forces(b.pos)).max() == 0.0
+
+
+def test_angle_closed_form():
+    # bonded.rs:91-121 / forces_bonded.rs:77-116: V = K (theta - theta0)^2 (no 1/2),
+    # kcal -> kJ.  Atoms a = (r1, 0, 0), b = 0, c = r2 (cos t, sin t, 0).
+    K, th0, th, r1, r2 = 50.0, math.radians(107.0), math.radians(96.0), 1.45, 1.53
+    pos = [[r1, 0, 0], [0, 0, 0], [r2 * math.cos(th), r2 * math.sin(th), 0]]
+    s = make_sys(pos, angles=[(0, 1, 2)], angle_p=[(K, th0)])
+    orc = O.Oracle(s)
+    e = orc.bonded_energy(s.pos)
+    assert abs(e["angle"] - KJ * K * (th - th0) ** 2) < 1e-12
+    f = orc.bonded_forces(s.pos)
+    dv = 2.0 * KJ * K * (th - th0)                    # dV/dtheta
+    # dtheta/da = (0, -1, 0)/r1 and dtheta/dc = (-sin t, cos t, 0)/r2
+    assert np.abs(f[0] - np.array([0.0, dv / r1, 0.0])).max() < 1e-10
+    assert np.abs(f[2] + dv / r2 * np.array([-math.sin(th), math.cos(th), 0.0])).max() < 1e-10
+    assert np.abs(f.sum(0)).max() < 1e-10
+
+
+def dihedral_frame(phi):
+    """a = (0, 1, 0), b = 0, c = (1, 0, 0), d = c + (0, cos phi, sin phi): the
+    reference's own geometry (measure.rs:73-79, phi = +pi/2 there), whose
+    dihedral is phi with the IUPAC sign."""
+    return [[0, 1, 0], [0, 0, 0], [1, 0, 0], [1, math.cos(phi), math.sin(phi)]]
+
+
+def test_dihedral_closed_form_sign_and_phase():
+    # bonded.rs:123-157: sum_n k_n (1 + cos(n chi - delta)) with chi from
+    # measure::dihedral (IUPAC sign).  delta not in {0, pi} and n > 1, so a
+    # flipped dihedral sign or a dropped phase changes the value.
+    phi = math.radians(70.0)
+    terms = [(0.9, 2, math.radians(30.0)), (0.25, 3, 0.0)]
+    s = make_sys(dihedral_frame(phi), dih=[(0, 1, 2, 3)] * 2, dih_p=terms)
+    orc = O.Oracle(s)
+    e = orc.bonded_energy(s.pos)["dihedral"]
+    want = KJ * sum(k * (1 + math.cos(n * phi - d)) for k, n, d in terms)
+    assert abs(e - want) < 1e-12
+    assert abs(e - KJ * sum(k * (1 + math.cos(-n * phi - d)) for k, n, d in terms)) > 0.1
+    s1 = make_sys(dihedral_frame(math.pi / 2), dih=[(0, 1, 2, 3)], dih_p=[(1.0, 1, 0.0)])
+    assert abs(O.Oracle(s1).bonded_energy(s1.pos)["dihedral"] - KJ) < 1e-12   # 1 + cos(pi/2)
+    # Forces (forces_bonded.rs:124-202) take phi_g = atan2((s x b2^).t, s.t),
+    # which the reference states is opposite-signed (forces_bonded.rs:144-147):
+    # phi_g = -phi, d phi_g/dd = -(0, -sin phi, cos phi) (d at unit distance from
+    # the b-c axis), d phi_g/da = (0, 0, 1); F = -dV/dphi_g(phi_g) d phi_g/dr.
+    def dv(p, tt):                                   # dV/dphi at dihedral p, kJ/mol/rad
+        return -KJ * sum(k * n * math.sin(n * p - d) for k, n, d in tt)
+    ud = np.array([0.0, -math.sin(phi), math.cos(phi)])
+    ua = np.array([0.0, 0.0, -1.0])
+    f = orc.bonded_forces(s.pos)
+    assert np.abs(f[3] - dv(-phi, terms) * ud).max() < 1e-10
+    assert np.abs(f[0] - dv(-phi, terms) * ua).max() < 1e-10
+    assert np.abs(f.sum(0)).max() < 1e-10
+    # with phases in {0, pi} (every CHARMM36m protein dihedral with k != 0: the
+    # only other phase, X-CS-SS-X 0.2 degrees, has k = 0) that force is -grad V
+    t0 = [(0.9, 2, math.pi), (0.25, 3, 0.0)]
+    s0 = make_sys(dihedral_frame(phi), dih=[(0, 1, 2, 3)] * 2, dih_p=t0)
+    f0 = O.Oracle(s0).bonded_forces(s0.pos)
+    assert np.abs(f0[3] + dv(phi, t0) * ud).max() < 1e-10
+    assert np.abs(f0[0] + dv(phi, t0) * ua).max() < 1e-10
+
+
+def wrap_pi(x):
+    while x > math.pi:
+        x -= 2 * math.pi
+    while x < -math.pi:
+        x += 2 * math.pi
+    return x
+
+
+@pytest.mark.parametrize("omega_deg, omega0_deg", [
+    (-175.0, 170.0),    # energy: -345 wraps to +15
+    (175.0, -170.0),    # energy: +345 wraps to -15
+    (175.0, 170.0),     # force (omega_g = -175): -345 wraps to +15
+    (20.0, 0.0),        # psi0 = 0 (every CHARMM36m protein improper): force = -grad V
+])
+def test_improper_closed_form_and_wrap(omega_deg, omega0_deg):
+    # bonded.rs:159-191: K (omega - omega0)^2 with the difference wrapped to
+    # [-pi, pi]; forces_bonded.rs:207-236 wrap omega_g - omega0 with the
+    # opposite-signed omega_g = -omega of dihedral_gradient.
+    K = 120.0
+    om, om0 = math.radians(omega_deg), math.radians(omega0_deg)
+    s = make_sys(dihedral_frame(om), imps=[(0, 1, 2, 3)], imp_p=[(K, om0)])
+    orc = O.Oracle(s)
+    d = wrap_pi(om - om0)
+    assert abs(d) < math.radians(25.0)
+    assert abs(orc.bonded_energy(s.pos)["improper"] - KJ * K * d * d) < 1e-9
+    dg = wrap_pi(-om - om0)
+    ud = np.array([0.0, -math.sin(om), math.cos(om)])
+    f = orc.bonded_forces(s.pos)
+    # F_d = -2K dg * d omega_g/dd = 2K dg * ud
+    assert np.abs(f[3] - 2.0 * KJ * K * dg * ud).max() < 1e-9
+    assert np.abs(f[0] - 2.0 * KJ * K * dg * np.array([0.0, 0.0, -1.0])).max() < 1e-9
+    if om0 == 0.0:
+        assert np.abs(f[3] + 2.0 * KJ * K * d * ud).max() < 1e-9
+
+
+def test_one_four_pair_closed_form():
+    # nonbonded.rs:97-174 / forces_nonbonded.rs:196-232: a 1-4 pair takes the
+    # 1-4 LJ parameters (CT1: eps14 0.01, Rmin14/2 1.9 against eps 0.032,
+    # Rmin/2 2.0) and the full Coulomb term.
+    q = [0.3, -0.2]
+    kw = dict(q=q, eps=[0.032, 0.032], rmin=[2.0, 2.0], eps14=[0.01, 0.01], rmin14=[1.9, 1.9])
+    r = 3.8                                   # = Rmin14_i/2 + Rmin14_j/2: the 1-4 minimum
+    s = make_sys([[0, 0, 0], [r, 0, 0]], p14=[(0, 1)], **kw)
+    orc = O.Oracle(s)
+    e = orc.nonbonded_energy(s.pos, math.inf)
+    assert abs(e["lj"] - (-0.01 * KJ)) < 1e-12
+    assert abs(e["coulomb"] - KC * KJ * q[0] * q[1] / r) < 1e-10
+    f = orc.nonbonded_forces(s.pos, math.inf)
+    fc = KC * KJ * q[0] * q[1] / r ** 2          # Coulomb only: the 1-4 LJ force is 0 here
+    assert abs(f[1, 0] - fc) < 1e-10 and abs(f[0, 0] + fc) < 1e-10
+    # the same pair unlisted takes the plain parameters: minimum at 4.0, depth 0.032
+    s2 = make_sys([[0, 0, 0], [4.0, 0, 0]], **kw)
+    assert abs(O.Oracle(s2).nonbonded_energy(s2.pos, math.inf)["lj"] + 0.032 * KJ) < 1e-12
+    s3 = make_sys([[0, 0, 0], [4.0, 0, 0]], p14=[(0, 1)], **kw)
+    t = (3.8 / 4.0) ** 6
+    assert abs(O.Oracle(s3).nonbonded_energy(s3.pos, math.inf)["lj"]
+               - 0.01 * KJ * (t * t - 2 * t)) < 1e-12
+
+
+def test_extended_trp_cage_energy_anchor():
+    # The reference's published figures for the extended Trp-cage
+    # NLYIQWLKDGGPSSGRPPPS built by its NeRF builder: bond term +23 494 kJ/mol
+    # (dynamics/tests/trp_cage_minimise.rs:1-5), total +47 506 kJ/mol
+    # (README.md:34-36; a historical figure, quoted before the reaction-field
+    # Coulomb of the 10 Å path: with bare Coulomb it agrees to 0.04 %).
+    s = S.build_system([S.FROM_ONE_LETTER[c] for c in "NLYIQWLKDGGPSSGRPPPS"])
+    orc = O.Oracle(s)
+    e_b = orc.bonded_energy(s.pos)
+    assert abs(e_b["bond"] - 23494.0) < 5e-4 * 23494.0
+    e10 = orc.total_energy(s.pos, cut_nb=10.0)["total"]       # energy_eval.rs:20-34
+    assert abs(e10 - 47506.0) < 1e-2 * 47506.0
+    e_bare = orc.total_energy(s.pos)["total"]
+    assert abs(e_bare - 47506.0) < 1e-3 * 47506.0
 
 
 # ------------------------------------------------------------- SASA --

@@ -57,27 +57,29 @@ def test_nerf_reproduces_internal_coordinates():
 
 
 def test_force_field_known_answers():
-    # forcefield.rs:622-714
+    # forcefield.rs:622-714 checks these within tolerances; the values are the
+    # exact entries of par_all36m_prot.prm / top_all36_prot.rtf
     ff = S.standard_ff()
     T = S.TYPE_INDEX
-    k, r0 = ff.bond(T["CT1"], T["CT3"])
-    assert abs(k - 222.5) < 1.0 and abs(r0 - 1.538) < 0.05
-    k, th = ff.angle(T["NH1"], T["CT1"], T["C"])
-    assert k > 0 and abs(th - 110.0) < 15.0
-    eps, rmin, _, _ = ff.nonbonded_params(T["C"])
-    assert abs(eps - 0.11) < 0.005 and abs(rmin - 2.0) < 0.05
-    assert ff.nonbonded_params(T["H"])[0] < 0.1
-    assert ff.dihedral(T["HA3"], T["CT3"], T["CT2"], T["HA2"]) is not None
-    assert abs(ff.partial_charge("Ala", "N") + 0.47) < 0.01
-    assert abs(ff.partial_charge("Ala", "CA") - 0.07) < 0.01
-    assert abs(ff.partial_charge("Ala", "H") - 0.31) < 0.01
-    assert abs(ff.partial_charge("Leu", "HB2") - 0.09) < 0.01
-    assert abs(ff.partial_charge("Leu", "HB3") - 0.09) < 0.01
-    assert ff.partial_charge("Ile", "CD1") is not None
-    assert ff.partial_charge("Ile", "HD11") is not None
-    assert abs(ff.partial_charge("His", "HD1") - 0.32) < 0.02
-    assert abs(ff.partial_charge("His", "ND1") + 0.36) < 0.02
-    assert ff.improper(T["CT1"], T["C"], T["O"], T["NH1"]) is not None
+    assert ff.bond(T["CT1"], T["CT3"]) == (222.5, 1.538)          # prm: CT3 CT1 222.500 1.5380
+    assert ff.angle(T["NH1"], T["CT1"], T["C"]) == (50.0, 107.0)   # prm: NH1 CT1 C 50.000 107.0000
+    assert ff.nonbonded_params(T["C"])[:2] == (0.11, 2.0)         # prm: C 0.0 -0.110000 2.000000
+    assert ff.nonbonded_params(T["H"])[:2] == (0.046, 0.2245)     # prm: H 0.0 -0.046000 0.224500
+    # CT1 carries 1-4 parameters: -0.0320 2.000 0.0 -0.01 1.9
+    assert ff.nonbonded_params(T["CT1"]) == (0.032, 2.0, 0.01, 1.9)
+    # wildcard X CT2 CT3 X 0.1600 3 0.00
+    assert ff.dihedral(T["HA3"], T["CT3"], T["CT2"], T["HA2"]) == [(0.16, 3, 0.0)]
+    assert ff.partial_charge("Ala", "N") == -0.47
+    assert ff.partial_charge("Ala", "CA") == 0.07
+    assert ff.partial_charge("Ala", "H") == 0.31
+    assert ff.partial_charge("Leu", "HB2") == 0.09
+    assert ff.partial_charge("Leu", "HB3") == 0.09
+    assert ff.partial_charge("Ile", "CD1") == -0.27                # rtf ILE CD CT3 -0.27
+    assert ff.partial_charge("Ile", "HD11") == 0.09
+    assert ff.partial_charge("His", "HD1") == 0.32                 # HSD
+    assert ff.partial_charge("His", "ND1") == -0.36
+    # O X X C 120.0000 0 0.0000 (peptide carbonyl improper)
+    assert ff.improper(T["CT1"], T["C"], T["O"], T["NH1"]) == (120.0, 0.0)
 
 
 def test_classification_known_answers():
