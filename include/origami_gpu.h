@@ -148,6 +148,18 @@ int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, const double* t
  * og_get_status gives each system's OG_STATUS_* bits. */
 int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 
+/* og_run without device allocation.  No counterpart in origami: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The same steps as
+ * og_run, with the per-step kinetic-energy accumulators in a file-scope
+ * device array of OG_NOALLOC_KE_MAX int64 entries (one per GPU, held until
+ * the process ends) instead of a per-call allocation, and direct launches
+ * (no captured graph is instantiated).  With ke_per_step, n_steps x
+ * n_systems must not exceed OG_NOALLOC_KE_MAX (else OG_ERR_ARG).  Every
+ * call on one GPU shares that array: calls that overlap on one GPU race. */
+#define OG_NOALLOC_KE_MAX 65536
+int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step);
+
 /* og_run replays one captured hipGraph per BAOAB step (all systems, every
  * launch of the step) when profiling is off and no per-step kinetic energy
  * is requested.  on = 1: always; 0: direct launches; 2 (the default): below

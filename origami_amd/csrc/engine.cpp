@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -98,6 +99,7 @@ struct og_engine {
     int graph_mode = 2;                    // og_run: 1 captured steps, 0 direct launches, 2 by size
     bool graph_ok = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
+    long long* noalloc_ke = nullptr;       // this device's og_run_noalloc scratch array
     int pair_grid = 0;                     // autotuned pair-kernel width (0: default)
     std::vector<double> mass_h;            // host copies kept for the surrogate set-up
     std::vector<int> sys_tile_off_h;
@@ -208,6 +210,8 @@ extern "C" const char* og_last_error(void) { return g_err.c_str(); }
 
 static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys);
 
+static int noalloc_setup(og_engine* e);
+
 extern "C" int og_engine_create(const og_system_desc* systems, int32_t n_systems, og_engine** out) {
     if (!out) return fail(OG_ERR_ARG, "out is NULL");
     *out = nullptr;
@@ -244,6 +248,7 @@ extern "C" int og_engine_create(const og_system_desc* systems, int32_t n_systems
     }
     og_engine* e = new og_engine();
     int rc = engine_build(e, systems, n_systems);
+    if (rc == OG_OK) rc = noalloc_setup(e);
     if (rc != OG_OK) {
         std::string msg = g_err;
         free_all(e);
@@ -942,16 +947,11 @@ static int capture_step(og_engine* e, const og_cluster_args& c, const og_back_ar
     return OG_OK;
 }
 
-extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
-    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
-    if (!e->dyn_ready) return fail(OG_ERR_STATE, "og_dynamics_setup has not been called");
-    if (n_steps < 0) return fail(OG_ERR_ARG, "negative step count");
+// The BAOAB step loop shared by og_run and og_run_noalloc.  ke_dev (may be
+// null): n_steps x n_systems int64 kinetic-energy accumulators, zeroed by the
+// caller; captured graphs only where allow_graphs.
+static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allow_graphs) {
     const og_dynamics_params& p = e->dyn;
-    long long* ke_dev = nullptr;
-    if (ke_per_step && n_steps > 0) {
-        HIPCHK(hipMallocAsync((void**)&ke_dev, (size_t)n_steps * e->n_sys * sizeof(long long), e->stream));
-        HIPCHK(hipMemsetAsync(ke_dev, 0, (size_t)n_steps * e->n_sys * sizeof(long long), e->stream));
-    }
     og_cluster_args c{};
     c.pos = e->pos; c.vel = e->vel; c.ref = e->ref; c.force = e->force; c.facc = e->facc;
     c.iacc = e->iacc; c.mass = e->mass; c.atom_sys = e->atom_sys; c.atom_local = e->atom_local;
@@ -1000,7 +1000,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     // Captured steps pay off where launch overhead matters (small batches);
     // on large batches direct launches measured 2-3 % faster per step.
     const bool want = e->graph_mode == 1 || (e->graph_mode == 2 && e->n_atoms < OG_GRAPH_MAX_ATOMS);
-    const bool graphs = want && !e->prof && !ke_dev && n_steps >= 2;
+    const bool graphs = allow_graphs && want && !e->prof && !ke_dev && n_steps >= 2;
     if (graphs) {
         if (!e->graph_ok) {
             invalidate_graphs(e);
@@ -1020,10 +1020,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     } else {
         for (int k = 0; k < n_steps; ++k) {
             b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
-            if (int rc = record_step(e, c, b, -1, nzp)) {
-                if (ke_dev) (void)hipFreeAsync(ke_dev, e->stream);
-                return rc;
-            }
+            if (int rc = record_step(e, c, b, -1, nzp)) return rc;
             if (e->prof && (k & 63) == 63) {
                 HIPCHK(hipStreamSynchronize(e->stream));
                 prof_collect(e);
@@ -1032,30 +1029,119 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     }
     e->step += (uint32_t)n_steps;
     HIPCHK(hipGetLastError());
-    if (ke_dev) {
-        std::vector<long long> h((size_t)n_steps * e->n_sys);
-        HIPCHK(hipMemcpyAsync(h.data(), ke_dev, h.size() * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
-        HIPCHK(hipFreeAsync(ke_dev, e->stream));
-        HIPCHK(hipStreamSynchronize(e->stream));
-        for (size_t k = 0; k < h.size(); ++k) ke_per_step[k] = (double)h[k] * OG_FIX_INV;
-    }
+    return OG_OK;
+}
+
+// After a run: SASA status, then OG_ERR_NONFINITE naming the first stopped
+// system (the diverged flags are read in chunks on the stack).
+static int run_status(og_engine* e) {
     HIPCHK(hipStreamSynchronize(e->stream));
     if (e->prof) prof_collect(e);
     int ovf = 0;
-    if ((p.flags & OG_SASA) && og_sasa_overflow(&e->sasa, &ovf) != 0)
+    if ((e->dyn.flags & OG_SASA) && og_sasa_overflow(&e->sasa, &ovf) != 0)
         return fail(OG_ERR_HIP, "reading SASA status failed");
-    std::vector<int> div(e->n_sys);
-    HIPCHK(hipMemcpy(div.data(), e->diverged, e->n_sys * sizeof(int), hipMemcpyDeviceToHost));
-    int n_div = 0, first = -1;
-    for (int s = 0; s < e->n_sys; ++s)
-        if (div[s]) {
-            n_div++;
-            if (first < 0) first = s;
-        }
+    int n_div = 0, first = -1, first_bits = 0;
+    int chunk[256];
+    for (int s0 = 0; s0 < e->n_sys; s0 += 256) {
+        const int m = std::min(256, e->n_sys - s0);
+        HIPCHK(hipMemcpy(chunk, e->diverged + s0, (size_t)m * sizeof(int), hipMemcpyDeviceToHost));
+        for (int k = 0; k < m; ++k)
+            if (chunk[k]) {
+                n_div++;
+                if (first < 0) { first = s0 + k; first_bits = chunk[k]; }
+            }
+    }
     if (n_div)
         return fail(OG_ERR_NONFINITE, "%d system(s) stopped; system %d: %s (the others ran all steps)", n_div,
-                    first, status_reason(div[first]));
+                    first, status_reason(first_bits));
     return OG_OK;
+}
+
+// int64 fixed-point kinetic energies, copied into the caller's buffer as is,
+// converted in place to kJ/mol
+static void ke_to_double(double* ke, size_t n) {
+    for (size_t k = 0; k < n; ++k) {
+        long long v;
+        std::memcpy(&v, ke + k, sizeof v);
+        ke[k] = (double)v * OG_FIX_INV;
+    }
+}
+
+extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (!e->dyn_ready) return fail(OG_ERR_STATE, "og_dynamics_setup has not been called");
+    if (n_steps < 0) return fail(OG_ERR_ARG, "negative step count");
+    long long* ke_dev = nullptr;
+    const size_t n_ke = (size_t)n_steps * e->n_sys;
+    if (ke_per_step && n_steps > 0) {
+        HIPCHK(hipMallocAsync((void**)&ke_dev, n_ke * sizeof(long long), e->stream));
+        HIPCHK(hipMemsetAsync(ke_dev, 0, n_ke * sizeof(long long), e->stream));
+    }
+    if (int rc = run_steps(e, n_steps, ke_dev, true)) {
+        if (ke_dev) (void)hipFreeAsync(ke_dev, e->stream);
+        return rc;
+    }
+    if (ke_dev) {
+        HIPCHK(hipMemcpyAsync(ke_per_step, ke_dev, n_ke * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipFreeAsync(ke_dev, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        ke_to_double(ke_per_step, n_ke);
+    }
+    return run_status(e);
+}
+
+// ---------------------------------------------------------------------------
+// og_run_noalloc: the per-step kinetic-energy accumulators live in this
+// file-scope device array instead of a per-call allocation.  Each GPU holds
+// its own copy until the process ends; host code reaches it only through the
+// address hipGetSymbolAddress returns on that device (g_noalloc_ke[device]),
+// never through the array's name.  Every call on a GPU shares the array, so
+// calls that overlap on one GPU race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) long long scratch[OG_NOALLOC_KE_MAX];
+#define OG_MAX_DEVICES 64
+static long long* g_noalloc_ke[OG_MAX_DEVICES];
+static std::mutex g_noalloc_mu;
+
+// Resolves this device's copy of the scratch array (once per device, with
+// the device current); called from og_engine_create.
+static int noalloc_setup(og_engine* e) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= OG_MAX_DEVICES) return fail(OG_ERR_STATE, "device %d beyond the supported range", dev);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ke[dev]) {
+        void* ptr = nullptr;
+        const hipError_t er = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (er != hipSuccess || !ptr)
+            return fail(OG_ERR_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(er));
+        g_noalloc_ke[dev] = (long long*)ptr;
+    }
+    e->noalloc_ke = g_noalloc_ke[dev];
+    return OG_OK;
+}
+
+extern "C" int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (!e->dyn_ready) return fail(OG_ERR_STATE, "og_dynamics_setup has not been called");
+    if (n_steps < 0) return fail(OG_ERR_ARG, "negative step count");
+    if (!e->noalloc_ke) return fail(OG_ERR_STATE, "the engine's scratch array is not set up");
+    long long* ke_dev = nullptr;
+    const size_t n_ke = (size_t)n_steps * e->n_sys;
+    if (ke_per_step && n_steps > 0) {
+        if (n_ke > (size_t)OG_NOALLOC_KE_MAX)
+            return fail(OG_ERR_ARG, "n_steps x n_systems = %zu exceeds the %d kinetic-energy entries of og_run_noalloc",
+                        n_ke, OG_NOALLOC_KE_MAX);
+        ke_dev = e->noalloc_ke;
+        HIPCHK(hipMemsetAsync(ke_dev, 0, n_ke * sizeof(long long), e->stream));
+    }
+    if (int rc = run_steps(e, n_steps, ke_dev, false)) return rc;
+    if (ke_dev) {
+        HIPCHK(hipMemcpyAsync(ke_per_step, ke_dev, n_ke * sizeof(long long), hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        ke_to_double(ke_per_step, n_ke);
+    }
+    return run_status(e);
 }
 
 extern "C" int og_use_graphs(og_engine* e, int32_t on) {

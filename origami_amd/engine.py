@@ -101,6 +101,7 @@ def lib():
         L.og_born_radii.argtypes = [vp, _dp, _dp]
         L.og_dynamics_setup.argtypes = [vp, C.POINTER(DynamicsParams), _dp]
         L.og_run.argtypes = [vp, C.c_int32, _dp]
+        L.og_run_noalloc.argtypes = [vp, C.c_int32, _dp]
         L.og_set_temperature.argtypes = [vp, C.c_int32, C.c_double]
         L.og_swap_systems.argtypes = [vp, C.c_int32, C.c_int32]
         L.og_get_status.argtypes = [vp, _ip, _ip, _ip]
@@ -292,14 +293,17 @@ class Engine:
         self.dt_fs = dt_fs
         _check(lib().og_dynamics_setup(self._h, C.byref(p), _d(t)))
 
-    def run(self, n_steps, record_ke=False, allow_stopped=False):
+    def run(self, n_steps, record_ke=False, allow_stopped=False, noalloc=False):
         """n_steps BAOAB steps of every system.  A system whose velocities turn
         non-finite (or whose forces leave the fixed-point range, or whose SASA
         exceeds capacity) stops at that step while the others keep running
         (remd.rs:382-389); that raises unless allow_stopped, and status()
-        names the stopped systems."""
+        names the stopped systems.  noalloc: og_run_noalloc (no device
+        allocation; per-step kinetic energies in the library's fixed scratch
+        array of OG_NOALLOC_KE_MAX entries, direct launches)."""
         ke = np.zeros((n_steps, self.n_systems)) if record_ke and n_steps > 0 else None
-        rc = lib().og_run(self._h, int(n_steps), _d(ke) if ke is not None else None)
+        fn = lib().og_run_noalloc if noalloc else lib().og_run
+        rc = fn(self._h, int(n_steps), _d(ke) if ke is not None else None)
         if rc == -4:
             if allow_stopped:
                 return ke
