@@ -242,6 +242,51 @@ def pmc_probe(E):
     return 0
 
 
+# The reference's published timings (README.md:406-410, release build on an
+# Apple M3 Pro): total_force per step with the 10 Å integrator cutoffs, and
+# simulated time per wall day at dt = 2 fs with SHAKE.
+REFERENCE_SYSTEMS = [("1UAO_chignolin", 134, 0.27, 500.0), ("1L2Y_model1", 300, 1.12, 150.0),
+                     ("2F4K_villin_hp35", 520, 2.76, 60.0)]
+
+
+def fixtures_leg(E, n_calls=200, n_steps=2000):
+    """og_forces (origami's cutoffs, host buffers in and out: the entry-point
+    replacement of total_force_with_scratch) and device-resident dynamics
+    (dt 2 fs, SHAKE, 300 K) on the reference's own PDB systems."""
+    from origami_amd import pdb as P
+    from origami_amd import system as S
+    from origami_amd.minimize import minimize
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
+    out = []
+    for name, atoms, ref_ms, ref_ns_day in REFERENCE_SYSTEMS:
+        s = P.system_from_residues(P.read_pdb(os.path.join(golden, name + ".pdb")))
+        eng = E.Engine([s])
+        x = s.pos.copy()
+        for _ in range(5):
+            eng.total_force(x, mode_a=True)
+        t0 = time.perf_counter()
+        for _ in range(n_calls):
+            eng.total_force(x, mode_a=True)
+        ms_call = (time.perf_counter() - t0) * 1e3 / n_calls
+        # dynamics from the relaxed entry (the deposited coordinates carry
+        # close contacts that a 2 fs step turns into a blow-up)
+        minimize(eng, max_steps=200, mode_a=True)
+        eng.set_velocities(E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(7)))
+        eng.dynamics_setup([300.0], dt_fs=2.0, friction_ps_inv=2.0, seed=7, constrain_h_bonds=True,
+                           mode_a=True)
+        eng.run(100)
+        t0 = time.perf_counter()
+        eng.run(n_steps)
+        ms_step = (time.perf_counter() - t0) * 1e3 / n_steps
+        ns_day = 2.0 * 1e-6 / (ms_step * 1e-3) * 86400.0
+        out.append({"system": name, "atoms": s.n_atoms, "og_forces_ms_per_call": ms_call,
+                    "reference_total_force_ms_m3pro": ref_ms, "dynamics_ms_per_step": ms_step,
+                    "dynamics_ns_per_day": ns_day, "reference_ns_per_day_m3pro": ref_ns_day})
+        log("fixture %s: og_forces %.3f ms/call (reference %.2f), dynamics %.4f ms/step = %.0f ns/day "
+            "(reference %.0f)" % (name, ms_call, ref_ms, ms_step, ns_day, ref_ns_day))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -258,6 +303,8 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short workload for hardware-counter runs: a 512 MiB device copy (the "
                          "calibration kernel) and three force evaluations of the 64-chain batch")
+    ap.add_argument("--no-fixtures", action="store_true",
+                    help="skip the timing of the reference's own PDB systems (1UAO, 1L2Y, 2F4K)")
     ap.add_argument("--profile-steps", type=int, default=20,
                     help="steps of the separate profiled run that gives the kernel breakdown")
     args = ap.parse_args()
@@ -404,6 +451,8 @@ def main():
     if not args.no_surrogate and args.config == 4:
         out["surrogate"] = surrogate_leg(eng, systems, temps, args, dt_fs, gamma_scale, include_sasa,
                                          ns_day, pairs)
+    if not args.no_fixtures:
+        out["reference_systems"] = fixtures_leg(E)
     if not args.no_cpu_baseline:
         t0 = time.time()
         out["cpu_baseline"] = cpu_baseline(systems, temps, dt_fs, gamma_scale)

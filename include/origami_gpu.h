@@ -160,6 +160,31 @@ int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 #define OG_NOALLOC_KE_MAX 65536
 int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step);
 
+/* The force pass's tile shape (no counterpart in origami): on systems above
+ * four tiles a wave evaluates two i tiles (128 atoms) against a strip of up
+ * to `set` j tiles of 64 atoms (1..8), keeping the i-side sums in registers
+ * across the strip.  og_autotune times 2, 4 and 8 with the launch widths and
+ * keeps the fastest; set > 0 sets it directly (e.g. to restore an autotuned
+ * choice), get (optional) receives the current value.  Forces change only
+ * by FP32 rounding (the per-strip partial sums), not in their pairs. */
+int og_strip_tiles(og_engine* e, int32_t set, int32_t* get);
+
+/* Trajectory frames (langevin.rs save_every and its frame callback,
+ * :205-220, :332-341): with capacity > 0 and every > 0, og_run and
+ * og_run_noalloc copy every system's positions into a device ring of
+ * `capacity` frames after each step whose number is a multiple of `every`
+ * (a device-to-device copy in order with the steps; no host round trip).
+ * Set-up records the current positions as the first frame, as origami emits
+ * step 0.  A run that would add more frames than the ring has free slots is
+ * refused with OG_ERR_ARG before any step; og_frames_read frees them.
+ * capacity = every = 0 switches frames off. */
+int og_frames_setup(og_engine* e, int32_t capacity, int32_t every);
+
+/* The oldest unread frames, up to max_frames: xyz (max_frames x N x 3, all
+ * systems in engine atom order) and their step numbers; *n_frames receives
+ * the count.  Either output may be NULL (the frames are still consumed). */
+int og_frames_read(og_engine* e, double* xyz, uint32_t* steps, int32_t max_frames, int32_t* n_frames);
+
 /* og_run replays one captured hipGraph per BAOAB step (all systems, every
  * launch of the step) when profiling is off and no per-step kinetic energy
  * is requested.  on = 1: always; 0: direct launches; 2 (the default): below
@@ -176,7 +201,10 @@ int og_use_graphs(og_engine* e, int32_t on);
  * pair kernels), with OG_SURROGATE (surrogate enabled) on the evaluation
  * with learned radii.  choice[2] (optional) receives the pair and SASA block
  * counts, best_ms[2] the pair kernel's time alone and the best evaluation's.
- * Results do not change: accumulation is order-independent fixed point. */
+ * It also times the force pass's strip length (og_strip_tiles: 2, 4, 8 j
+ * tiles) with the pair widths and keeps the fastest pair.  Other than the
+ * strip length (FP32 rounding of the per-strip partial sums), results do not
+ * change: accumulation is order-independent fixed point. */
 int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms);
 
 /* Launch widths without timing (no counterpart in origami), e.g. to restore

@@ -25,6 +25,7 @@
 // og_run's default: captured steps below this many atoms
 #define OG_GRAPH_MAX_ATOMS 20000
 #define OG_SPLIT_MAX_TILES 4           // systems up to 256 atoms: off-diagonal tile pairs in halves
+#define OG_STRIP_DEFAULT 4             // j tiles per force strip until the autotuner picks
 
 static thread_local std::string g_err;
 
@@ -80,6 +81,11 @@ struct og_engine {
     float* qs = nullptr;
     int *tile_atom0 = nullptr, *tile_cnt = nullptr;
     og_work* work = nullptr;
+    og_strip* strips = nullptr;            // force pass: strips of large systems
+    og_work* fwork = nullptr;              // force pass: tile pairs outside the strips
+    int n_strip = 0, n_fwork = 0;
+    int strip_j = 0;                       // j tiles per strip
+    std::vector<og_work> work_h;           // host copy of the tile-pair list (strip rebuilds)
     unsigned long long* masks = nullptr;
     int nb = 0, na = 0, nd = 0, ni = 0;
     int *bonds = nullptr, *angles = nullptr, *dih = nullptr, *imp = nullptr;
@@ -100,6 +106,12 @@ struct og_engine {
     bool graph_ok = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
     long long* noalloc_ke = nullptr;       // this device's og_run_noalloc scratch array
+    // frame ring (og_frames_setup): every system's positions after every
+    // frame_every-th step, kept on the device until og_frames_read
+    double* frames = nullptr;              // [frame_cap][3N]
+    int frame_cap = 0, frame_every = 0;
+    long long frame_head = 0, frame_tail = 0;   // frames written / read
+    std::vector<uint32_t> frame_step;      // step number held by each ring slot
     int pair_grid = 0;                     // autotuned pair-kernel width (0: default)
     std::vector<double> mass_h;            // host copies kept for the surrogate set-up
     std::vector<int> sys_tile_off_h;
@@ -169,7 +181,7 @@ static void free_all(og_engine* e) {
                     e->rho, e->charge, e->radii, e->sasa_r, e->sasa_g, e->eps, e->rmin, e->eps14,
                     e->rmin14, e->atom_sys, e->atom_local,
                     e->atom_slot, e->par, e->gb, e->qs, e->tile_atom0, e->tile_cnt, e->work,
-                    e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
+                    e->strips, e->fwork, e->masks, e->bonds, e->angles, e->dih, e->imp, e->bond_p, e->angle_p,
                     e->bonded_tf, e->bonded_csr_start, e->bonded_csr_slot,
                     e->dih_p, e->imp_p, e->cl_start, e->cl_atoms, e->cn_start,
                     e->cn_len, e->sys_temp, e->sys_dofc, e->shake_fail, e->diverged, e->clamped,
@@ -179,6 +191,7 @@ static void free_all(og_engine* e) {
     for (auto& g : e->gexec)
         if (g) (void)hipGraphExecDestroy(g);
     if (e->step_dev) (void)hipFree(e->step_dev);
+    if (e->frames) (void)hipFree(e->frames);
     if (e->noise) (void)hipFree(e->noise);
     if (e->sur) {
         og_sur_free(e->sur);
@@ -211,6 +224,76 @@ extern "C" const char* og_last_error(void) { return g_err.c_str(); }
 static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys);
 
 static int noalloc_setup(og_engine* e);
+
+// The force pass's work lists for a strip length of strip_j j tiles.  Systems
+// above OG_SPLIT_MAX_TILES tiles: tiles in pairs (I, I + 1) for even I; the
+// pair's tile pairs with every later tile J go into strips of up to strip_j j
+// tiles, except where either tile pair holds an excluded or 1-4 pair; the
+// diagonal and adjacent tile pairs and those masked ones stay single work
+// items.  Smaller systems keep their (split) tile pairs.  Every tile pair of
+// a system is covered exactly once (checked).
+static int build_force_lists(og_engine* e, int strip_j) {
+    if (strip_j < 1 || strip_j > OG_STRIP_J) return fail(OG_ERR_ARG, "strip length must be 1..%d", OG_STRIP_J);
+    std::vector<og_work> fwork;
+    std::vector<og_strip> strips;
+    const std::vector<og_work>& work = e->work_h;
+    size_t w = 0;
+    for (int s = 0; s < e->n_sys; ++s) {
+        const int t0 = e->sys_tile_off_h[s], nt = e->sys_tile_off_h[s + 1] - t0;
+        const size_t w0 = w;
+        while (w < work.size() && work[w].sys == s) ++w;
+        if (nt <= OG_SPLIT_MAX_TILES) {
+            fwork.insert(fwork.end(), work.begin() + w0, work.begin() + w);
+            continue;
+        }
+        if ((long long)(w - w0) != (long long)nt * (nt + 1) / 2)
+            return fail(OG_ERR_STATE, "internal: system %d holds %zu tile pairs", s, w - w0);
+        auto tpi = [&](int I, int J) { return w0 + (size_t)I * nt - (size_t)I * (I - 1) / 2 + (J - I); };
+        const size_t f0 = fwork.size(), s0 = strips.size();
+        auto pushw = [&](int I, int J) { fwork.push_back(work[tpi(I, J)]); };
+        for (int I = 0; I < nt; I += 2) {
+            pushw(I, I);
+            if (I + 1 >= nt) break;
+            pushw(I + 1, I + 1);
+            pushw(I, I + 1);
+            og_strip cur{};
+            cur.ti = t0 + I;
+            cur.sys = s;
+            for (int J = I + 2; J < nt; ++J) {
+                if (work[tpi(I, J)].mask >= 0 || work[tpi(I + 1, J)].mask >= 0) {
+                    pushw(I, J);
+                    pushw(I + 1, J);
+                    continue;
+                }
+                cur.tj[cur.nj++] = t0 + J;
+                if (cur.nj == strip_j) {
+                    strips.push_back(cur);
+                    cur.nj = 0;
+                }
+            }
+            if (cur.nj > 0) strips.push_back(cur);
+        }
+        std::vector<int> cov((size_t)nt * nt, 0);
+        for (size_t k = f0; k < fwork.size(); ++k) cov[(size_t)(fwork[k].ti - t0) * nt + (fwork[k].tj - t0)]++;
+        for (size_t k = s0; k < strips.size(); ++k)
+            for (int q = 0; q < strips[k].nj; ++q)
+                for (int h = 0; h < 2; ++h) cov[(size_t)(strips[k].ti + h - t0) * nt + (strips[k].tj[q] - t0)]++;
+        for (int I = 0; I < nt; ++I)
+            for (int J = 0; J < nt; ++J)
+                if (cov[(size_t)I * nt + J] != (J >= I ? 1 : 0))
+                    return fail(OG_ERR_STATE, "internal: force work list covers tile pair (%d, %d) of system %d %d times",
+                                I, J, s, cov[(size_t)I * nt + J]);
+    }
+    if (w != work.size()) return fail(OG_ERR_STATE, "internal: tile-pair list out of system order");
+    if (e->strips) { HIPCHK(hipFree(e->strips)); e->strips = nullptr; }
+    if (e->fwork) { HIPCHK(hipFree(e->fwork)); e->fwork = nullptr; }
+    HIPCHK(upload(&e->strips, strips));
+    HIPCHK(upload(&e->fwork, fwork));
+    e->n_strip = (int)strips.size();
+    e->n_fwork = (int)fwork.size();
+    e->strip_j = strip_j;
+    return OG_OK;
+}
 
 extern "C" int og_engine_create(const og_system_desc* systems, int32_t n_systems, og_engine** out) {
     if (!out) return fail(OG_ERR_ARG, "out is NULL");
@@ -585,6 +668,8 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->clamped, zi));
     e->mass_h = mass;
     e->sys_tile_off_h = sys_tile_off;
+    e->work_h = work;
+    if (int rc = build_force_lists(e, OG_STRIP_DEFAULT)) return rc;
     int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg, sys_tile_off,
                            e->tile_atom0, e->tile_cnt, e->n_tiles);
     if (rc != OG_OK) return fail(rc, "SASA set-up failed: %s", og_sasa_error());
@@ -641,6 +726,7 @@ static og_pair_args pair_args(og_engine* e, double cutoff_a) {
     a.pos = e->pos; a.par = e->par; a.gb = e->gb; a.qs = e->qs;
     a.tile_atom0 = e->tile_atom0; a.tile_cnt = e->tile_cnt;
     a.work = e->work; a.n_work = e->n_work; a.masks = e->masks;
+    a.strips = e->strips; a.n_strip = e->n_strip; a.fwork = e->fwork; a.n_fwork = e->n_fwork;
     a.facc = e->facc; a.iacc = e->iacc; a.eacc = e->eacc;
     a.grid = e->pair_grid;
     a.sflag = e->sflag;
@@ -947,11 +1033,36 @@ static int capture_step(og_engine* e, const og_cluster_args& c, const og_back_ar
     return OG_OK;
 }
 
+// Appends the current positions of every system to the frame ring (a
+// device-to-device copy on the engine stream, in order with the steps).
+static int save_frame(og_engine* e, uint32_t step) {
+    const size_t n3 = 3 * (size_t)e->n_atoms;
+    const int slot = (int)(e->frame_head % e->frame_cap);
+    HIPCHK(hipMemcpyAsync(e->frames + (size_t)slot * n3, e->pos, n3 * sizeof(double), hipMemcpyDeviceToDevice,
+                          e->stream));
+    e->frame_step[slot] = step;
+    e->frame_head++;
+    return OG_OK;
+}
+
+// Frames the next n_steps steps add to the ring (steps that are multiples of
+// frame_every), refused up front when the ring would overflow.
+static int check_frames(og_engine* e, int32_t n_steps) {
+    if (e->frame_every <= 0) return OG_OK;
+    const long long s0 = e->step, s1 = (long long)e->step + n_steps;
+    const long long n_new = s1 / e->frame_every - s0 / e->frame_every;
+    if (e->frame_head - e->frame_tail + n_new > e->frame_cap)
+        return fail(OG_ERR_ARG, "%lld frame(s) would exceed the ring of %d (%lld unread): read frames first",
+                    n_new, e->frame_cap, e->frame_head - e->frame_tail);
+    return OG_OK;
+}
+
 // The BAOAB step loop shared by og_run and og_run_noalloc.  ke_dev (may be
 // null): n_steps x n_systems int64 kinetic-energy accumulators, zeroed by the
 // caller; captured graphs only where allow_graphs.
 static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allow_graphs) {
     const og_dynamics_params& p = e->dyn;
+    if (int rc = check_frames(e, n_steps)) return rc;
     og_cluster_args c{};
     c.pos = e->pos; c.vel = e->vel; c.ref = e->ref; c.force = e->force; c.facc = e->facc;
     c.iacc = e->iacc; c.mass = e->mass; c.atom_sys = e->atom_sys; c.atom_local = e->atom_local;
@@ -1016,11 +1127,17 @@ static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allo
                 sur->calls++;
             }
             HIPCHK(hipGraphLaunch(e->gexec[chk], e->stream));
+            const uint32_t st = e->step + (uint32_t)k + 1u;
+            if (e->frame_every > 0 && st % (uint32_t)e->frame_every == 0)
+                if (int rc = save_frame(e, st)) return rc;
         }
     } else {
         for (int k = 0; k < n_steps; ++k) {
             b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
             if (int rc = record_step(e, c, b, -1, nzp)) return rc;
+            const uint32_t st = e->step + (uint32_t)k + 1u;
+            if (e->frame_every > 0 && st % (uint32_t)e->frame_every == 0)
+                if (int rc = save_frame(e, st)) return rc;
             if (e->prof && (k & 63) == 63) {
                 HIPCHK(hipStreamSynchronize(e->stream));
                 prof_collect(e);
@@ -1071,6 +1188,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     if (!e->dyn_ready) return fail(OG_ERR_STATE, "og_dynamics_setup has not been called");
     if (n_steps < 0) return fail(OG_ERR_ARG, "negative step count");
+    if (int rc = check_frames(e, n_steps)) return rc;
     long long* ke_dev = nullptr;
     const size_t n_ke = (size_t)n_steps * e->n_sys;
     if (ke_per_step && n_steps > 0) {
@@ -1192,17 +1310,22 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
         return OG_OK;
     };
     const bool mode_a = flags & OG_MODE_A;
-    int best_grid = 0;
+    int best_grid = 0, best_strip = e->strip_j;
     float best_pair = 1e30f;
-    for (int per_cu : {2, 4, 8, 16, 32}) {
-        const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
-        og_pair_args a = pair_args(e, 10.0);
-        a.grid = g;
-        float ms = 0.0f;
-        if (int rc = timed([&] { og_launch_pair_force(a, mode_a, e->stream); }, ms)) return rc;
-        if (ms < best_pair) { best_pair = ms; best_grid = g; }
-        if (g == std::max((e->n_work + 3) / 4, 1)) break;
+    // the force pass's tile shape (j tiles per strip) with its launch width
+    for (int sj : {2, 4, 8}) {
+        if (int rc = build_force_lists(e, sj)) return rc;
+        for (int per_cu : {2, 4, 8, 16, 32}) {
+            const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
+            og_pair_args a = pair_args(e, 10.0);
+            a.grid = g;
+            float ms = 0.0f;
+            if (int rc = timed([&] { og_launch_pair_force(a, mode_a, e->stream); }, ms)) return rc;
+            if (ms < best_pair) { best_pair = ms; best_grid = g; best_strip = sj; }
+            if (g == std::max((e->n_work + 3) / 4, 1)) break;
+        }
     }
+    if (int rc = build_force_lists(e, best_strip)) return rc;
     e->pair_grid = best_grid;
     int best_sasa = og_sasa_max_blocks(&e->sasa);
     float best_s = 0.0f;
@@ -1243,6 +1366,59 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     invalidate_graphs(e);
     if (choice) { choice[0] = best_grid; choice[1] = best_sasa; }
     if (best_ms) { best_ms[0] = best_pair; best_ms[1] = best_s; }
+    return OG_OK;
+}
+
+extern "C" int og_strip_tiles(og_engine* e, int32_t set, int32_t* get) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (set > 0) {
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if (int rc = build_force_lists(e, set)) return rc;
+        invalidate_graphs(e);
+    }
+    if (get) *get = e->strip_j;
+    return OG_OK;
+}
+
+extern "C" int og_frames_setup(og_engine* e, int32_t capacity, int32_t every) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (capacity < 0 || every < 0 || (capacity > 0) != (every > 0))
+        return fail(OG_ERR_ARG, "capacity and every must both be > 0 (or both 0 to switch frames off)");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    if (e->frames) {
+        HIPCHK(hipFree(e->frames));
+        e->frames = nullptr;
+    }
+    e->frame_cap = 0;
+    e->frame_every = 0;
+    e->frame_head = e->frame_tail = 0;
+    e->frame_step.clear();
+    if (capacity == 0) return OG_OK;
+    HIPCHK(hipMalloc((void**)&e->frames, (size_t)capacity * 3 * (size_t)e->n_atoms * sizeof(double)));
+    e->frame_cap = capacity;
+    e->frame_every = every;
+    e->frame_step.assign(capacity, 0u);
+    // the frame at the current step, as langevin.rs:205-220 emits step 0
+    if (int rc = save_frame(e, e->step)) return rc;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return OG_OK;
+}
+
+extern "C" int og_frames_read(og_engine* e, double* xyz, uint32_t* steps, int32_t max_frames, int32_t* n_frames) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    if (n_frames) *n_frames = 0;
+    if (max_frames < 0) return fail(OG_ERR_ARG, "max_frames must be >= 0");
+    if (e->frame_cap == 0) return OG_OK;
+    HIPCHK(hipStreamSynchronize(e->stream));
+    const size_t n3 = 3 * (size_t)e->n_atoms;
+    int k = 0;
+    for (; k < max_frames && e->frame_tail < e->frame_head; ++k, ++e->frame_tail) {
+        const int slot = (int)(e->frame_tail % e->frame_cap);
+        if (xyz) HIPCHK(hipMemcpy(xyz + (size_t)k * n3, e->frames + (size_t)slot * n3, n3 * sizeof(double),
+                                  hipMemcpyDeviceToHost));
+        if (steps) steps[k] = e->frame_step[slot];
+    }
+    if (n_frames) *n_frames = k;
     return OG_OK;
 }
 

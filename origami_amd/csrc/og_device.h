@@ -40,6 +40,20 @@ struct og_work {              // one tile pair (I <= J) of one system
     int32_t part;             // -1: all 64 steps; 0 / 1: steps 0-31 / 32-63 (I < J only)
 };
 
+// Force work item of a large system: the two i tiles (ti, ti + 1) against
+// up to OG_STRIP_J j tiles of the same system (the engine's strip length, set
+// by the autotuner), none of these tile pairs holding an excluded or 1-4
+// pair.  One wave keeps the i-side sums of both i tiles in registers across
+// the j tiles and flushes them once.
+#define OG_STRIP_J 8
+struct og_strip {
+    int32_t ti;               // first i tile (global index); the second is ti + 1
+    int32_t nj;               // j tiles in use (1..OG_STRIP_J)
+    int32_t tj[OG_STRIP_J];   // j tiles (global indices), each > ti + 1
+    int32_t sys;              // system index
+    int32_t pad;
+};
+
 // Per-system status bits (the engine's diverged flags and the per-evaluation
 // range flags share them).
 #define OG_ST_NONFINITE 1     // non-finite velocities after the closing half-kick

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 struct og_work;
+struct og_strip;
 
 struct og_pair_args {
     const double* pos;               // [N*3] FP64 master positions (atom index space)
@@ -24,6 +25,13 @@ struct og_pair_args {
     const int* skip_sys;             // [S] descreening skipped where nonzero (may be null)
     int grid;                        // blocks of 256 threads (0: default)
     int* sflag;                      // [S] status bits (OG_ST_RANGE on a contribution out of range)
+    // force pass only: the strips of large systems (taken first), then the
+    // tile pairs that no strip covers (fwork: diagonal, adjacent and masked
+    // tile pairs, and every tile pair of small systems)
+    const og_strip* strips;
+    int n_strip;
+    const og_work* fwork;
+    int n_fwork;
 };
 
 // energy slots per system

@@ -14,7 +14,8 @@
 // j-side sums rotate.  A diagonal tile (I = J) runs steps 1..32 (step 32 on
 // lanes 0..31) and swaps the halves of the wave at the end.  Per tile pair every lane
 // flushes one FP32 partial per component into the int64 fixed-point
-// accumulators.  The FP64 energy kernel keeps v_readlane broadcasts.
+// accumulators.  The FP64 energy kernel reads its j atoms from LDS (every lane
+// the same entry: a broadcast) and accumulates the pair energies per lane.
 //
 // Physics follows origami's SoA kernels:
 //   descreening   energy/src/gb.rs:271-283 (pair), :347-369 (h)
@@ -378,6 +379,126 @@ __device__ __forceinline__ void force_tile_lds(const og_pair_args& a, const Lane
     acc.gz += gz;
 }
 
+// One i atom of a strip: position relative to the strip's origin, charge,
+// LJ and GB parameters, and its force sum over the strip.
+struct StripAtom {
+    float x, y, z, q;
+    float se, rm, R, gw;
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ StripAtom strip_atom(const LaneData& d) {
+    return {d.x, d.y, d.z, d.q, d.par.x, d.par.y, d.gb.z, d.gb.w, 0.0f, 0.0f, 0.0f};
+}
+
+// Pair coefficient c (force on i = c * d, d = x_j - x_i) of the fused
+// LJ + Coulomb + GB-pair force for an unmasked pair; j data as in the LDS
+// layout of force_tile_lds (P = x, y, z, q; Q = se, rm, R, 1/R scaled).
+// Mode B groups the two charge terms: q_i q_j (k_GB (1 - e/4) / f^3 - 1/r^3).
+template <bool MODEA>
+__device__ __forceinline__ float strip_coef(const og_pair_args& a, const StripAtom& I, float4 pj, float4 qj,
+                                            float dx, float dy, float dz) {
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    const float inv_r = __builtin_amdgcn_rsqf(r2);
+    const float inv_r2 = inv_r * inv_r;
+    const float rmin = I.rm + qj.y;
+    const float t2 = rmin * rmin * inv_r2;
+    const float t6 = t2 * t2 * t2;
+    const float lj = (I.se * qj.x) * inv_r2 * fmaf(-t6, t6, t6);
+    const float e = __builtin_amdgcn_exp2f(-r2 * (I.gw * qj.w));
+    const float f2 = fmaf(I.R * qj.z, e, r2);
+    const float inv_f = __builtin_amdgcn_rsqf(f2);
+    const float qq = I.q * pj.w;
+    const float gb = fmaf(-0.25f * kGbK, e, kGbK) * (inv_f * inv_f * inv_f);
+    if (MODEA) {
+        float cnb = fmaf(-qq, inv_r2 * inv_r - a.inv_rc3, lj);
+        cnb = (r2 <= a.rc2_nb) ? cnb : 0.0f;
+        const float gbc = (r2 <= a.rc2_gb) ? qq * gb : 0.0f;
+        return cnb + gbc;
+    }
+    return fmaf(qq, gb - inv_r2 * inv_r, lj);
+}
+
+// A strip: two i tiles (64 i atoms each, two per lane) against up to four j
+// tiles of the same system, j data in the wave's LDS as in force_tile_lds.
+// Each step evaluates the lane's two i atoms against one j atom, adds both
+// reactions into the rotating j-side sums and rotates them once, so the
+// rotation and the LDS read serve two pairs.  The i-side sums stay in
+// registers over the whole strip and are flushed once.  No pair of a strip
+// is excluded or 1-4, and no i atom is a padding slot (the i tiles precede a
+// j tile of their system, so both are full); distinct atoms at one position
+// give a non-finite coefficient, which the fixed-point conversion reports.
+template <bool MODEA>
+__device__ __forceinline__ void force_strip(const og_pair_args& a, const og_strip& st, int lane,
+                                            float4* P, float4* Q) {
+    const int ta = st.ti, tb = st.ti + 1;
+    const int a0 = a.tile_atom0[ta], b0 = a.tile_atom0[tb];
+    const int ca = a.tile_cnt[ta], cb = a.tile_cnt[tb];
+    const double ox = a.pos[3 * (size_t)a0], oy = a.pos[3 * (size_t)a0 + 1], oz = a.pos[3 * (size_t)a0 + 2];
+    StripAtom A = strip_atom(load_lane(a, ta, a0, ca, lane, ox, oy, oz));
+    StripAtom B = strip_atom(load_lane(a, tb, b0, cb, lane, ox, oy, oz));
+    int* fl = a.sflag ? a.sflag + st.sys : nullptr;
+    int tj = st.tj[0];
+    LaneData dj = load_lane(a, tj, a.tile_atom0[tj], a.tile_cnt[tj], lane, ox, oy, oz);
+    for (int q = 0; q < st.nj; ++q) {
+        const int a0j = a.tile_atom0[tj], cj = a.tile_cnt[tj];
+        const float4 pjl = make_float4(dj.x, dj.y, dj.z, dj.q);
+        const float4 qjl = make_float4(dj.par.x, dj.par.y, dj.gb.z, dj.gb.w);
+        wave_lds_order();                       // the previous j tile's reads are done
+        P[lane] = pjl;
+        Q[lane] = qjl;
+        if (lane == 0) { P[OG_TILE] = pjl; Q[OG_TILE] = qjl; }
+        wave_lds_order();
+        // the next j tile's data load while this one is evaluated
+        const int tn = st.tj[q + 1 < st.nj ? q + 1 : q];
+        dj = load_lane(a, tn, a.tile_atom0[tn], a.tile_cnt[tn], lane, ox, oy, oz);
+        float gx = 0.0f, gy = 0.0f, gz = 0.0f;
+        auto step = [&](float4 pj, float4 qj) {
+            const float dxa = pj.x - A.x, dya = pj.y - A.y, dza = pj.z - A.z;
+            const float dxb = pj.x - B.x, dyb = pj.y - B.y, dzb = pj.z - B.z;
+            const float c_a = strip_coef<MODEA>(a, A, pj, qj, dxa, dya, dza);
+            const float c_b = strip_coef<MODEA>(a, B, pj, qj, dxb, dyb, dzb);
+            A.fx = fmaf(c_a, dxa, A.fx);
+            A.fy = fmaf(c_a, dya, A.fy);
+            A.fz = fmaf(c_a, dza, A.fz);
+            B.fx = fmaf(c_b, dxb, B.fx);
+            B.fy = fmaf(c_b, dyb, B.fy);
+            B.fz = fmaf(c_b, dzb, B.fz);
+            gx = rot(fmaf(-c_b, dxb, fmaf(-c_a, dxa, gx)));     // after step 63: home
+            gy = rot(fmaf(-c_b, dyb, fmaf(-c_a, dya, gy)));
+            gz = rot(fmaf(-c_b, dzb, fmaf(-c_a, dza, gz)));
+        };
+        for (int k = 0; k < OG_TILE; k += 2) {
+            const int idx = (lane + k) & 63;
+            const float4 p0 = P[idx], q0 = Q[idx];
+            const float4 p1 = P[idx + 1], q1 = Q[idx + 1];
+            step(p0, q0);
+            step(p1, q1);
+        }
+        if (lane < cj) {
+            long long* f = a.facc + 3 * (size_t)(a0j + lane);
+            og_atomic_add_fix(f + 0, og_to_fix_chk((double)gx, fl));
+            og_atomic_add_fix(f + 1, og_to_fix_chk((double)gy, fl));
+            og_atomic_add_fix(f + 2, og_to_fix_chk((double)gz, fl));
+        }
+        tj = tn;
+    }
+    if (lane < ca) {
+        long long* f = a.facc + 3 * (size_t)(a0 + lane);
+        og_atomic_add_fix(f + 0, og_to_fix_chk((double)A.fx, fl));
+        og_atomic_add_fix(f + 1, og_to_fix_chk((double)A.fy, fl));
+        og_atomic_add_fix(f + 2, og_to_fix_chk((double)A.fz, fl));
+    }
+    if (lane < cb) {
+        long long* f = a.facc + 3 * (size_t)(b0 + lane);
+        og_atomic_add_fix(f + 0, og_to_fix_chk((double)B.fx, fl));
+        og_atomic_add_fix(f + 1, og_to_fix_chk((double)B.fy, fl));
+        og_atomic_add_fix(f + 2, og_to_fix_chk((double)B.fz, fl));
+    }
+}
+
+// The force pass: the strips first (the heavier items), then the remaining
+// tile pairs (fwork), grid-stride over both.
 template <bool MODEA>
 __device__ __forceinline__ void force_body(const og_pair_args& a) {
     __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
@@ -386,8 +507,14 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * blockDim.x) >> 6;
-    for (int w = wave; w < a.n_work; w += n_waves) {
-        const og_work it = a.work[w];
+    const int total = a.n_strip + a.n_fwork;
+    for (int w = wave; w < total; w += n_waves) {
+        if (w < a.n_strip) {
+            const og_strip st = a.strips[w];
+            force_strip<MODEA>(a, st, lane, P, Q);
+            continue;
+        }
+        const og_work it = a.fwork[w - a.n_strip];
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],
@@ -466,8 +593,9 @@ void og_launch_descreen_bonded(const og_pair_args& a, const og_bonded_args& b, b
 }
 
 void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
-    if (a.n_work <= 0) return;
-    const int g = a.grid > 0 ? a.grid : pair_grid(a.n_work);
+    const int n = a.n_strip + a.n_fwork;
+    if (n <= 0) return;
+    const int g = a.grid > 0 ? a.grid : pair_grid(n);
     if (mode_a) hipLaunchKernelGGL(og_k_force_a, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_force_b, dim3(g), dim3(256), 0, s, a);
 }
@@ -482,15 +610,31 @@ void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
 // ---------------------------------------------------------------------------
 namespace {
 
-__device__ __forceinline__ double rld(double v, int j) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), j);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), j);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+// 1/sqrt(x) for x > 0 in FP64: the hardware estimate refined by two Newton
+// steps (each doubles the number of correct bits), in place of a correctly
+// rounded square root and division.
+__device__ __forceinline__ double rsqrt64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    double h = x * y;
+    double e = fma(-h, y, 1.0);
+    y = fma(0.5 * y, e, y);
+    h = x * y;
+    e = fma(-h, y, 1.0);
+    return fma(0.5 * y, e, y);
 }
+
+// j atom data for the FP64 energy pass, one entry per j atom of the tile in
+// the wave's LDS: every lane reads the same entry in a step (a broadcast).
+struct EJ {
+    double x, y, z, q;
+    double rm, se, rm14, se14;     // Rmin/2, sqrt(eps), 1-4 variants
+    double R, iR;                  // Born radius and its reciprocal
+};
 
 template <bool MODEA>
 __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
+    __shared__ EJ s_j[4][OG_TILE];
+    EJ* J = s_j[threadIdx.x >> 6];
     const int lane = threadIdx.x & 63;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int n_waves = (gridDim.x * blockDim.x) >> 6;
@@ -499,51 +643,55 @@ __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
     for (int w = wave; w < a.n_work; w += n_waves) {
         const og_work it = a.work[w];
         if (it.part == 1) continue;                 // a split tile pair is evaluated whole once
+        if (a.skip_sys && a.skip_sys[it.sys]) continue;
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const bool vi = lane < ci, vj = lane < cj;
         const int gi = a0i + (vi ? lane : 0), gj = a0j + (vj ? lane : 0);
         const double xi = a.pos[3 * (size_t)gi], yi = a.pos[3 * (size_t)gi + 1], zi = a.pos[3 * (size_t)gi + 2];
-        const double xjv = a.pos[3 * (size_t)gj], yjv = a.pos[3 * (size_t)gj + 1], zjv = a.pos[3 * (size_t)gj + 2];
-        const double qi = e.charge[gi], qjv = e.charge[gj];
-        const double epi = e.eps[gi], rmi = e.rmin[gi], ep14i = e.eps14[gi], rm14i = e.rmin14[gi];
-        const double epjv = e.eps[gj], rmjv = e.rmin[gj], ep14jv = e.eps14[gj], rm14jv = e.rmin14[gj];
-        const double Ri = e.radii[gi], Rjv = e.radii[gj];
+        const double qi = e.charge[gi];
+        const double rmi = e.rmin[gi], sei = sqrt(e.eps[gi]);
+        const double rm14i = e.rmin14[gi], se14i = sqrt(e.eps14[gi]);
+        const double Ri = e.radii[gi], iRi = 1.0 / Ri;
         unsigned long long xm = 0, m14 = 0;
         if (it.mask >= 0) {
             xm = a.masks[(size_t)it.mask * 128 + lane];
             m14 = a.masks[(size_t)it.mask * 128 + 64 + lane];
         }
+        EJ mine;
+        mine.x = a.pos[3 * (size_t)gj]; mine.y = a.pos[3 * (size_t)gj + 1]; mine.z = a.pos[3 * (size_t)gj + 2];
+        mine.q = e.charge[gj];
+        mine.rm = e.rmin[gj]; mine.se = sqrt(e.eps[gj]);
+        mine.rm14 = e.rmin14[gj]; mine.se14 = sqrt(e.eps14[gj]);
+        mine.R = e.radii[gj]; mine.iR = 1.0 / mine.R;
+        wave_lds_order();                           // the previous tile's reads are done
+        J[lane] = mine;
+        wave_lds_order();
+        const bool diag = it.ti == it.tj;
         double elj = 0.0, ec = 0.0, eg = 0.0;
         for (int jj = 0; jj < cj; ++jj) {
-            const double dx = xi - rld(xjv, jj), dy = yi - rld(yjv, jj), dz = zi - rld(zjv, jj);
-            const double r2 = dx * dx + dy * dy + dz * dz;
-            bool ok = vi;
-            if (it.ti == it.tj) ok = ok && (jj > lane);
-            if (!ok) continue;
-            const double qq = qi * rld(qjv, jj);
-            const bool excl = (xm >> jj) & 1ull;
-            bool nb = !excl;
-            if (MODEA) nb = nb && (r2 <= (double)e.rc2_nb);
-            if (nb) {
-                const bool one4 = (m14 >> jj) & 1ull;
-                const double rmij = one4 ? rm14i + rld(rm14jv, jj) : rmi + rld(rmjv, jj);
-                const double eij = sqrt(one4 ? ep14i * rld(ep14jv, jj) : epi * rld(epjv, jj));
-                const double r = sqrt(r2);
-                const double t = rmij / r;
-                const double t2 = t * t;
-                const double t6 = t2 * t2 * t2;
-                elj += eij * (t6 * t6 - 2.0 * t6);
-                if (qq != 0.0) {
-                    if (MODEA) ec += qq * (1.0 / r + e.kappa * r * r - e.crf);
-                    else ec += qq * (1.0 / r);
-                }
-            }
-            if (qq != 0.0) {
-                const double rp = Ri * rld(Rjv, jj);
-                const double f = sqrt(r2 + rp * exp(-r2 / (4.0 * rp)));
-                eg += qq / f;
-            }
+            const EJ d = J[jj];
+            const double dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
+            const bool ok = vi && (!diag || jj > lane);
+            const double r2 = ok ? fma(dx, dx, fma(dy, dy, dz * dz)) : 1.0;
+            const double qq = qi * d.q;
+            const bool one4 = (m14 >> jj) & 1ull;
+            bool nb = ok && !((xm >> jj) & 1ull);
+            if (MODEA) nb = nb && (r2 <= e.rc2_nb);
+            const double inv_r = rsqrt64(r2);
+            const double rmij = one4 ? rm14i + d.rm14 : rmi + d.rm;
+            const double eij = one4 ? se14i * d.se14 : sei * d.se;
+            const double t = rmij * inv_r;
+            const double t2 = t * t;
+            const double t6 = t2 * t2 * t2;
+            const double lj = eij * fma(t6, t6, -2.0 * t6);
+            double c = qq * inv_r;
+            if (MODEA) c = qq * (inv_r + e.kappa * r2 - e.crf);
+            elj += nb ? lj : 0.0;
+            ec += nb ? c : 0.0;
+            const double rp = Ri * d.R;
+            const double f2 = fma(rp, exp(-0.25 * r2 * (iRi * d.iR)), r2);
+            eg += ok ? qq * rsqrt64(f2) : 0.0;
         }
         elj = og_wave_sum_d(elj * OG_KCAL_TO_KJ);
         ec = og_wave_sum_d(ec * kc);

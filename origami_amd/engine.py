@@ -102,6 +102,9 @@ def lib():
         L.og_dynamics_setup.argtypes = [vp, C.POINTER(DynamicsParams), _dp]
         L.og_run.argtypes = [vp, C.c_int32, _dp]
         L.og_run_noalloc.argtypes = [vp, C.c_int32, _dp]
+        L.og_frames_setup.argtypes = [vp, C.c_int32, C.c_int32]
+        L.og_strip_tiles.argtypes = [vp, C.c_int32, _ip]
+        L.og_frames_read.argtypes = [vp, _dp, C.POINTER(C.c_uint32), C.c_int32, C.POINTER(C.c_int32)]
         L.og_set_temperature.argtypes = [vp, C.c_int32, C.c_double]
         L.og_swap_systems.argtypes = [vp, C.c_int32, C.c_int32]
         L.og_get_status.argtypes = [vp, _ip, _ip, _ip]
@@ -328,7 +331,14 @@ class Engine:
                                  | (SURROGATE if surrogate else 0),
                                  _i(ch), _d(ms)))
         return {"pair_blocks": int(ch[0]), "sasa_blocks": int(ch[1]), "pair_ms": float(ms[0]),
-                "sasa_ms": float(ms[1])}
+                "sasa_ms": float(ms[1]), "strip_tiles": self.strip_tiles()}
+
+    def strip_tiles(self, n=0):
+        """The force pass's strip length (j tiles per wave on systems above
+        four tiles); n > 0 sets it.  Returns the current value."""
+        out = np.zeros(1, np.int32)
+        _check(lib().og_strip_tiles(self._h, int(n), _i(out)))
+        return int(out[0])
 
     def set_temperature(self, system, t):
         _check(lib().og_set_temperature(self._h, int(system), float(t)))
@@ -336,6 +346,25 @@ class Engine:
 
     def swap(self, a, b):
         _check(lib().og_swap_systems(self._h, int(a), int(b)))
+
+    def frames_setup(self, capacity, every):
+        """Keep every system's positions after every `every`-th step in a
+        device ring of `capacity` frames (the current positions are the
+        first frame); capacity = every = 0 switches frames off."""
+        _check(lib().og_frames_setup(self._h, int(capacity), int(every)))
+        self._frame_cap = int(capacity)
+
+    def read_frames(self):
+        """(steps, positions [n_frames, N, 3]) of the unread frames, oldest
+        first."""
+        cap = getattr(self, "_frame_cap", 0)
+        if cap == 0:
+            return [], np.zeros((0, self.n_atoms, 3))
+        xyz = np.zeros((cap, self.n_atoms, 3))
+        steps = (C.c_uint32 * cap)()
+        n = C.c_int32()
+        _check(lib().og_frames_read(self._h, _d(xyz), steps, cap, C.byref(n)))
+        return [int(steps[k]) for k in range(n.value)], xyz[:n.value]
 
     def status(self):
         n = self.n_systems
@@ -445,25 +474,47 @@ class LangevinSummary:
     atoms_count: int
     diverged: bool
     shake_failures: int
+    # (step, positions [N, 3]) every save_every steps from step 0 (the
+    # frames the reference hands its callback, langevin.rs:205-220, :332-341)
+    frames: List = field(default_factory=list)
 
 
 def run_langevin(sys: S.ResolvedSystem, steps: int, dt_fs=1.0, temperature_k=310.0,
                  friction_ps_inv=1.0, seed=0, randomise_initial_velocities=True,
                  constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0,
-                 mode_a=False, engine: Optional[Engine] = None):
-    """BAOAB Langevin for one system on the GPU (langevin.rs:128-367)."""
+                 mode_a=False, engine: Optional[Engine] = None, save_every=0, frame_chunk=256):
+    """BAOAB Langevin for one system on the GPU (langevin.rs:128-367).
+    save_every > 0 keeps a frame every save_every steps (and step 0) in the
+    engine's device frame ring, read back once per frame_chunk frames."""
     eng = engine or Engine([sys])
     if randomise_initial_velocities:
         eng.set_velocities(maxwell_boltzmann(sys, temperature_k, S.Xoshiro256pp(seed)))
     eng.dynamics_setup([temperature_k], dt_fs=dt_fs, friction_ps_inv=friction_ps_inv, seed=seed,
                        constrain_h_bonds=constrain_h_bonds, mode_a=mode_a,
                        include_sasa=include_sasa, sasa_gamma_scale=sasa_gamma_scale)
+    frames = []
+    if save_every > 0:
+        eng.frames_setup(frame_chunk + 1, save_every)
     diverged = False
     try:
-        ke = eng.run(steps, record_ke=True)
+        if save_every > 0:
+            kes = []
+            done = 0
+            while done < steps:
+                n = min(steps - done, frame_chunk * save_every)
+                kes.append(eng.run(n, record_ke=True))
+                done += n
+                st_, xs = eng.read_frames()
+                frames.extend(zip(st_, xs))
+            ke = np.concatenate(kes)
+        else:
+            ke = eng.run(steps, record_ke=True)
     except EngineError:
         diverged = True
         ke = np.zeros((0, 1))
+        if save_every > 0:
+            st_, xs = eng.read_frames()
+            frames.extend(zip(st_, xs))
     st = eng.status()
     t = kinetic_temperature(ke[:, 0], sys.n_atoms) if len(ke) else np.zeros(0)
     ke_final = float(ke[-1, 0]) if len(ke) else 0.0
@@ -473,7 +524,8 @@ def run_langevin(sys: S.ResolvedSystem, steps: int, dt_fs=1.0, temperature_k=310
         temperature_stddev_k=float(t.std()) if len(t) else 0.0,
         equipartition_ratio=ke_final / (0.5 * dof * S.BOLTZMANN_KJ * temperature_k),
         final_kinetic_energy_kj_mol=ke_final, atoms_count=sys.n_atoms,
-        diverged=diverged or bool(st["diverged"][0]), shake_failures=int(st["shake_failures"][0]))
+        diverged=diverged or bool(st["diverged"][0]), shake_failures=int(st["shake_failures"][0]),
+        frames=frames)
 
 
 @dataclass

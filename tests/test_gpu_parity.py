@@ -349,13 +349,26 @@ def test_graph_replay_matches_eager(E, relaxed):
 
 
 def test_autotune_keeps_results(E, c2_system):
+    """Launch widths leave forces bit-identical; the strip length (the force
+    pass's tile shape) changes only the FP32 rounding of per-strip sums."""
     eng = E.Engine([c2_system])
+    s0 = eng.strip_tiles()
     f0 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
     ch = eng.autotune(include_sasa=True)
     print("autotune", ch)
+    assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0 and ch["strip_tiles"] in (2, 4, 8)
+    eng.strip_tiles(s0)
     f1 = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
     assert np.array_equal(f0, f1)
-    assert ch["pair_blocks"] > 0 and ch["sasa_blocks"] > 0
+    fref = O.Oracle(c2_system).total_force(c2_system.pos) + O.Oracle(c2_system).sasa_forces(
+        c2_system.pos, 0.25)[0]
+    for sj in (1, 2, 3, 8):
+        eng.strip_tiles(sj)
+        f = eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
+        assert rel_rms(f, f0) < 1e-6, sj
+        assert rel_rms(f, fref) < FORCE_RMS_TOL, sj
+    with pytest.raises(E.EngineError, match="strip length"):
+        eng.strip_tiles(9)
 
 
 def test_sasa_side_stream_matches_serial(E, relaxed):
