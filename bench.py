@@ -255,7 +255,7 @@ def fixtures_leg(E, n_calls=200, n_steps=2000):
     (dt 2 fs, SHAKE, 300 K) on the reference's own PDB systems."""
     from origami_amd import pdb as P
     from origami_amd import system as S
-    from origami_amd.minimize import minimize
+    from origami_amd.minimize import minimize_device
     golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden")
     out = []
     for name, atoms, ref_ms, ref_ns_day in REFERENCE_SYSTEMS:
@@ -270,7 +270,7 @@ def fixtures_leg(E, n_calls=200, n_steps=2000):
         ms_call = (time.perf_counter() - t0) * 1e3 / n_calls
         # dynamics from the relaxed entry (the deposited coordinates carry
         # close contacts that a 2 fs step turns into a blow-up)
-        minimize(eng, max_steps=200, mode_a=True)
+        minimize_device(eng, max_steps=200, mode_a=True)
         eng.set_velocities(E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(7)))
         eng.dynamics_setup([300.0], dt_fs=2.0, friction_ps_inv=2.0, seed=7, constrain_h_bonds=True,
                            mode_a=True)
@@ -315,7 +315,7 @@ def main():
         return 2
 
     from origami_amd import engine as E
-    from origami_amd.minimize import minimize
+    from origami_amd.minimize import minimize_device
     if args.pmc_probe:
         return pmc_probe(E)
 
@@ -343,9 +343,10 @@ def main():
     info = eng.info()
     log("engine:", info)
     t0 = time.time()
-    res = minimize(eng, max_steps=args.minimize_steps, mode_a=False)
-    log("L-BFGS relax (%d steps max): %.1f s, energy %.3e -> %.3e" % (
-        args.minimize_steps, time.time() - t0, sum(r.initial_energy for r in res),
+    res = minimize_device(eng, max_steps=args.minimize_steps, mode_a=False)
+    relax_s = time.time() - t0
+    log("L-BFGS relax on the device (%d steps max): %.2f s, energy %.3e -> %.3e" % (
+        args.minimize_steps, relax_s, sum(r.initial_energy for r in res),
         sum(r.final_energy for r in res)))
     from origami_amd import system as S
     v = [E.maxwell_boltzmann(s, T, S.Xoshiro256pp(1000 + k)) for k, (s, T) in
@@ -438,6 +439,9 @@ def main():
                          "launches, one kernel at a time (the timed run has SASA on a second "
                          "stream beside the pair kernels)" % args.profile_steps,
         "autotune": tune,
+        "relax": {"minimiser": "og_minimize (device-resident batched L-BFGS, all pairs)",
+                  "max_steps": args.minimize_steps, "seconds": relax_s,
+                  "steps_max_over_systems": max(r.steps for r in res)},
         "pair_kernels": pk,
         "roofline": roof,
         "dominant_kernel": dominant,

@@ -160,6 +160,42 @@ int og_run(og_engine* e, int32_t n_steps, double* ke_per_step);
 #define OG_NOALLOC_KE_MAX 65536
 int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step);
 
+/* Batched L-BFGS energy minimisation on the device, every system of the
+ * engine independently (lbfgs, dynamics/src/lbfgs.rs:61-254, with
+ * backtracking, line_search.rs:52-101, and minimize.rs:27-38's defaults):
+ * history ring, two-loop recursion, Armijo line search on FP64 energies,
+ * steepest-descent retry after a failed search and stop after a second,
+ * origami's convergence tests.  Forces are the engine's (FP32 pairs, int64
+ * accumulation), energies its FP64 totals; flags OG_MODE_A (origami's
+ * minimiser uses DEFAULT_CUTOFF_A = 10 Å, energy_eval.rs:20-50) and OG_SASA.
+ * Positions are updated in place; results (optional) get one entry per
+ * system.  The control state stays on the device; between phases the host
+ * reads four counters. */
+typedef struct og_minimize_params {
+    int32_t max_steps;              /* 500 */
+    int32_t history;                /* 10 (1..32) */
+    double gradient_tol;            /* 1.0 kJ/mol/Å, |g|inf */
+    double energy_tol;              /* 0.01 kJ/mol */
+    double max_step_a;              /* 0.1 Å: caps alpha0 by max |p| */
+    double alpha0;                  /* 1.0 */
+    double contraction;             /* 0.5 */
+    double c1;                      /* 1e-4 */
+    double min_alpha;               /* 1e-12 */
+    uint32_t flags;                 /* OG_MODE_A, OG_SASA */
+    double cutoff_a;                /* LJ / Coulomb cutoff under OG_MODE_A (10 Å) */
+    double sasa_gamma_scale;        /* SASA force scale under OG_SASA */
+} og_minimize_params;
+
+typedef struct og_minimize_result {
+    int32_t steps;                  /* LbfgsResult.steps */
+    int32_t converged;
+    int32_t line_search_failures;
+    int32_t stopped;                /* 1: both searches of a step failed */
+    double initial_energy, final_energy, max_force;
+} og_minimize_result;
+
+int og_minimize(og_engine* e, const og_minimize_params* p, og_minimize_result* results);
+
 /* The force pass's tile shape (no counterpart in origami): on systems above
  * four tiles a wave evaluates two i tiles (128 atoms) against a strip of up
  * to `set` j tiles of 64 atoms (1..8), keeping the i-side sums in registers

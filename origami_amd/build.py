@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "liborigami_gpu.so")
 
-SOURCES = ["engine.cpp", "pair_kernels.hip", "md_kernels.hip", "sasa_kernel.hip", "surrogate.hip"]
+SOURCES = ["engine.cpp", "pair_kernels.hip", "md_kernels.hip", "sasa_kernel.hip", "surrogate.hip",
+           "lbfgs.hip"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

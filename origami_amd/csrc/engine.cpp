@@ -106,6 +106,7 @@ struct og_engine {
     bool graph_ok = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
     long long* noalloc_ke = nullptr;       // this device's og_run_noalloc scratch array
+    int* eval_skip = nullptr;              // [S] systems an energy-only evaluation leaves out (minimiser probes)
     // frame ring (og_frames_setup): every system's positions after every
     // frame_every-th step, kept on the device until og_frames_read
     double* frames = nullptr;              // [frame_cap][3N]
@@ -821,7 +822,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
     {
         ProfScope p(e, OG_K_DESCREEN);
         og_pair_args ad = a;
-        ad.skip_sys = (sur && !check) ? sur->mode : nullptr;
+        ad.skip_sys = (sur && !check) ? sur->mode : (!forces ? e->eval_skip : nullptr);
         if (fuse) og_launch_descreen_bonded(ad, b, mode_a, e->stream);
         else og_launch_pair_descreen(ad, mode_a, e->stream);
     }
@@ -852,7 +853,9 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
         en.rc2_nb = cutoff_a * cutoff_a;
         en.kappa = 1.0 / (2.0 * cutoff_a * cutoff_a * cutoff_a);
         en.crf = 3.0 / (2.0 * cutoff_a);
-        og_launch_pair_energy64(a, en, mode_a, e->stream);
+        og_pair_args ae = a;
+        ae.skip_sys = forces ? nullptr : e->eval_skip;
+        og_launch_pair_energy64(ae, en, mode_a, e->stream);
     }
     if (fork) HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
     if (!fork && (flags & OG_SASA)) {
@@ -1367,6 +1370,134 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
     if (choice) { choice[0] = best_grid; choice[1] = best_sasa; }
     if (best_ms) { best_ms[0] = best_pair; best_ms[1] = best_s; }
     return OG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// og_minimize: the batched L-BFGS of lbfgs.hip driven phase by phase; the
+// host reads four counters between phases (how many systems search, failed,
+// are active, accepted) and launches only the phases some system needs.
+// ---------------------------------------------------------------------------
+extern "C" int og_minimize(og_engine* e, const og_minimize_params* prm, og_minimize_result* results) {
+    if (!e || !prm) return fail(OG_ERR_ARG, "NULL argument");
+    const og_minimize_params& P = *prm;
+    if (P.max_steps < 0) return fail(OG_ERR_ARG, "max_steps must be >= 0");
+    if (P.history < 1 || P.history > OG_LB_MAX_HISTORY)
+        return fail(OG_ERR_ARG, "history must be 1..%d", OG_LB_MAX_HISTORY);
+    if (!(P.alpha0 > 0.0) || !(P.contraction > 0.0 && P.contraction < 1.0) || !(P.max_step_a > 0.0) ||
+        !(P.min_alpha > 0.0) || !(P.c1 >= 0.0) || !(P.gradient_tol >= 0.0) || !(P.energy_tol >= 0.0))
+        return fail(OG_ERR_ARG, "minimiser parameters out of range");
+    if (P.flags & ~(OG_MODE_A | OG_SASA)) return fail(OG_ERR_ARG, "og_minimize takes OG_MODE_A and OG_SASA only");
+    if (int rc = check_cutoff(P.flags, P.cutoff_a)) return rc;
+    const int S = e->n_sys;
+    const size_t n3 = 3 * (size_t)e->n_atoms;
+    // workspace: one allocation, carved
+    const size_t hist_n = (size_t)P.history * n3;
+    const size_t bytes_d = (3 * n3 + 2 * hist_n + (size_t)S * OG_LB_MAX_HISTORY) * sizeof(double);
+    const size_t bytes_s = (size_t)S * sizeof(og_lb_sys);
+    const size_t bytes_i = ((size_t)S + 1 + (size_t)S + 4) * sizeof(int);
+    char* ws = nullptr;
+    HIPCHK(hipMalloc((void**)&ws, bytes_d + bytes_s + bytes_i));
+    og_lbfgs_args a{};
+    a.n_sys = S;
+    a.n_atoms = e->n_atoms;
+    a.history = P.history;
+    double* d = (double*)ws;
+    a.x0 = d; a.g = d + n3; a.p = d + 2 * n3;
+    a.hs = d + 3 * n3; a.hy = a.hs + hist_n;
+    a.rho = a.hy + hist_n;
+    a.sys = (og_lb_sys*)(ws + bytes_d);
+    int* ip = (int*)(ws + bytes_d + bytes_s);
+    a.atom_off = ip; a.skip = ip + S + 1; a.counters = ip + 2 * S + 1;
+    a.pos = e->pos; a.force = e->force; a.eacc = e->eacc;
+    a.gradient_tol = P.gradient_tol; a.energy_tol = P.energy_tol; a.max_step = P.max_step_a;
+    a.alpha0 = P.alpha0; a.contraction = P.contraction; a.c1 = P.c1; a.min_alpha = P.min_alpha;
+    std::vector<int> off(e->atom_off.begin(), e->atom_off.end());
+    off.push_back(e->n_atoms);
+    off.resize(S + 1);
+    int rc = OG_OK;
+    auto run = [&]() -> int {
+        HIPCHK(hipMemcpy(ip, off.data(), (S + 1) * sizeof(int), hipMemcpyHostToDevice));
+        auto energies = [&](bool probes) -> int {
+            HIPCHK(hipMemsetAsync(e->eacc, 0, (size_t)S * OG_NE * sizeof(long long), e->stream));
+            HIPCHK(hipMemsetAsync(e->sflag, 0, (size_t)S * sizeof(int), e->stream));
+            og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
+            e->eval_skip = probes ? a.skip : nullptr;
+            const int r = launch_forces(e, P.flags | OG_ENERGY, P.cutoff_a, P.sasa_gamma_scale, true, -1, false);
+            e->eval_skip = nullptr;
+            return r;
+        };
+        auto forces = [&]() -> int {
+            HIPCHK(hipMemsetAsync(e->sflag, 0, (size_t)S * sizeof(int), e->stream));
+            og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
+            if (int r = launch_forces(e, P.flags & ~OG_ENERGY, P.cutoff_a, P.sasa_gamma_scale, false, -1, true)) return r;
+            og_back_args b{};
+            b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+            b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = 0.0; b.do_kick = 0;
+            b.diverged = e->diverged; b.sflag = e->sflag;
+            og_launch_back(b, e->stream);
+            return OG_OK;
+        };
+        int cnt[4] = {0, 0, 0, 0};
+        auto read_counts = [&](int phase) -> int {
+            og_launch_lbfgs(phase, a, 0, e->stream);
+            HIPCHK(hipMemcpyAsync(cnt, a.counters, sizeof cnt, hipMemcpyDeviceToHost, e->stream));
+            HIPCHK(hipStreamSynchronize(e->stream));
+            return OG_OK;
+        };
+        auto probes = [&]() -> int {
+            while (cnt[0] > 0) {
+                og_launch_lbfgs(OG_LB_K_TRIAL, a, 0, e->stream);
+                if (int r = energies(true)) return r;
+                if (int r = read_counts(OG_LB_K_ACCEPT)) return r;
+            }
+            return OG_OK;
+        };
+        if (int r = energies(false)) return r;
+        og_launch_lbfgs(OG_LB_K_INIT, a, 0, e->stream);
+        if (P.max_steps > 0)
+            if (int r = forces()) return r;
+        for (int step = 0; step < P.max_steps; ++step) {
+            og_launch_lbfgs(OG_LB_K_BEGIN, a, step, e->stream);
+            if (int r = read_counts(OG_LB_K_COUNT)) return r;
+            if (cnt[0] == 0 && cnt[1] == 0) break;          // every system converged or stopped
+            if (int r = probes()) return r;
+            if (cnt[1] > 0) {                               // steepest-descent retries
+                og_launch_lbfgs(OG_LB_K_FALLBACK, a, 0, e->stream);
+                if (int r = read_counts(OG_LB_K_COUNT)) return r;
+                if (int r = probes()) return r;
+                if (cnt[1] > 0) {
+                    og_launch_lbfgs(OG_LB_K_FALLBACK, a, 0, e->stream);
+                    if (int r = read_counts(OG_LB_K_COUNT)) return r;
+                }
+            }
+            if (cnt[3] > 0) {
+                if (int r = forces()) return r;
+                og_launch_lbfgs(OG_LB_K_UPDATE, a, step, e->stream);
+            }
+        }
+        HIPCHK(hipGetLastError());
+        std::vector<og_lb_sys> st(S);
+        HIPCHK(hipMemcpyAsync(st.data(), a.sys, bytes_s, hipMemcpyDeviceToHost, e->stream));
+        HIPCHK(hipStreamSynchronize(e->stream));
+        if (results)
+            for (int s = 0; s < S; ++s) {
+                og_minimize_result& r = results[s];
+                r.steps = st[s].steps;
+                r.converged = st[s].state == OG_LB_CONVERGED ? 1 : 0;
+                r.line_search_failures = st[s].failures;
+                r.stopped = st[s].state == OG_LB_STOPPED ? 1 : 0;
+                r.initial_energy = st[s].e_init;
+                r.final_energy = st[s].e_prev;
+                r.max_force = st[s].max_force;
+            }
+        return OG_OK;
+    };
+    rc = run();
+    e->eval_skip = nullptr;
+    (void)hipStreamSynchronize(e->stream);
+    (void)hipFree(ws);
+    invalidate_graphs(e);
+    return rc;
 }
 
 extern "C" int og_strip_tiles(og_engine* e, int32_t set, int32_t* get) {

@@ -165,3 +165,49 @@ void og_launch_back(const og_back_args& a, hipStream_t s);
 void og_launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
 void og_launch_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
                     double scale_a, double scale_b, hipStream_t s);
+
+// ---------------------------------------------------------------------------
+// Device-resident batched L-BFGS (lbfgs.hip); one block per system.
+// ---------------------------------------------------------------------------
+#define OG_LB_MAX_HISTORY 32
+enum { OG_LB_ACTIVE = 0, OG_LB_SEARCHING, OG_LB_ACCEPTED, OG_LB_FAILED, OG_LB_CONVERGED, OG_LB_STOPPED };
+enum { OG_LB_K_INIT = 0, OG_LB_K_BEGIN, OG_LB_K_TRIAL, OG_LB_K_ACCEPT, OG_LB_K_COUNT, OG_LB_K_FALLBACK,
+       OG_LB_K_UPDATE };
+
+struct og_lb_sys {                 // one system's minimiser state
+    int state, phase;              // OG_LB_*; 0 = L-BFGS direction, 1 = steepest-descent retry
+    int count, head;               // history entries and the ring slot of the oldest
+    int steps, failures;           // origami's LbfgsResult fields
+    int pad0, pad1;
+    double alpha, gdotp;           // current step length and p.g
+    double e_now, e_prev, e_new;   // energies at x0, at the last accepted point, at the trial
+    double e_init, max_force;
+};
+
+struct og_lbfgs_args {
+    int n_sys, n_atoms, history;
+    const int* atom_off;           // [S+1]
+    double* pos;                   // engine positions (trial points are written here)
+    const double* force;           // engine forces (FP64, after the last force evaluation)
+    const long long* eacc;         // [S*OG_NE] energies of the last evaluation
+    double *x0, *g, *p;            // [3N] each
+    double *hs, *hy;               // [history][3N] each
+    double* rho;                   // [S][OG_LB_MAX_HISTORY]
+    og_lb_sys* sys;                // [S]
+    int* skip;                     // [S] 1 = leave out of the next energy evaluation
+    int* counters;                 // [4] searching, failed, active, accepted
+    double gradient_tol, energy_tol, max_step, alpha0, contraction, c1, min_alpha;
+};
+void og_launch_lbfgs(int phase, const og_lbfgs_args& a, int step, hipStream_t s);
+
+// A system's total energy from its fixed-point terms, summed as the host sums
+// them: bonded four, LJ, Coulomb, GB (self + pair), SASA.
+__host__ __device__ inline double og_lb_total_energy(const long long* acc) {
+#pragma clang fp contract(off)
+    const double k = 1.0 / 4294967296.0;
+    const double gb = (double)acc[OG_E_GB_SELF] * k + (double)acc[OG_E_GB_PAIR] * k;
+    return (double)acc[OG_E_BOND] * k + (double)acc[OG_E_ANGLE] * k + (double)acc[OG_E_DIHEDRAL] * k +
+           (double)acc[OG_E_IMPROPER] * k + (double)acc[OG_E_LJ] * k + (double)acc[OG_E_COUL] * k + gb +
+           (double)acc[OG_E_SASA] * k;
+}
+

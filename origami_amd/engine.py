@@ -76,6 +76,20 @@ class EngineError(RuntimeError):
     pass
 
 
+class MinimizeParams(C.Structure):
+    _fields_ = [("max_steps", C.c_int32), ("history", C.c_int32), ("gradient_tol", C.c_double),
+                ("energy_tol", C.c_double), ("max_step_a", C.c_double), ("alpha0", C.c_double),
+                ("contraction", C.c_double), ("c1", C.c_double), ("min_alpha", C.c_double),
+                ("flags", C.c_uint32), ("cutoff_a", C.c_double), ("sasa_gamma_scale", C.c_double)]
+
+
+class MinimizeResult(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("converged", C.c_int32),
+                ("line_search_failures", C.c_int32), ("stopped", C.c_int32),
+                ("initial_energy", C.c_double), ("final_energy", C.c_double),
+                ("max_force", C.c_double)]
+
+
 _lib = None
 
 
@@ -104,6 +118,7 @@ def lib():
         L.og_run_noalloc.argtypes = [vp, C.c_int32, _dp]
         L.og_frames_setup.argtypes = [vp, C.c_int32, C.c_int32]
         L.og_strip_tiles.argtypes = [vp, C.c_int32, _ip]
+        L.og_minimize.argtypes = [vp, C.POINTER(MinimizeParams), C.POINTER(MinimizeResult)]
         L.og_frames_read.argtypes = [vp, _dp, C.POINTER(C.c_uint32), C.c_int32, C.POINTER(C.c_int32)]
         L.og_set_temperature.argtypes = [vp, C.c_int32, C.c_double]
         L.og_swap_systems.argtypes = [vp, C.c_int32, C.c_int32]

@@ -52,6 +52,26 @@ def _two_loop(g, hs, hy, hr):
     return -r
 
 
+def minimize_device(engine, max_steps=500, gradient_tol=1.0, energy_tol=0.01, max_step_a=0.1,
+                    history=10, mode_a=True, alpha0=1.0, contraction=0.5, c1=1e-4,
+                    min_alpha=1e-12, include_sasa=False, sasa_gamma_scale=1.0,
+                    cutoff_a=10.0) -> List[LbfgsResult]:
+    """The same minimiser run on the device (og_minimize): every system's
+    L-BFGS state, history and line search stay in device memory; the host
+    only sequences the phases.  Same arguments and results as minimize()."""
+    import ctypes as C
+    from . import engine as E
+    p = E.MinimizeParams(int(max_steps), int(history), gradient_tol, energy_tol, max_step_a, alpha0,
+                         contraction, c1, min_alpha,
+                         (E.MODE_A if mode_a else 0) | (E.SASA if include_sasa else 0), cutoff_a,
+                         sasa_gamma_scale)
+    out = (E.MinimizeResult * engine.n_systems)()
+    E._check(E.lib().og_minimize(engine._h, C.byref(p), out))
+    return [LbfgsResult(int(r.steps), float(r.final_energy), float(r.initial_energy),
+                        float(r.max_force), bool(r.converged), int(r.line_search_failures))
+            for r in out]
+
+
 def minimize(engine, max_steps=500, gradient_tol=1.0, energy_tol=0.01, max_step_a=0.1,
              history=10, mode_a=True, alpha0=1.0, contraction=0.5, c1=1e-4,
              min_alpha=1e-12) -> List[LbfgsResult]:
