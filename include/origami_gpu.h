@@ -45,6 +45,9 @@ enum og_status {
 #define OG_SASA 2u          /* add analytical (PowerSasa) hydrophobic SASA forces */
 #define OG_ENERGY 4u        /* also accumulate per-system energy terms */
 #define OG_SURROGATE 8u     /* Born radii from the learned surrogate (og_surrogate_*) */
+#define OG_SASA_SR 16u      /* og_energies only: the SASA energy term by Shrake-Rupley dot
+                               density (960 dots, sasa.rs:113-178), the term origami's REMD
+                               adds to its swap energies (remd.rs:529-539); not with OG_SASA */
 
 /* Per-system energy terms written by og_forces / og_energies
  * (n_systems x OG_N_ENERGY doubles, kJ/mol). */

@@ -54,6 +54,11 @@ int orc_psa_atom_gradient(int n, const double *pos, const double *radii, int ato
 double orc_psa_area(int n, const double *pos, const double *radii, double *per_atom);
 double orc_psa_area_from_caps(int nc, const double *axes, const double *cos_a, double radius);
 
+/* Shrake-Rupley (oracle_sasa.c) */
+void orc_sr_dots(int n, double *out);
+double orc_sr_energy(int n, const double *pos, const double *radii, const double *gamma, int n_dots,
+                     double *per_atom);
+
 #ifdef __cplusplus
 }
 #endif

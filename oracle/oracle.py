@@ -67,6 +67,9 @@ def lib():
         L.orc_psa_atom_gradient.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, ip, dp]
         L.orc_psa_area.restype = C.c_double
         L.orc_psa_area.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_sr_dots.argtypes = [C.c_int, dp]
+        L.orc_sr_energy.restype = C.c_double
+        L.orc_sr_energy.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp]
         L.orc_psa_area_from_caps.restype = C.c_double
         L.orc_psa_area_from_caps.argtypes = [C.c_int, dp, dp, C.c_double]
         _lib = L
@@ -283,3 +286,20 @@ def psa_area_from_caps(axes, cos_a, radius):
     a, ap = _d(np.asarray(axes, dtype=np.float64).reshape(-1, 3))
     c, cp = _d(cos_a)
     return lib().orc_psa_area_from_caps(len(c), ap, cp, radius)
+
+
+def sr_dots(n):
+    out, op = _d(np.zeros((n, 3)))
+    lib().orc_sr_dots(n, op)
+    return out
+
+
+def sr_energy(pos, radii, gamma, n_dots=960):
+    """Shrake-Rupley energy and per-atom areas (sasa.rs:113-178)."""
+    p, pp = _d(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    r, rp = _d(radii)
+    g, gp = _d(gamma)
+    area, ap = _d(np.zeros(len(p)))
+    e = lib().orc_sr_energy(len(p), pp, rp, gp, n_dots, ap)
+    return e, area
+

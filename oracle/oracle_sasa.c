@@ -828,3 +828,65 @@ double orc_psa_area_from_caps(int nc, const double *axes, const double *cos_a, d
     free(caps);
     return a;
 }
+
+/* ------------------------------------------------------------------------ */
+/* Shrake-Rupley dot-density SASA energy, energy/src/sasa.rs:60-178         */
+/* ------------------------------------------------------------------------ */
+/* fibonacci_unit_sphere, sasa.rs:182-199: y = 1 - 2 i / (n - 1), r =
+ * sqrt(1 - y^2), theta = pi (sqrt 3 + 1) i, (cos theta r, y, sin theta r). */
+void orc_sr_dots(int n, double *out) {
+    const double phi = M_PI * (sqrt(3.0) + 1.0);
+    const double nf = (double)n;
+    for (int i = 0; i < n; ++i) {
+        const double y = 1.0 - 2.0 * (double)i / (nf - 1.0);
+        const double r = sqrt(1.0 - y * y);
+        const double th = phi * (double)i;
+        out[3 * i] = cos(th) * r;
+        out[3 * i + 1] = y;
+        out[3 * i + 2] = sin(th) * r;
+    }
+}
+
+/* sasa_energy_with_dots, sasa.rs:113-178: per atom, the dots of its expanded
+ * sphere not strictly inside any neighbour's (neighbours: distance <=
+ * R_i + R_j); area = accessible / n_dots * 4 pi R^2; energy = sum gamma_i A_i
+ * (gamma in kJ/mol/Å^2 here; origami sums in kcal and converts once).
+ * per_atom (may be NULL) receives the areas; returns the energy. */
+double orc_sr_energy(int n, const double *pos, const double *radii, const double *gamma, int n_dots,
+                     double *per_atom) {
+    double *dots = (double *)malloc(sizeof(double) * 3 * (size_t)(n_dots > 0 ? n_dots : 1));
+    orc_sr_dots(n_dots, dots);
+    int *nb = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    double energy = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double *pi = pos + 3 * i;
+        const double ri = radii[i];
+        int m = 0;
+        for (int j = 0; j < n; ++j) {
+            if (j == i) continue;
+            const double dx = pos[3 * j] - pi[0], dy = pos[3 * j + 1] - pi[1], dz = pos[3 * j + 2] - pi[2];
+            if (sqrt(dx * dx + dy * dy + dz * dz) <= ri + radii[j]) nb[m++] = j;
+        }
+        int accessible = 0;
+        for (int k = 0; k < n_dots; ++k) {
+            const double tx = pi[0] + dots[3 * k] * ri, ty = pi[1] + dots[3 * k + 1] * ri,
+                         tz = pi[2] + dots[3 * k + 2] * ri;
+            int buried = 0;
+            for (int q = 0; q < m; ++q) {
+                const int j = nb[q];
+                const double ex = tx - pos[3 * j], ey = ty - pos[3 * j + 1], ez = tz - pos[3 * j + 2];
+                if (ex * ex + ey * ey + ez * ez < radii[j] * radii[j]) {
+                    buried = 1;
+                    break;
+                }
+            }
+            accessible += !buried;
+        }
+        const double area = (double)accessible / (double)n_dots * 4.0 * M_PI * ri * ri;
+        if (per_atom) per_atom[i] = area;
+        energy += gamma[i] * area;
+    }
+    free(dots);
+    free(nb);
+    return energy;
+}

@@ -26,6 +26,7 @@
 #define OG_GRAPH_MAX_ATOMS 20000
 #define OG_SPLIT_MAX_TILES 4           // systems up to 256 atoms: off-diagonal tile pairs in halves
 #define OG_STRIP_DEFAULT 4             // j tiles per force strip until the autotuner picks
+#define OG_SR_DOTS 960                 // Shrake-Rupley dots per atom (SHRAKE_RUPLEY_DOTS, sasa.rs:27)
 
 static thread_local std::string g_err;
 
@@ -107,6 +108,7 @@ struct og_engine {
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
     long long* noalloc_ke = nullptr;       // this device's og_run_noalloc scratch array
     int* eval_skip = nullptr;              // [S] systems an energy-only evaluation leaves out (minimiser probes)
+    double* sr_dots = nullptr;             // [OG_SR_DOTS][3] Shrake-Rupley unit-sphere dots
     // frame ring (og_frames_setup): every system's positions after every
     // frame_every-th step, kept on the device until og_frames_read
     double* frames = nullptr;              // [frame_cap][3N]
@@ -193,6 +195,7 @@ static void free_all(og_engine* e) {
         if (g) (void)hipGraphExecDestroy(g);
     if (e->step_dev) (void)hipFree(e->step_dev);
     if (e->frames) (void)hipFree(e->frames);
+    if (e->sr_dots) (void)hipFree(e->sr_dots);
     if (e->noise) (void)hipFree(e->noise);
     if (e->sur) {
         og_sur_free(e->sur);
@@ -674,6 +677,20 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     int rc = og_sasa_build(&e->sasa, e->n_atoms, a_sys, e->atom_off, sr, sg, sys_tile_off,
                            e->tile_atom0, e->tile_cnt, e->n_tiles);
     if (rc != OG_OK) return fail(rc, "SASA set-up failed: %s", og_sasa_error());
+    // Shrake-Rupley dots: the Fibonacci spiral of sasa.rs:182-199
+    {
+        std::vector<double> dots(3 * OG_SR_DOTS);
+        const double phi = M_PI * (std::sqrt(3.0) + 1.0);
+        for (int i = 0; i < OG_SR_DOTS; ++i) {
+            const double y = 1.0 - 2.0 * (double)i / ((double)OG_SR_DOTS - 1.0);
+            const double r = std::sqrt(1.0 - y * y);
+            const double th = phi * (double)i;
+            dots[3 * i] = std::cos(th) * r;
+            dots[3 * i + 1] = y;
+            dots[3 * i + 2] = std::sin(th) * r;
+        }
+        HIPCHK(upload(&e->sr_dots, dots));
+    }
     HIPCHK(hipStreamSynchronize(e->stream));
     HIPCHK(hipDeviceSynchronize());
     return OG_OK;
@@ -856,6 +873,14 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
         og_pair_args ae = a;
         ae.skip_sys = forces ? nullptr : e->eval_skip;
         og_launch_pair_energy64(ae, en, mode_a, e->stream);
+        if (flags & OG_SASA_SR) {
+            const int rc = og_sasa_sr_launch(&e->sasa, e->pos, e->sr_dots, OG_SR_DOTS, e->stream, e->eacc,
+                                             e->atom_sys, e->sflag);
+            if (rc != OG_OK) {
+                if (fork) (void)hipStreamWaitEvent(e->stream, e->join_ev, 0);
+                return fail(OG_ERR_HIP, "Shrake-Rupley launch failed: %s", og_sasa_error());
+            }
+        }
     }
     if (fork) HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
     if (!fork && (flags & OG_SASA)) {
@@ -869,7 +894,7 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
 }
 
 static int check_sasa_overflow(og_engine* e, uint32_t flags) {
-    if (!(flags & OG_SASA)) return OG_OK;
+    if (!(flags & (OG_SASA | OG_SASA_SR))) return OG_OK;
     int ovf = 0;
     if (og_sasa_overflow(&e->sasa, &ovf) != 0) return fail(OG_ERR_HIP, "reading SASA status failed");
     if (ovf) return fail(OG_ERR_ARG, "%d SASA atom evaluations exceeded the per-atom capacity "
@@ -900,6 +925,10 @@ static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t fla
         return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
     int rc;
     if ((rc = check_cutoff(flags, cutoff_a))) return rc;
+    if (flags & OG_SASA_SR) {
+        if (forces) return fail(OG_ERR_ARG, "OG_SASA_SR is an energy term only (og_energies)");
+        if (flags & OG_SASA) return fail(OG_ERR_ARG, "OG_SASA and OG_SASA_SR select two SASA energies; pass one");
+    }
     if (xyz && (rc = og_set_positions(e, xyz))) return rc;
     const bool energy = (flags & OG_ENERGY) || energies;
     if (energy) HIPCHK(hipMemsetAsync(e->eacc, 0, (size_t)e->n_sys * OG_NE * sizeof(long long), e->stream));

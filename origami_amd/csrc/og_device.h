@@ -32,6 +32,7 @@
 #define OG_F_SASA 2u          // add analytical SASA forces
 #define OG_F_ENERGY 4u        // accumulate energies
 #define OG_F_SURROGATE 8u     // Born radii from the learned surrogate
+#define OG_F_SASA_SR 16u      // energies only: Shrake-Rupley SASA energy term
 
 struct og_work {              // one tile pair (I <= J) of one system
     int32_t ti, tj;           // global tile indices

@@ -70,6 +70,10 @@ void og_sasa_free(og_sasa_data* d);
 // atom_sys and sflag (engine-owned) are required.
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
                    hipStream_t s, long long* eacc, const int* atom_sys, int* sflag);
+// Shrake-Rupley energy (sasa.rs:113-178): gamma_i * A_i of every SASA atom into
+// each system's SASA energy slot, from n_dots unit-sphere dots (device, 3 per dot).
+int og_sasa_sr_launch(og_sasa_data* d, const double* pos, const double* dots, int n_dots, hipStream_t s,
+                      long long* eacc, const int* atom_sys, int* sflag);
 // Arrangement-kernel width (blocks of 2 waves); at most the staging slots allocated.
 int og_sasa_max_blocks(const og_sasa_data* d);
 void og_sasa_set_blocks(og_sasa_data* d, int blocks);

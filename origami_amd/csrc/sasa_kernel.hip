@@ -1458,6 +1458,110 @@ void og_sasa_free(og_sasa_data* d) {
     *d = og_sasa_data{};
 }
 
+// ---------------------------------------------------------------------------
+// Shrake-Rupley dot-density SASA energy (energy/src/sasa.rs:113-178): the
+// swap energy of origami's REMD with SASA (remd.rs:529-539).  One wave per
+// SASA atom (gamma != 0; the others add nothing to the energy): the atoms
+// whose expanded spheres reach it (distance <= R_i + R_j, from the bounding
+// spheres' near tiles, ballot-compacted into LDS), then lane k tests dots
+// k, k + 64, ...: a dot is buried when it lies strictly inside a
+// neighbour's expanded sphere.  Area = accessible / n_dots * 4 pi R^2, in
+// origami's evaluation order and without contraction, so the dot counts are
+// the reference arithmetic's.
+// ---------------------------------------------------------------------------
+#define OG_SR_MAX_NB 256
+__global__ void __launch_bounds__(256) og_k_sasa_sr(og_sasa_data d, const double* pos, const double* dots,
+                                                    int n_dots) {
+#pragma clang fp contract(off)
+    __shared__ double4 s_nb[4][OG_SR_MAX_NB];
+    double4* nb = s_nb[threadIdx.x >> 6];
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    for (int w = wave; w < d.n_work; w += n_waves) {
+        const int i = d.work[w];
+        const int sys = d.atom_sys[i];
+        const double ri = d.radius[i];
+        const double xi = pos[3 * (size_t)i], yi = pos[3 * (size_t)i + 1], zi = pos[3 * (size_t)i + 2];
+        int n = 0;
+        bool over = false;
+        for (int t = d.tile_begin[w]; t < d.tile_end[w]; ++t) {
+            const double4 b = d.tile_bound[t];
+            const double bx = b.x - xi, by = b.y - yi, bz = b.z - zi;
+            const double reach = ri + d.max_radius + b.w;
+            if (bx * bx + by * by + bz * bz > reach * reach) continue;
+            const int j = d.tile_atom0[t] + lane;
+            bool ok = lane < d.tile_cnt[t] && j != i;
+            double4 q = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (ok) {
+                q.x = pos[3 * (size_t)j];
+                q.y = pos[3 * (size_t)j + 1];
+                q.z = pos[3 * (size_t)j + 2];
+                const double rj = d.radius[j];
+                const double dx = q.x - xi, dy = q.y - yi, dz = q.z - zi;
+                ok = sqrt(dx * dx + dy * dy + dz * dz) <= ri + rj;
+                q.w = rj * rj;
+            }
+            const unsigned long long m = __ballot(ok);
+            const int slot = n + __popcll(m & ((1ull << lane) - 1ull));
+            if (ok && slot < OG_SR_MAX_NB) nb[slot] = q;
+            n += __popcll(m);
+        }
+        if (n > OG_SR_MAX_NB) {
+            over = true;
+            n = OG_SR_MAX_NB;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        int acc = 0;
+        for (int k = lane; k < n_dots; k += 64) {
+            const double tx = xi + dots[3 * k] * ri, ty = yi + dots[3 * k + 1] * ri, tz = zi + dots[3 * k + 2] * ri;
+            bool buried = false;
+            for (int q = 0; q < n && !buried; ++q) {
+                const double4 c = nb[q];
+                const double ex = tx - c.x, ey = ty - c.y, ez = tz - c.z;
+                buried = ex * ex + ey * ey + ez * ez < c.w;
+            }
+            acc += buried ? 0 : 1;
+        }
+        for (int off = 32; off >= 1; off >>= 1) acc += __shfl_xor(acc, off, 64);
+        if (lane == 0) {
+            if (over) {
+                atomicAdd(d.overflow, 1);
+                atomicOr(d.sflag + sys, OG_ST_SASA);
+            } else {
+                const double area = (double)acc / (double)n_dots * 4.0 * 3.14159265358979323846 * ri * ri;
+                og_atomic_add_fix(d.eacc + (size_t)sys * OG_NE + OG_E_SASA,
+                                  og_to_fix_chk(d.gamma[i] * area, d.sflag + sys));
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
+int og_sasa_sr_launch(og_sasa_data* d, const double* pos, const double* dots, int n_dots, hipStream_t s,
+                      long long* eacc, const int* atom_sys, int* sflag) {
+    if (d->n_work == 0) return 0;
+    if (!eacc || !atom_sys || !sflag || !dots || n_dots < 2) {
+        g_sasa_err = "Shrake-Rupley: eacc, atom_sys, sflag and at least two dots are required";
+        return -1;
+    }
+    d->eacc = eacc;
+    d->atom_sys = atom_sys;
+    d->sflag = sflag;
+    hipLaunchKernelGGL(og_k_tile_bounds, dim3(d->n_tiles > 0 ? d->n_tiles : 1), dim3(64), 0, s, pos,
+                       d->tile_atom0, d->tile_cnt, d->n_tiles, d->tile_bound, nullptr);
+    int blocks = (d->n_work + 3) / 4;
+    if (blocks > 8 * d->n_cu) blocks = 8 * d->n_cu;
+    hipLaunchKernelGGL(og_k_sasa_sr, dim3(blocks > 0 ? blocks : 1), dim3(256), 0, s, *d, pos, dots, n_dots);
+    if (hipGetLastError() != hipSuccess) {
+        g_sasa_err = "kernel launch failed";
+        return -2;
+    }
+    return 0;
+}
+
 int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double gamma_scale,
                    hipStream_t s, long long* eacc, const int* atom_sys, int* sflag) {
     const bool area = eacc != nullptr;

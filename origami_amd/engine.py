@@ -31,6 +31,7 @@ MODE_A = 1
 SASA = 2
 ENERGY = 4
 SURROGATE = 8
+SASA_SR = 16
 
 ENERGY_TERMS = ["bond", "angle", "dihedral", "improper", "lj", "coulomb", "gb_self", "gb_pair",
                 "sasa", "kinetic"]
@@ -269,16 +270,24 @@ class Engine:
         return f
 
     def total_energy(self, xyz=None, mode_a=False, surrogate=False, include_sasa=False,
-                     cutoff_a=DEFAULT_CUTOFF_A, allow_nonfinite=False):
+                     cutoff_a=DEFAULT_CUTOFF_A, allow_nonfinite=False, sasa_method="powersasa"):
         """Per-system energy terms.  allow_nonfinite: a system whose sums left
-        the fixed-point range gets NaN terms instead of an exception."""
+        the fixed-point range gets NaN terms instead of an exception.
+        include_sasa with sasa_method "powersasa" (analytical areas,
+        powersasa_energy) or "shrake_rupley" (960-dot areas, sasa_energy: the
+        term origami's REMD swaps use)."""
+        if include_sasa and sasa_method not in ("powersasa", "shrake_rupley"):
+            raise ValueError("sasa_method must be 'powersasa' or 'shrake_rupley'")
+        sasa_flag = 0
+        if include_sasa:
+            sasa_flag = SASA if sasa_method == "powersasa" else SASA_SR
         e = np.zeros((self.n_systems, len(ENERGY_TERMS)))
         x = None
         if xyz is not None:
             x = np.ascontiguousarray(xyz, dtype=np.float64).reshape(self.n_atoms, 3)
         rc = lib().og_energies(self._h, _d(x) if x is not None else None,
                                (MODE_A if mode_a else 0) | (SURROGATE if surrogate else 0)
-                               | (SASA if include_sasa else 0), float(cutoff_a), _d(e))
+                               | sasa_flag, float(cutoff_a), _d(e))
         if not (allow_nonfinite and rc == -4):
             _check(rc)
         out = []
@@ -563,11 +572,22 @@ def geometric_ladder(t_min, t_max, n):
     return [t_min * (t_max / t_min) ** (k / (n - 1)) for k in range(n)]
 
 
+def swap_energies(eng: Engine, include_sasa: bool):
+    """remd::potential_energy of every replica (remd.rs:529-539)."""
+    return [e["total"] for e in eng.total_energy(mode_a=True, include_sasa=include_sasa,
+                                                 sasa_method="shrake_rupley", allow_nonfinite=True)]
+
+
 def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs: float,
              swap_interval_fs: float, dt_fs=1.0, friction_ps_inv=2.0, seed=0,
-             constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0, mode_a=False,
+             constrain_h_bonds=False, include_sasa=False, sasa_gamma_scale=1.0, mode_a=True,
              engine: Optional[Engine] = None, velocities=None):
     """Parallel tempering with every replica in one device batch (remd.rs:136-498).
+
+    Forces use origami's integrator cutoffs by default (mode_a: remd.rs:209-214
+    passes DEFAULT_CUTOFF_A); swap energies are remd::potential_energy
+    (remd.rs:529-539): bonded + 10 A LJ / reaction-field Coulomb + GB, plus,
+    with include_sasa, the Shrake-Rupley SASA energy (960 dots).
 
     velocities: initial velocities of all replicas, (n_replicas * N, 3) in
     ladder order (a restart); by default Maxwell-Boltzmann draws from each
@@ -578,8 +598,7 @@ def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs
     Metropolis rule on alternating adjacent pairs, then exchange positions,
     velocities and forces with the sqrt(T_a/T_b) velocity rescale.  A replica
     that diverges stops while the others keep running, and pairs with a
-    stopped replica are not attempted (remd.rs:382-389, :430).  With
-    include_sasa the swap energies carry the SASA term (remd.rs:533-537).
+    stopped replica are not attempted (remd.rs:382-389, :430).
     """
     temps = sorted(float(t) for t in temperatures)
     n_rep = len(temps)
@@ -606,8 +625,7 @@ def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs
             break
         eng.run(n, allow_stopped=True)
         stopped = eng.status()["diverged"] != 0
-        energies = [e["total"] for e in eng.total_energy(mode_a=mode_a, include_sasa=include_sasa,
-                                                          allow_nonfinite=True)]
+        energies = swap_energies(eng, include_sasa)
         for pair in range(rnd % 2, n_rep - 1, 2):
             a, b = pair, pair + 1
             if stopped[a] or stopped[b]:
@@ -621,7 +639,6 @@ def run_remd(sys: S.ResolvedSystem, temperatures: Sequence[float], total_time_fs
                 accepts[pair] += 1
                 eng.swap(a, b)
                 energies[a], energies[b] = energies[b], energies[a]
-    final = [e["total"] for e in eng.total_energy(mode_a=mode_a, include_sasa=include_sasa,
-                                                   allow_nonfinite=True)]
+    final = swap_energies(eng, include_sasa)
     stopped = [bool(x) for x in eng.status()["diverged"]]
     return eng, RemdSummary(n_rep, sys.n_atoms, attempts, accepts, temps, final, stopped)
