@@ -303,6 +303,9 @@ def main():
     ap.add_argument("--pmc-probe", action="store_true",
                     help="short workload for hardware-counter runs: a 512 MiB device copy (the "
                          "calibration kernel) and three force evaluations of the 64-chain batch")
+    ap.add_argument("--step-mode", choices=["auto", "graph", "direct", "persistent"], default="auto",
+                    help="how og_run issues a step: by batch size (auto), captured graphs, direct "
+                         "launches, or the persistent replica-step kernel")
     ap.add_argument("--no-fixtures", action="store_true",
                     help="skip the timing of the reference's own PDB systems (1UAO, 1L2Y, 2F4K)")
     ap.add_argument("--profile-steps", type=int, default=20,
@@ -357,6 +360,7 @@ def main():
                        sasa_gamma_scale=gamma_scale)
     tune = eng.autotune(include_sasa=include_sasa)
     log("autotune:", tune)
+    eng.use_graphs({"auto": None, "graph": True, "direct": False, "persistent": "persistent"}[args.step_mode])
     eng.run(args.warmup)
     # timed run: no per-kernel events (captured graphs below 20 000 atoms)
     t0 = time.perf_counter()
@@ -439,6 +443,7 @@ def main():
                          "launches, one kernel at a time (the timed run has SASA on a second "
                          "stream beside the pair kernels)" % args.profile_steps,
         "autotune": tune,
+        "step_mode": args.step_mode,
         "relax": {"minimiser": "og_minimize (device-resident batched L-BFGS, all pairs)",
                   "max_steps": args.minimize_steps, "seconds": relax_s,
                   "steps_max_over_systems": max(r.steps for r in res)},

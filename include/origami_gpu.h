@@ -224,12 +224,18 @@ int og_frames_setup(og_engine* e, int32_t capacity, int32_t every);
  * the count.  Either output may be NULL (the frames are still consumed). */
 int og_frames_read(og_engine* e, double* xyz, uint32_t* steps, int32_t max_frames, int32_t* n_frames);
 
-/* og_run replays one captured hipGraph per BAOAB step (all systems, every
- * launch of the step) when profiling is off and no per-step kinetic energy
- * is requested.  on = 1: always; 0: direct launches; 2 (the default): below
- * 20 000 atoms, where launch overhead matters (on larger batches direct
- * launches measured 2-3 % faster).  No counterpart in origami (its step
- * loop is langevin.rs:195-367, run per replica by remd.rs:286). */
+/* How og_run issues a step (no counterpart in origami; its step loop is
+ * langevin.rs:195-367, run per replica by remd.rs:286):
+ *   0 direct launches (five per step on small batches);
+ *   1 one captured hipGraph per BAOAB step (all systems, every launch of the
+ *     step), replayed, when profiling is off and no per-step kinetic energy
+ *     is requested;
+ *   2 the engine's choice (the default): direct launches, measured fastest
+ *     on the REMD batch and the 64-chain batch;
+ *   3 persistent replica steps: one launch per run of steps, one block of
+ *     512 threads per system, for batches of systems of up to 512 atoms
+ *     without SASA or the surrogate (else direct launches).
+ * Every mode gives bit-identical trajectories. */
 int og_use_graphs(og_engine* e, int32_t on);
 
 /* Autotuner (no counterpart in origami): times the candidate launch widths

@@ -23,10 +23,11 @@
 #include "og_sur.h"
 
 // og_run's default: captured steps below this many atoms
-#define OG_GRAPH_MAX_ATOMS 20000
 #define OG_SPLIT_MAX_TILES 4           // systems up to 256 atoms: off-diagonal tile pairs in halves
 #define OG_STRIP_DEFAULT 4             // j tiles per force strip until the autotuner picks
 #define OG_SR_DOTS 960                 // Shrake-Rupley dots per atom (SHRAKE_RUPLEY_DOTS, sasa.rs:27)
+#define OG_PERSIST_MAX_TILES 8         // persistent replica steps: systems of up to 512 atoms
+#define OG_PERSIST_CHUNK 512           // steps per persistent launch
 
 static thread_local std::string g_err;
 
@@ -103,12 +104,18 @@ struct og_engine {
     og_sur* sur = nullptr;                 // learned Born-radius surrogate (optional)
     uint32_t* step_dev = nullptr;          // BAOAB step counter (Philox counter word)
     double* noise = nullptr;               // [2][3N] the O step's normals, step s in half s & 1
-    int graph_mode = 2;                    // og_run: 1 captured steps, 0 direct launches, 2 by size
+    int graph_mode = 2;                    // og_run: 0 direct launches, 1 captured steps, 3 persistent
+                                           // replica steps, 2 the engine's choice (direct launches)
     bool graph_ok = false;
     hipGraphExec_t gexec[2] = {nullptr, nullptr};   // step; step with surrogate check
     long long* noalloc_ke = nullptr;       // this device's og_run_noalloc scratch array
     int* eval_skip = nullptr;              // [S] systems an energy-only evaluation leaves out (minimiser probes)
     double* sr_dots = nullptr;             // [OG_SR_DOTS][3] Shrake-Rupley unit-sphere dots
+    // persistent replica steps: per-system index ranges (og_persist_args)
+    int *p_atom_off = nullptr, *p_cl_small_off = nullptr, *p_cl_small = nullptr, *p_cl_big_off = nullptr,
+        *p_cl_big = nullptr, *p_term_off = nullptr, *p_work_off = nullptr, *p_fwork_off = nullptr,
+        *p_strip_off = nullptr;
+    int max_sys_tiles = 0;
     // frame ring (og_frames_setup): every system's positions after every
     // frame_every-th step, kept on the device until og_frames_read
     double* frames = nullptr;              // [frame_cap][3N]
@@ -196,6 +203,9 @@ static void free_all(og_engine* e) {
     if (e->step_dev) (void)hipFree(e->step_dev);
     if (e->frames) (void)hipFree(e->frames);
     if (e->sr_dots) (void)hipFree(e->sr_dots);
+    for (int* q : {e->p_atom_off, e->p_cl_small_off, e->p_cl_small, e->p_cl_big_off, e->p_cl_big, e->p_term_off,
+                   e->p_work_off, e->p_fwork_off, e->p_strip_off})
+        if (q) (void)hipFree(q);
     if (e->noise) (void)hipFree(e->noise);
     if (e->sur) {
         og_sur_free(e->sur);
@@ -240,6 +250,7 @@ static int build_force_lists(og_engine* e, int strip_j) {
     if (strip_j < 1 || strip_j > OG_STRIP_J) return fail(OG_ERR_ARG, "strip length must be 1..%d", OG_STRIP_J);
     std::vector<og_work> fwork;
     std::vector<og_strip> strips;
+    std::vector<int> fwork_off{0}, strip_off{0};
     const std::vector<og_work>& work = e->work_h;
     size_t w = 0;
     for (int s = 0; s < e->n_sys; ++s) {
@@ -248,6 +259,8 @@ static int build_force_lists(og_engine* e, int strip_j) {
         while (w < work.size() && work[w].sys == s) ++w;
         if (nt <= OG_SPLIT_MAX_TILES) {
             fwork.insert(fwork.end(), work.begin() + w0, work.begin() + w);
+            fwork_off.push_back((int)fwork.size());
+            strip_off.push_back((int)strips.size());
             continue;
         }
         if ((long long)(w - w0) != (long long)nt * (nt + 1) / 2)
@@ -287,12 +300,18 @@ static int build_force_lists(og_engine* e, int strip_j) {
                 if (cov[(size_t)I * nt + J] != (J >= I ? 1 : 0))
                     return fail(OG_ERR_STATE, "internal: force work list covers tile pair (%d, %d) of system %d %d times",
                                 I, J, s, cov[(size_t)I * nt + J]);
+        fwork_off.push_back((int)fwork.size());
+        strip_off.push_back((int)strips.size());
     }
     if (w != work.size()) return fail(OG_ERR_STATE, "internal: tile-pair list out of system order");
     if (e->strips) { HIPCHK(hipFree(e->strips)); e->strips = nullptr; }
     if (e->fwork) { HIPCHK(hipFree(e->fwork)); e->fwork = nullptr; }
+    if (e->p_fwork_off) { HIPCHK(hipFree(e->p_fwork_off)); e->p_fwork_off = nullptr; }
+    if (e->p_strip_off) { HIPCHK(hipFree(e->p_strip_off)); e->p_strip_off = nullptr; }
     HIPCHK(upload(&e->strips, strips));
     HIPCHK(upload(&e->fwork, fwork));
+    HIPCHK(upload(&e->p_fwork_off, fwork_off));
+    HIPCHK(upload(&e->p_strip_off, strip_off));
     e->n_strip = (int)strips.size();
     e->n_fwork = (int)fwork.size();
     e->strip_j = strip_j;
@@ -408,6 +427,7 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     std::vector<int> a_sys(N), a_loc(N), a_slot(N);
     std::vector<int> tile_atom0, tile_cnt, sys_tile_off{0};
     std::vector<og_work> work;
+    std::vector<int> term_cnt(4 * (size_t)n_sys, 0), sys_work_end;
     std::vector<unsigned long long> masks;
     std::vector<float4> par, gbv;
     std::vector<float> qs;
@@ -422,6 +442,10 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     const double sqrt_kc = std::sqrt(OG_COULOMB_KCAL * OG_KCAL_TO_KJ);
     for (int s = 0; s < n_sys; ++s) {
         const og_system_desc& d = sys[s];
+        term_cnt[0 * (size_t)n_sys + s] = d.n_bonds;
+        term_cnt[1 * (size_t)n_sys + s] = d.n_angles;
+        term_cnt[2 * (size_t)n_sys + s] = d.n_dihedrals;
+        term_cnt[3 * (size_t)n_sys + s] = d.n_impropers;
         const int n = d.n_atoms, off = e->atom_off[s];
         e->topo_hash[s] = topology_hash(d);
         const int t0 = (int)tile_atom0.size();
@@ -509,6 +533,8 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
                 }
             }
         }
+        sys_work_end.push_back((int)work.size());
+        e->max_sys_tiles = std::max(e->max_sys_tiles, nt);
         // Bonded terms (global atom indices).
         for (int k = 0; k < d.n_bonds; ++k) {
             bonds.push_back(off + d.bonds[2 * k]); bonds.push_back(off + d.bonds[2 * k + 1]);
@@ -588,6 +614,39 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
             s_cstart.push_back((int)s_len.size());
         }
         cl_start.swap(s_start); cl_atoms.swap(s_atoms); cn_start.swap(s_cstart); cn_len.swap(s_len);
+    }
+    // persistent replica steps: each system's clusters, bonded terms and
+    // descreening items as index ranges
+    {
+        std::vector<std::vector<int>> small(n_sys), big(n_sys);
+        for (int c = 0; c < e->n_clusters; ++c) {
+            const int sy = a_sys[cl_atoms[cl_start[c]]];
+            (c < e->n_clusters_small ? small : big)[sy].push_back(c);
+        }
+        std::vector<int> so{0}, sl, bo{0}, bl;
+        for (int sy = 0; sy < n_sys; ++sy) {
+            sl.insert(sl.end(), small[sy].begin(), small[sy].end());
+            so.push_back((int)sl.size());
+            bl.insert(bl.end(), big[sy].begin(), big[sy].end());
+            bo.push_back((int)bl.size());
+        }
+        std::vector<int> toff(4 * ((size_t)n_sys + 1), 0);
+        for (int kind = 0; kind < 4; ++kind)
+            for (int sy = 0; sy < n_sys; ++sy)
+                toff[kind * ((size_t)n_sys + 1) + sy + 1] =
+                    toff[kind * ((size_t)n_sys + 1) + sy] + term_cnt[kind * (size_t)n_sys + sy];
+        std::vector<int> woff{0};
+        woff.insert(woff.end(), sys_work_end.begin(), sys_work_end.end());
+        std::vector<int> aoff(e->atom_off.begin(), e->atom_off.end());
+        aoff.resize(n_sys);
+        aoff.push_back(N);
+        HIPCHK(upload(&e->p_atom_off, aoff));
+        HIPCHK(upload(&e->p_cl_small_off, so));
+        HIPCHK(upload(&e->p_cl_small, sl));
+        HIPCHK(upload(&e->p_cl_big_off, bo));
+        HIPCHK(upload(&e->p_cl_big, bl));
+        HIPCHK(upload(&e->p_term_off, toff));
+        HIPCHK(upload(&e->p_work_off, woff));
     }
     e->nb = (int)bond_p.size() / 2;
     e->na = (int)angle_p.size() / 2;
@@ -1137,12 +1196,67 @@ static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allo
         c.n_atoms = e->n_atoms;
     }
     og_sur* sur = (p.flags & OG_SURROGATE) ? e->sur : nullptr;
-    // Captured steps: every launch of one BAOAB step (cluster B-A-SHAKE-O-A,
-    // forces, final kick) is recorded once into a hipGraph and replayed; the
-    // step counter lives in device memory so one graph serves every step.
-    // Captured steps pay off where launch overhead matters (small batches);
-    // on large batches direct launches measured 2-3 % faster per step.
-    const bool want = e->graph_mode == 1 || (e->graph_mode == 2 && e->n_atoms < OG_GRAPH_MAX_ATOMS);
+    // Persistent replica steps (og_use_graphs 3): one launch runs many whole
+    // steps, a block per system (batches of small systems without SASA or the
+    // surrogate), chunked at frame boundaries.  Measured on the REMD batch (8
+    // and 64 replicas of 171 atoms): 56 us per step against 36.5 us with
+    // direct launches and 39.4 us with captured steps: one CU's four SIMDs
+    // take a replica's 18 tile-pair items in turn, where the launches spread
+    // them over the chip.
+    const bool persist_ok = !(p.flags & (OG_SASA | OG_SURROGATE)) && !e->prof &&
+                            e->max_sys_tiles <= OG_PERSIST_MAX_TILES && e->n_clusters_small == e->n_clusters &&
+                            n_steps > 0;
+    if (persist_ok && e->graph_mode == 3) {
+        og_pair_args pa = pair_args(e, p.cutoff_a > 0.0 && (p.flags & OG_MODE_A) ? p.cutoff_a : OG_GB_CUTOFF_A);
+        og_bonded_args ba{};
+        ba.pos = e->pos; ba.facc = e->facc; ba.eacc = e->eacc; ba.atom_sys = e->atom_sys;
+        ba.nb = e->nb; ba.na = e->na; ba.nd = e->nd; ba.ni = e->ni;
+        ba.bonds = e->bonds; ba.bond_p = e->bond_p; ba.angles = e->angles; ba.angle_p = e->angle_p;
+        ba.dih = e->dih; ba.dih_p = e->dih_p; ba.imp = e->imp; ba.imp_p = e->imp_p;
+        ba.tf = e->bonded_tf; ba.csr_start = e->bonded_csr_start; ba.csr_slot = e->bonded_csr_slot;
+        ba.n_atoms = e->n_atoms;
+        ba.sflag = e->sflag;
+        og_radii_args ra{};
+        ra.iacc = e->iacc; ra.atom_sys = e->atom_sys; ra.atom_slot = e->atom_slot; ra.rho = e->rho;
+        ra.charge = e->charge; ra.gb = e->gb; ra.radii = e->radii; ra.clamped = e->clamped;
+        ra.eacc = e->eacc; ra.n_atoms = e->n_atoms; ra.sflag = e->sflag;
+        og_back_args bk = b;
+        bk.step_dev = nullptr;
+        og_persist_args pp{};
+        pp.n_sys = e->n_sys;
+        pp.atom_off = e->p_atom_off; pp.cl_small_off = e->p_cl_small_off; pp.cl_small = e->p_cl_small;
+        pp.cl_big_off = e->p_cl_big_off; pp.cl_big = e->p_cl_big; pp.term_off = e->p_term_off;
+        pp.work_off = e->p_work_off; pp.fwork_off = e->p_fwork_off; pp.strip_off = e->p_strip_off;
+        og_noise_args pn{};
+        if (nzp) pn = *nzp;
+        int done = 0;
+        while (done < n_steps) {
+            int chunk = std::min(n_steps - done, OG_PERSIST_CHUNK);
+            const uint32_t st0 = e->step + (uint32_t)done;
+            if (e->frame_every > 0) {
+                const int to_frame = e->frame_every - (int)(st0 % (uint32_t)e->frame_every);
+                chunk = std::min(chunk, to_frame);
+            }
+            pp.n_steps = chunk;
+            pp.step0 = st0;
+            pp.ke_acc = ke_dev ? ke_dev + (size_t)done * e->n_sys : nullptr;
+            og_launch_replica_steps(pa, ba, ra, c, bk, pn, pp, (p.flags & OG_MODE_A) != 0, e->stream);
+            HIPCHK(hipGetLastError());
+            done += chunk;
+            const uint32_t st = e->step + (uint32_t)done;
+            if (e->frame_every > 0 && st % (uint32_t)e->frame_every == 0)
+                if (int rc = save_frame(e, st)) return rc;
+        }
+        e->step += (uint32_t)n_steps;
+        return OG_OK;
+    }
+    // Captured steps (og_use_graphs 1): every launch of one BAOAB step
+    // (cluster B-A-SHAKE-O-A, forces, final kick) is recorded once into a
+    // hipGraph and replayed; the step counter lives in device memory so one
+    // graph serves every step.  Direct launches measured faster on both the
+    // REMD batch (36.5 against 39.4 us per step) and the 64-chain batch, so
+    // they are the engine's choice.
+    const bool want = e->graph_mode == 1;
     const bool graphs = allow_graphs && want && !e->prof && !ke_dev && n_steps >= 2;
     if (graphs) {
         if (!e->graph_ok) {
@@ -1296,7 +1410,7 @@ extern "C" int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step
 
 extern "C" int og_use_graphs(og_engine* e, int32_t on) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
-    if (on < 0 || on > 2) return fail(OG_ERR_ARG, "graph mode must be 0, 1 or 2");
+    if (on < 0 || on > 3) return fail(OG_ERR_ARG, "graph mode must be 0, 1, 2 or 3");
     e->graph_mode = on;
     invalidate_graphs(e);
     return OG_OK;

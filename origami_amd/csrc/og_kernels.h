@@ -211,3 +211,29 @@ __host__ __device__ inline double og_lb_total_energy(const long long* acc) {
            (double)acc[OG_E_SASA] * k;
 }
 
+// ---------------------------------------------------------------------------
+// Persistent replica steps (pair_kernels.hip): one block per system runs
+// n_steps whole BAOAB steps (normals, cluster B-A-SHAKE-O-A,
+// bonded terms, descreening, radii + bonded gather, pair forces, closing
+// kick) with block barriers between the phases; systems share nothing, so
+// no grid-wide synchronisation is needed.  Per-system index ranges below.
+// ---------------------------------------------------------------------------
+struct og_persist_args {
+    int n_steps;
+    uint32_t step0;                  // Philox step counter of the first step
+    int n_sys;
+    const int* atom_off;             // [S+1]
+    const int* cl_small_off;         // [S+1] into cl_small (clusters of <= OG_SMALL_CL atoms; the
+    const int* cl_small;             //   kernel runs batches that have no larger ones)
+    const int* cl_big_off;           // [S+1] into cl_big
+    const int* cl_big;
+    const int* term_off;             // [4][S+1]: bonds, angles, dihedrals, impropers
+    const int* work_off;             // [S+1] descreening items (og_pair_args::work)
+    const int* fwork_off;            // [S+1] force items (og_pair_args::fwork)
+    const int* strip_off;            // [S+1] strips (og_pair_args::strips)
+    long long* ke_acc;               // [n_steps][S] or null
+};
+void og_launch_replica_steps(const og_pair_args& pa, const og_bonded_args& ba, const og_radii_args& ra,
+                             const og_cluster_args& ca, const og_back_args& bk, const og_noise_args& nz,
+                             const og_persist_args& pp, bool mode_a, hipStream_t s);
+

@@ -27,6 +27,7 @@
 #include "og_device.h"
 #include "og_kernels.h"
 #include "bonded_core.h"
+#include "md_core.h"
 
 namespace {
 
@@ -167,18 +168,12 @@ __device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const L
     acc_j += gj;
 }
 
-// blk / nblk: this block among the nblk blocks sharing the tile pairs.
+// One descreening work item (a tile pair, or half of one) of the wave; P, Q:
+// the wave's LDS staging.
 template <bool MODEA>
-__device__ __forceinline__ void descreen_body(const og_pair_args& a, int blk, int nblk) {
-    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
-    float4* P = s_pq[threadIdx.x >> 6];
-    float4* Q = P + (OG_TILE + 1);
-    const int lane = threadIdx.x & 63;
-    const int wave = (blk * blockDim.x + threadIdx.x) >> 6;
-    const int n_waves = (nblk * blockDim.x) >> 6;
-    for (int w = wave; w < a.n_work; w += n_waves) {
-        const og_work it = a.work[w];
-        if (a.skip_sys && a.skip_sys[it.sys]) continue;     // radii from the surrogate
+__device__ __forceinline__ void descreen_item(const og_pair_args& a, const og_work& it, int lane, float4* P,
+                                              float4* Q) {
+    {
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],
@@ -203,6 +198,22 @@ __device__ __forceinline__ void descreen_body(const og_pair_args& a, int blk, in
         int* fl = a.sflag ? a.sflag + it.sys : nullptr;
         if (lane < ci) og_atomic_add_fix(a.iacc + a0i + lane, og_to_fix_chk((double)acc_i, fl));
         if (lane < cj) og_atomic_add_fix(a.iacc + a0j + lane, og_to_fix_chk((double)acc_j, fl));
+    }
+}
+
+// blk / nblk: this block among the nblk blocks sharing the tile pairs.
+template <bool MODEA>
+__device__ __forceinline__ void descreen_body(const og_pair_args& a, int blk, int nblk) {
+    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
+    float4* P = s_pq[threadIdx.x >> 6];
+    float4* Q = P + (OG_TILE + 1);
+    const int lane = threadIdx.x & 63;
+    const int wave = (blk * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (nblk * blockDim.x) >> 6;
+    for (int w = wave; w < a.n_work; w += n_waves) {
+        const og_work it = a.work[w];
+        if (a.skip_sys && a.skip_sys[it.sys]) continue;     // radii from the surrogate
+        descreen_item<MODEA>(a, it, lane, P, Q);
     }
 }
 
@@ -497,24 +508,12 @@ __device__ __forceinline__ void force_strip(const og_pair_args& a, const og_stri
     }
 }
 
-// The force pass: the strips first (the heavier items), then the remaining
-// tile pairs (fwork), grid-stride over both.
+// One force work item outside the strips (a diagonal, adjacent or masked
+// tile pair, or a tile pair of a small system).
 template <bool MODEA>
-__device__ __forceinline__ void force_body(const og_pair_args& a) {
-    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
-    float4* P = s_pq[threadIdx.x >> 6];
-    float4* Q = P + (OG_TILE + 1);
-    const int lane = threadIdx.x & 63;
-    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int n_waves = (gridDim.x * blockDim.x) >> 6;
-    const int total = a.n_strip + a.n_fwork;
-    for (int w = wave; w < total; w += n_waves) {
-        if (w < a.n_strip) {
-            const og_strip st = a.strips[w];
-            force_strip<MODEA>(a, st, lane, P, Q);
-            continue;
-        }
-        const og_work it = a.fwork[w - a.n_strip];
+__device__ __forceinline__ void force_item(const og_pair_args& a, const og_work& it, int lane, float4* P,
+                                           float4* Q) {
+    {
         const int a0i = a.tile_atom0[it.ti], ci = a.tile_cnt[it.ti];
         const int a0j = a.tile_atom0[it.tj], cj = a.tile_cnt[it.tj];
         const double ox = a.pos[3 * (size_t)a0i], oy = a.pos[3 * (size_t)a0i + 1],
@@ -560,6 +559,28 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
             og_atomic_add_fix(f + 1, og_to_fix_chk((double)acc.gy, fl));
             og_atomic_add_fix(f + 2, og_to_fix_chk((double)acc.gz, fl));
         }
+    }
+}
+
+// The force pass: the strips first (the heavier items), then the remaining
+// tile pairs (fwork), grid-stride over both.
+template <bool MODEA>
+__device__ __forceinline__ void force_body(const og_pair_args& a) {
+    __shared__ float4 s_pq[4][2 * (OG_TILE + 1)];       // per wave: P[65], then Q[65]
+    float4* P = s_pq[threadIdx.x >> 6];
+    float4* Q = P + (OG_TILE + 1);
+    const int lane = threadIdx.x & 63;
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int n_waves = (gridDim.x * blockDim.x) >> 6;
+    const int total = a.n_strip + a.n_fwork;
+    for (int w = wave; w < total; w += n_waves) {
+        if (w < a.n_strip) {
+            const og_strip st = a.strips[w];
+            force_strip<MODEA>(a, st, lane, P, Q);
+            continue;
+        }
+        const og_work it = a.fwork[w - a.n_strip];
+        force_item<MODEA>(a, it, lane, P, Q);
     }
 }
 
@@ -717,3 +738,92 @@ void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, b
     if (mode_a) hipLaunchKernelGGL(og_k_energy64_a, dim3(g), dim3(256), 0, s, a, e);
     else hipLaunchKernelGGL(og_k_energy64_b, dim3(g), dim3(256), 0, s, a, e);
 }
+
+// ---------------------------------------------------------------------------
+// Persistent replica steps: block s owns system s for n_steps steps.  Each
+// phase runs the same per-item code as the step's separate launches (the
+// same FP32 partial sums per work item, the same FP64 per-atom and per-term
+// arithmetic; integer sums do not depend on order), so the trajectory is the
+// one the launches give, bit for bit.  A system that stops (non-finite
+// velocities, a range flag) leaves the loop at that step.
+// ---------------------------------------------------------------------------
+// threads per block of the persistent replica-step kernel (16 waves: every
+// tile pair of a small system's force or descreening pass at once)
+#define OG_PERSIST_THREADS 512
+template <bool MODEA, bool PRE>
+__global__ void __launch_bounds__(OG_PERSIST_THREADS) og_k_replica_steps(og_pair_args pa, og_bonded_args ba, og_radii_args ra,
+                                                          og_cluster_args ca, og_back_args bk, og_noise_args nz,
+                                                          og_persist_args pp) {
+    constexpr int NT = OG_PERSIST_THREADS, NW = OG_PERSIST_THREADS / 64;
+    __shared__ float4 s_pq[NW][2 * (OG_TILE + 1)];
+    __shared__ int s_stop;
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    float4* P = s_pq[wave];
+    float4* Q = P + (OG_TILE + 1);
+    const int a0 = pp.atom_off[s], a1 = pp.atom_off[s + 1];
+    for (int k = 0; k < pp.n_steps; ++k) {
+        // one thread reads the stop flag and the block follows it, so every
+        // thread leaves the loop at the same step
+        if (tid == 0) s_stop = __hip_atomic_load(ca.diverged + s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (s_stop) break;
+        const uint32_t step = pp.step0 + (uint32_t)k;
+        if (PRE) {
+            for (int i = a0 + tid; i < a1; i += NT) noise_atom_at(nz, i, step);
+            __syncthreads();
+        }
+        for (int q = pp.cl_small_off[s] + tid; q < pp.cl_small_off[s + 1]; q += NT)
+            cluster_body<OG_SMALL_CL, PRE>(ca, pp.cl_small[q], step);
+        __syncthreads();
+#pragma unroll
+        for (int kind = 0; kind < 4; ++kind) {
+            // global term index of this kind's first term (bonds, angles, dihedrals, impropers)
+            const int base = kind == 0 ? 0 : kind == 1 ? ba.nb : kind == 2 ? ba.nb + ba.na : ba.nb + ba.na + ba.nd;
+            const int* off = pp.term_off + kind * (pp.n_sys + 1);
+            for (int t = off[s] + tid; t < off[s + 1]; t += NT) {
+                int e_slot = -1, e_sys = 0;
+                long long e_fix = 0;
+                bonded_term<false>(ba, base + t, e_slot, e_sys, e_fix);
+            }
+        }
+        for (int w = pp.work_off[s] + wave; w < pp.work_off[s + 1]; w += NW) {
+            const og_work it = pa.work[w];
+            descreen_item<MODEA>(pa, it, lane, P, Q);
+        }
+        __syncthreads();
+        for (int i = a0 + tid; i < a1; i += NT) {
+            radii_body<false>(ra, i);
+            gather_atom(ba, i);
+        }
+        __syncthreads();
+        for (int w = pp.strip_off[s] + wave; w < pp.strip_off[s + 1]; w += NW) {
+            const og_strip st = pa.strips[w];
+            force_strip<MODEA>(pa, st, lane, P, Q);
+        }
+        for (int w = pp.fwork_off[s] + wave; w < pp.fwork_off[s + 1]; w += NW) {
+            const og_work it = pa.fwork[w];
+            force_item<MODEA>(pa, it, lane, P, Q);
+        }
+        __syncthreads();
+        og_back_args b = bk;
+        b.ke_acc = pp.ke_acc ? pp.ke_acc + (size_t)k * pp.n_sys : nullptr;
+        for (int i = a0 + tid; i < a1; i += NT) back_atom(b, i);
+        __syncthreads();
+    }
+}
+
+void og_launch_replica_steps(const og_pair_args& pa, const og_bonded_args& ba, const og_radii_args& ra,
+                             const og_cluster_args& ca, const og_back_args& bk, const og_noise_args& nz,
+                             const og_persist_args& pp, bool mode_a, hipStream_t s) {
+    if (pp.n_sys <= 0 || pp.n_steps <= 0) return;
+    const bool pre = nz.noise != nullptr;
+    if (mode_a) {
+        if (pre) hipLaunchKernelGGL((og_k_replica_steps<true, true>), dim3(pp.n_sys), dim3(OG_PERSIST_THREADS), 0, s, pa, ba, ra, ca, bk, nz, pp);
+        else hipLaunchKernelGGL((og_k_replica_steps<true, false>), dim3(pp.n_sys), dim3(OG_PERSIST_THREADS), 0, s, pa, ba, ra, ca, bk, nz, pp);
+    } else {
+        if (pre) hipLaunchKernelGGL((og_k_replica_steps<false, true>), dim3(pp.n_sys), dim3(OG_PERSIST_THREADS), 0, s, pa, ba, ra, ca, bk, nz, pp);
+        else hipLaunchKernelGGL((og_k_replica_steps<false, false>), dim3(pp.n_sys), dim3(OG_PERSIST_THREADS), 0, s, pa, ba, ra, ca, bk, nz, pp);
+    }
+}
+

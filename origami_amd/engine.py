@@ -339,9 +339,12 @@ class Engine:
         return ke
 
     def use_graphs(self, on=True):
-        """True: captured steps; False: direct launches; None: by batch size
-        (the default: captured below 20 000 atoms)."""
-        _check(lib().og_use_graphs(self._h, 2 if on is None else (1 if on else 0)))
+        """True: captured steps; False: direct launches; "persistent": the
+        persistent replica-step kernel (one block per system for many steps;
+        systems of up to 512 atoms, no SASA or surrogate); None: the engine's
+        choice (direct launches, the default)."""
+        mode = 2 if on is None else (3 if on == "persistent" else (1 if on else 0))
+        _check(lib().og_use_graphs(self._h, mode))
 
     def set_launch(self, pair_blocks=0, sasa_blocks=0, sasa_passes=0):
         """Launch widths (0 keeps the current one) and the SASA arrangement's

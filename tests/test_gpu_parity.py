@@ -687,3 +687,38 @@ def test_remd_stopped_replica_is_skipped(E, c3_system):
     xs = eng.split(eng.positions())
     for k in (0, 1, 3, 4, 5):
         assert np.isfinite(xs[k]).all()
+
+
+@pytest.mark.parametrize("mode_a", [False, True])
+def test_persistent_replica_steps_match_launches(E, relaxed, mode_a):
+    """The persistent replica-step kernel (a block per system, many steps per
+    launch) against captured steps and direct launches: positions,
+    velocities, forces and per-step kinetic energies bit for bit, over
+    several calls (chunk boundaries), an uneven batch and a REMD-style ladder."""
+    c1, c3 = relaxed["c1"], relaxed["c3"]
+    for systems in ([c3] * 8, [c1, c3, c3, c1]):
+        def make(mode):
+            eng = E.Engine(systems)
+            v = np.concatenate([E.maxwell_boltzmann(s, 300.0, S.Xoshiro256pp(60 + k))
+                                for k, s in enumerate(systems)])
+            eng.set_velocities(v)
+            eng.dynamics_setup([280.0 + 10.0 * k for k in range(len(systems))], dt_fs=2.0,
+                               friction_ps_inv=2.0, seed=31, constrain_h_bonds=True, mode_a=mode_a)
+            eng.use_graphs(mode)
+            return eng
+        per, graph, direct = make("persistent"), make(True), make(False)
+        for n in (7, 1, 600):
+            print("persistent vs captured vs direct: %d systems, %d steps, mode_a=%s"
+                  % (len(systems), n, mode_a), flush=True)
+            per.run(n)
+            graph.run(n)
+            direct.run(n)
+            for other in (graph, direct):
+                assert np.array_equal(per.positions(), other.positions()), n
+                assert np.array_equal(per.velocities(), other.velocities()), n
+                assert np.array_equal(per.forces(), other.forces()), n
+        ke_p = per.run(5, record_ke=True)
+        ke_d = direct.run(5, record_ke=True)
+        assert np.array_equal(ke_p, ke_d)
+        st = per.status()
+        assert not st["diverged"].any() and st["shake_failures"].sum() == 0
