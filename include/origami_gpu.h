@@ -208,6 +208,23 @@ int og_minimize(og_engine* e, const og_minimize_params* p, og_minimize_result* r
  * by FP32 rounding (the per-strip partial sums), not in their pairs. */
 int og_strip_tiles(og_engine* e, int32_t set, int32_t* get);
 
+/* External cylindrical exit tunnel for dynamics (CylindricalTunnel of
+ * dynamics/src/cotranslate.rs:105-155, the co-translational driver's
+ * ExternalPotential): atoms with 0 <= (x - origin).direction <= length_a
+ * whose distance from the axis exceeds radius_a feel k_confine (d - radius_a)
+ * towards the axis, added to every dynamics force evaluation (og_run's steps
+ * and og_dynamics_setup's initial forces; not og_forces / og_energies).
+ * direction must be a unit vector; NULL removes the tunnel.  Set it before
+ * og_dynamics_setup so that the first step's forces carry it. */
+typedef struct og_tunnel {
+    double origin[3];
+    double direction[3];
+    double radius_a;                /* 12 Å */
+    double length_a;                /* 80 Å */
+    double k_confine;               /* 50 kJ/mol/Å^2 */
+} og_tunnel;
+int og_set_tunnel(og_engine* e, const og_tunnel* t);
+
 /* Trajectory frames (langevin.rs save_every and its frame callback,
  * :205-220, :332-341): with capacity > 0 and every > 0, og_run and
  * og_run_noalloc copy every system's positions into a device ring of

@@ -704,3 +704,25 @@ void orc_philox_normals(uint32_t k0, uint32_t k1, uint32_t sys, int n_atoms, uin
         out[3 * a + 2] = r * cos(th);
     }
 }
+
+/* ------------------------------------------------------------------------ */
+/* Cylindrical exit tunnel, dynamics/src/cotranslate.rs:131-155: atoms with  */
+/* 0 <= (x - o).d <= length and radial distance beyond radius feel          */
+/* -k (dist - radius) along the outward radial unit vector (added to f).    */
+/* ------------------------------------------------------------------------ */
+void orc_tunnel_force(int n, const double *pos, const double *origin, const double *dir, double radius,
+                      double length, double k, double *f) {
+    for (int i = 0; i < n; ++i) {
+        const double rx = pos[3 * i] - origin[0], ry = pos[3 * i + 1] - origin[1], rz = pos[3 * i + 2] - origin[2];
+        const double along = rx * dir[0] + ry * dir[1] + rz * dir[2];
+        if (along < 0.0 || along > length) continue;
+        const double px = rx - dir[0] * along, py = ry - dir[1] * along, pz = rz - dir[2] * along;
+        const double d = sqrt(px * px + py * py + pz * pz);
+        if (d > radius) {
+            const double over = d - radius;
+            f[3 * i] -= px / d * (k * over);
+            f[3 * i + 1] -= py / d * (k * over);
+            f[3 * i + 2] -= pz / d * (k * over);
+        }
+    }
+}

@@ -46,6 +46,9 @@ void orc_philox4x32(uint32_t k0, uint32_t k1, uint32_t c0, uint32_t c1, uint32_t
 void orc_philox_normals(uint32_t k0, uint32_t k1, uint32_t sys, int n_atoms, uint32_t step,
                         double *out);
 
+void orc_tunnel_force(int n, const double *pos, const double *origin, const double *dir, double radius,
+                      double length, double k, double *f);
+
 /* PowerSasa (oracle_sasa.c) */
 void orc_psa_forces(int n, const double *pos, const double *radii, const double *gamma,
                     double *f, double *stats);

@@ -67,6 +67,7 @@ def lib():
         L.orc_psa_atom_gradient.argtypes = [C.c_int, dp, dp, C.c_int, C.c_int, ip, dp]
         L.orc_psa_area.restype = C.c_double
         L.orc_psa_area.argtypes = [C.c_int, dp, dp, dp]
+        L.orc_tunnel_force.argtypes = [C.c_int, dp, dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_sr_dots.argtypes = [C.c_int, dp]
         L.orc_sr_energy.restype = C.c_double
         L.orc_sr_energy.argtypes = [C.c_int, dp, dp, dp, C.c_int, dp]
@@ -302,4 +303,14 @@ def sr_energy(pos, radii, gamma, n_dots=960):
     area, ap = _d(np.zeros(len(p)))
     e = lib().orc_sr_energy(len(p), pp, rp, gp, n_dots, ap)
     return e, area
+
+
+def tunnel_force(pos, origin, direction, radius, length, k):
+    """Forces of the cylindrical exit tunnel (cotranslate.rs:131-155)."""
+    p, pp = _d(np.asarray(pos, dtype=np.float64).reshape(-1, 3))
+    o, op = _d(origin)
+    d, dp_ = _d(direction)
+    f, fp = _d(np.zeros((len(p), 3)))
+    lib().orc_tunnel_force(len(p), pp, op, dp_, radius, length, k, fp)
+    return f
 

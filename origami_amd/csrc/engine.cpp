@@ -116,6 +116,7 @@ struct og_engine {
         *p_cl_big = nullptr, *p_term_off = nullptr, *p_work_off = nullptr, *p_fwork_off = nullptr,
         *p_strip_off = nullptr;
     int max_sys_tiles = 0;
+    og_tunnel_dev tun{};                   // external tunnel of the dynamics (og_set_tunnel)
     // frame ring (og_frames_setup): every system's positions after every
     // frame_every-th step, kept on the device until og_frames_read
     double* frames = nullptr;              // [frame_cap][3N]
@@ -978,7 +979,7 @@ static int read_energies(og_engine* e, double* out) {
 // forces = false: energies only; the engine's force array keeps the forces of
 // the last force evaluation (the next og_run step starts from them).
 static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t flags, double cutoff_a,
-                    double sasa_gamma_scale, double* energies, bool forces) {
+                    double sasa_gamma_scale, double* energies, bool forces, bool external = false) {
     if (!e) return fail(OG_ERR_ARG, "engine is NULL");
     if ((flags & OG_SURROGATE) && !(e->sur && e->sur->enabled))
         return fail(OG_ERR_STATE, "OG_SURROGATE needs og_surrogate_setup, training and og_surrogate_enable");
@@ -998,9 +999,10 @@ static int evaluate(og_engine* e, const double* xyz, double* f_out, uint32_t fla
     if (forces) {
         ProfScope p(e, OG_K_BACK);
         og_back_args b{};
-        b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+        b.pos = e->pos; b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
         b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = 0.0; b.do_kick = 0;
         b.ke_acc = nullptr; b.diverged = e->diverged; b.sflag = e->sflag;
+        if (external) b.tun = e->tun;
         og_launch_back(b, e->stream);
     }
     HIPCHK(hipGetLastError());
@@ -1078,8 +1080,10 @@ extern "C" int og_dynamics_setup(og_engine* e, const og_dynamics_params* p, cons
     e->step = 0;
     e->dyn_ready = true;
     invalidate_graphs(e);
-    // initial forces at the current positions
-    return og_forces(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->cutoff_a, p->sasa_gamma_scale, nullptr);
+    // initial forces at the current positions (with the external tunnel, as
+    // every step's forces)
+    return evaluate(e, nullptr, nullptr, p->flags & ~OG_ENERGY, p->cutoff_a, p->sasa_gamma_scale, nullptr, true,
+                    true);
 }
 
 extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
@@ -1166,11 +1170,12 @@ static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allo
     c.key1 = (uint32_t)(p.seed >> 32);
     c.step_dev = e->step_dev;
     og_back_args b{};
-    b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+    b.pos = e->pos; b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
     b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = p.dt_fs; b.do_kick = 1;
     b.diverged = e->diverged;
     b.step_dev = e->step_dev;
     b.sflag = e->sflag;
+    b.tun = e->tun;
     // the device step counter restarts from the host's count each call
     og_launch_set_u32(e->step_dev, e->step, e->stream);
     // status bits of this call's force evaluations (a system already stopped
@@ -1574,7 +1579,7 @@ extern "C" int og_minimize(og_engine* e, const og_minimize_params* prm, og_minim
             og_launch_zero_acc(e->facc, e->iacc, e->n_atoms, e->stream);
             if (int r = launch_forces(e, P.flags & ~OG_ENERGY, P.cutoff_a, P.sasa_gamma_scale, false, -1, true)) return r;
             og_back_args b{};
-            b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
+            b.pos = e->pos; b.vel = e->vel; b.force = e->force; b.facc = e->facc; b.mass = e->mass;
             b.atom_sys = e->atom_sys; b.n_atoms = e->n_atoms; b.dt = 0.0; b.do_kick = 0;
             b.diverged = e->diverged; b.sflag = e->sflag;
             og_launch_back(b, e->stream);
@@ -1651,6 +1656,26 @@ extern "C" int og_strip_tiles(og_engine* e, int32_t set, int32_t* get) {
         invalidate_graphs(e);
     }
     if (get) *get = e->strip_j;
+    return OG_OK;
+}
+
+extern "C" int og_set_tunnel(og_engine* e, const og_tunnel* t) {
+    if (!e) return fail(OG_ERR_ARG, "engine is NULL");
+    og_tunnel_dev d{};
+    if (t) {
+        const double n = std::sqrt(t->direction[0] * t->direction[0] + t->direction[1] * t->direction[1] +
+                                   t->direction[2] * t->direction[2]);
+        if (!(std::fabs(n - 1.0) < 1e-9)) return fail(OG_ERR_ARG, "tunnel direction must be a unit vector");
+        if (!(t->radius_a >= 0.0) || !(t->length_a >= 0.0) || !(t->k_confine >= 0.0) ||
+            !std::isfinite(t->origin[0] + t->origin[1] + t->origin[2]))
+            return fail(OG_ERR_ARG, "tunnel radius, length and k must be >= 0 and the origin finite");
+        d.ox = t->origin[0]; d.oy = t->origin[1]; d.oz = t->origin[2];
+        d.dx = t->direction[0]; d.dy = t->direction[1]; d.dz = t->direction[2];
+        d.radius = t->radius_a; d.length = t->length_a; d.k = t->k_confine;
+        d.on = 1;
+    }
+    e->tun = d;
+    invalidate_graphs(e);
     return OG_OK;
 }
 

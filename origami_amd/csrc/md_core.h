@@ -318,9 +318,25 @@ __device__ __forceinline__ void back_atom(const og_back_args& a, int i) {
     for (int d = 0; d < 3; ++d) {
         f[d] = (double)__hip_atomic_load(a.facc + 3 * (size_t)i + d, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) *
                OG_FIX_INV;
-        a.force[3 * (size_t)i + d] = f[d];
         big = big || !(fabs(f[d]) < OG_FIX_LIMIT);
     }
+    if (a.tun.on) {          // CylindricalTunnel::add_force (cotranslate.rs:131-155)
+        const og_tunnel_dev& t = a.tun;
+        const double rx = a.pos[3 * (size_t)i] - t.ox, ry = a.pos[3 * (size_t)i + 1] - t.oy,
+                     rz = a.pos[3 * (size_t)i + 2] - t.oz;
+        const double along = rx * t.dx + ry * t.dy + rz * t.dz;
+        if (!(along < 0.0 || along > t.length)) {
+            const double px = rx - t.dx * along, py = ry - t.dy * along, pz = rz - t.dz * along;
+            const double dist = sqrt(px * px + py * py + pz * pz);
+            if (dist > t.radius) {
+                const double push = t.k * (dist - t.radius);
+                f[0] -= px / dist * push;
+                f[1] -= py / dist * push;
+                f[2] -= pz / dist * push;
+            }
+        }
+    }
+    for (int d = 0; d < 3; ++d) a.force[3 * (size_t)i + d] = f[d];
     const int sys = a.atom_sys[i];
     // a summed force at or beyond the contribution limit: the int64 sum may
     // have wrapped, so the system is reported as out of range

@@ -121,7 +121,17 @@ struct og_cluster_args {
     int n_atoms;
 };
 
+// External cylindrical exit tunnel on every atom (the cotranslate driver's
+// ExternalPotential, cotranslate.rs:105-155): radial pushback -k (d - R)
+// beyond radius R for atoms within the axial extent.
+struct og_tunnel_dev {
+    double ox, oy, oz, dx, dy, dz;  // axis origin and unit direction
+    double radius, length, k;
+    int on;
+};
+
 struct og_back_args {
+    const double* pos;               // read for the external tunnel only
     double* vel; double* force;
     const long long* facc;
     const double* mass;
@@ -134,6 +144,7 @@ struct og_back_args {
     uint32_t* step_dev;              // if set (and do_kick), advanced by one per step
     int* sflag;                      // [S] status bits of this evaluation: with do_kick, a flagged
                                      //   system gets them in diverged and is not kicked
+    og_tunnel_dev tun;               // added to every atom's force when tun.on
 };
 
 void og_launch_pair_descreen(const og_pair_args& a, bool mode_a, hipStream_t s);

@@ -77,6 +77,11 @@ class EngineError(RuntimeError):
     pass
 
 
+class Tunnel(C.Structure):
+    _fields_ = [("origin", C.c_double * 3), ("direction", C.c_double * 3), ("radius_a", C.c_double),
+                ("length_a", C.c_double), ("k_confine", C.c_double)]
+
+
 class MinimizeParams(C.Structure):
     _fields_ = [("max_steps", C.c_int32), ("history", C.c_int32), ("gradient_tol", C.c_double),
                 ("energy_tol", C.c_double), ("max_step_a", C.c_double), ("alpha0", C.c_double),
@@ -119,6 +124,7 @@ def lib():
         L.og_run_noalloc.argtypes = [vp, C.c_int32, _dp]
         L.og_frames_setup.argtypes = [vp, C.c_int32, C.c_int32]
         L.og_strip_tiles.argtypes = [vp, C.c_int32, _ip]
+        L.og_set_tunnel.argtypes = [vp, C.POINTER(Tunnel)]
         L.og_minimize.argtypes = [vp, C.POINTER(MinimizeParams), C.POINTER(MinimizeResult)]
         L.og_frames_read.argtypes = [vp, _dp, C.POINTER(C.c_uint32), C.c_int32, C.POINTER(C.c_int32)]
         L.og_set_temperature.argtypes = [vp, C.c_int32, C.c_double]
@@ -373,6 +379,19 @@ class Engine:
 
     def swap(self, a, b):
         _check(lib().og_swap_systems(self._h, int(a), int(b)))
+
+    def set_tunnel(self, tunnel=None):
+        """External cylindrical exit tunnel for the dynamics (an object with
+        axis_origin, axis_direction, radius_a, length_a, k_confine, e.g.
+        cotranslate.CylindricalTunnel); None removes it.  Set before
+        dynamics_setup."""
+        if tunnel is None:
+            _check(lib().og_set_tunnel(self._h, None))
+            return
+        t = Tunnel((C.c_double * 3)(*map(float, tunnel.axis_origin)),
+                   (C.c_double * 3)(*map(float, tunnel.axis_direction)),
+                   float(tunnel.radius_a), float(tunnel.length_a), float(tunnel.k_confine))
+        _check(lib().og_set_tunnel(self._h, C.byref(t)))
 
     def frames_setup(self, capacity, every):
         """Keep every system's positions after every `every`-th step in a

@@ -472,66 +472,77 @@ class Residue:
         self.pos.append(p)
 
 
+def append_residue(residues: List[Residue], aa: str, phi=DEFAULT_PHI, psi=DEFAULT_PSI,
+                   omega=DEFAULT_OMEGA) -> Residue:
+    """Place residue aa after the last one of residues, at the given backbone
+    dihedrals, from that residue's current coordinates (builder.rs:78-214,
+    append_residue: the co-translational driver extends a chain that has
+    already moved), and append it."""
+    tmpl = residue_templates()
+    idx = len(residues)
+    topo = tmpl[aa]
+    r = Residue(aa)
+    if idx == 0:
+        n = (0.0, 0.0, 0.0)
+        ca = (N_CA, 0.0, 0.0)
+        dx = -CA_C * math.cos(N_CA_C_ANGLE)
+        dy = CA_C * math.sin(N_CA_C_ANGLE)
+        c = (ca[0] + dx, ca[1] + dy, 0.0)
+    else:
+        prev = residues[idx - 1]
+        pn, pca, pc = prev.position("N"), prev.position("CA"), prev.position("C")
+        n = place_atom(pn, pca, pc, PEPTIDE_C_N, CA_C_N_ANGLE, psi)
+        ca = place_atom(pca, pc, n, N_CA, C_N_CA_ANGLE, omega)
+        c = place_atom(pc, n, ca, CA_C, N_CA_C_ANGLE, phi)
+    r.push("N", "N", n)
+    r.push("CA", "C", ca)
+    r.push("C", "C", c)
+    o = place_atom(n, ca, c, C_O, CA_C_O_ANGLE, psi + PI)
+    r.push("O", "O", o)
+    if topo["has_amide_h"] and idx > 0:
+        prev = residues[idx - 1]
+        h = place_atom(prev.position("O"), prev.position("C"), n, N_H_AMIDE,
+                       HN_BOND_ANGLE, PI)
+        r.push("H", "H", h)
+    elif topo["has_amide_h"] and idx == 0:
+        h = place_atom(c, ca, n, N_H_AMIDE, HN_BOND_ANGLE, PI)
+        r.push("H", "H", h)
+    if topo["is_glycine"]:
+        r.push("HA2", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, -CB_DIHEDRAL))
+        r.push("HA3", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, CB_DIHEDRAL))
+    else:
+        r.push("HA", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, -CB_DIHEDRAL))
+    chi = topo["default_chi"]
+    for sc in topo["sidechain"]:
+        def look(nm):
+            p = r.position(nm)
+            if p is None and idx > 0:
+                p = residues[idx - 1].position(nm)
+            if p is None:
+                raise ValueError("residue %d references unplaced atom %s" % (idx, nm))
+            return p
+        pa, pb, pc_ = look(sc["dihedral_to"]), look(sc["angle_at"]), look(sc["bond_to"])
+        dv = sc["dihedral"]
+        if dv["kind"] == "fixed":
+            dih = dv["value"]
+        elif dv["kind"] == "chi":
+            dih = chi[dv["chi"] - 1]
+        else:
+            dih = chi[dv["chi"] - 1] + dv["offset"]
+        r.push(sc["name"], sc["element"],
+               place_atom(pa, pb, pc_, sc["bond_length"], sc["bond_angle"], dih))
+    residues.append(r)
+    return r
+
+
 def build_chain(sequence: Sequence[str], phi=DEFAULT_PHI, psi=DEFAULT_PSI,
                 omega=DEFAULT_OMEGA) -> List[Residue]:
     """Extended all-atom chain (crates/geom/src/builder.rs:50-214)."""
     if len(sequence) == 0:
         raise ValueError("empty sequence")
-    tmpl = residue_templates()
     residues: List[Residue] = []
-    for idx, aa in enumerate(sequence):
-        topo = tmpl[aa]
-        r = Residue(aa)
-        if idx == 0:
-            n = (0.0, 0.0, 0.0)
-            ca = (N_CA, 0.0, 0.0)
-            dx = -CA_C * math.cos(N_CA_C_ANGLE)
-            dy = CA_C * math.sin(N_CA_C_ANGLE)
-            c = (ca[0] + dx, ca[1] + dy, 0.0)
-        else:
-            prev = residues[idx - 1]
-            pn, pca, pc = prev.position("N"), prev.position("CA"), prev.position("C")
-            n = place_atom(pn, pca, pc, PEPTIDE_C_N, CA_C_N_ANGLE, psi)
-            ca = place_atom(pca, pc, n, N_CA, C_N_CA_ANGLE, omega)
-            c = place_atom(pc, n, ca, CA_C, N_CA_C_ANGLE, phi)
-        r.push("N", "N", n)
-        r.push("CA", "C", ca)
-        r.push("C", "C", c)
-        o = place_atom(n, ca, c, C_O, CA_C_O_ANGLE, psi + PI)
-        r.push("O", "O", o)
-        if topo["has_amide_h"] and idx > 0:
-            prev = residues[idx - 1]
-            h = place_atom(prev.position("O"), prev.position("C"), n, N_H_AMIDE,
-                           HN_BOND_ANGLE, PI)
-            r.push("H", "H", h)
-        elif topo["has_amide_h"] and idx == 0:
-            h = place_atom(c, ca, n, N_H_AMIDE, HN_BOND_ANGLE, PI)
-            r.push("H", "H", h)
-        if topo["is_glycine"]:
-            r.push("HA2", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, -CB_DIHEDRAL))
-            r.push("HA3", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, CB_DIHEDRAL))
-        else:
-            r.push("HA", "H", place_atom(c, n, ca, CA_HA, HA_BOND_ANGLE, -CB_DIHEDRAL))
-        chi = topo["default_chi"]
-        for sc in topo["sidechain"]:
-            def look(nm):
-                p = r.position(nm)
-                if p is None and idx > 0:
-                    p = residues[idx - 1].position(nm)
-                if p is None:
-                    raise ValueError("residue %d references unplaced atom %s" % (idx, nm))
-                return p
-            pa, pb, pc_ = look(sc["dihedral_to"]), look(sc["angle_at"]), look(sc["bond_to"])
-            dv = sc["dihedral"]
-            if dv["kind"] == "fixed":
-                dih = dv["value"]
-            elif dv["kind"] == "chi":
-                dih = chi[dv["chi"] - 1]
-            else:
-                dih = chi[dv["chi"] - 1] + dv["offset"]
-            r.push(sc["name"], sc["element"],
-                   place_atom(pa, pb, pc_, sc["bond_length"], sc["bond_angle"], dih))
-        residues.append(r)
+    for aa in sequence:
+        append_residue(residues, aa, phi, psi, omega)
     return residues
 
 

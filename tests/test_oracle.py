@@ -536,6 +536,26 @@ def test_sasa_forces_newton_third_law():
 
 
 # ------------------------------------------------------- integrator --
+# ------------------------------------------------ cotranslate tunnel --
+def test_tunnel_pushes_atoms_inward_and_spares_the_inside():
+    # cotranslate.rs:418-444: radius 5, length 50, k 10; an atom at (6, 0, 25)
+    # overshoots by 1 and feels -10 along x
+    f = O.tunnel_force([[6.0, 0.0, 25.0]], [0, 0, 0], [0, 0, 1], 5.0, 50.0, 10.0)
+    assert np.abs(f[0] - [-10.0, 0.0, 0.0]).max() < 1e-12
+    # cotranslate.rs:446-466: default tunnel (radius 12, length 80, k 50), atom inside
+    f = O.tunnel_force([[2.0, 0.0, 40.0]], [0, 0, 0], [0, 0, 1], 12.0, 80.0, 50.0)
+    assert (f == 0.0).all()
+    # beyond either end of the axis nothing acts; off-axis in a general
+    # direction the force is -k (d - R) along the radial unit vector
+    f = O.tunnel_force([[30.0, 0.0, -1.0], [30.0, 0.0, 81.0]], [0, 0, 0], [0, 0, 1], 12.0, 80.0, 50.0)
+    assert (f == 0.0).all()
+    o, u = np.array([1.0, -2.0, 3.0]), np.array([1.0, 2.0, 2.0]) / 3.0
+    perp = np.array([2.0, -1.0, 0.0]) / math.sqrt(5.0)
+    x = o + 20.0 * u + 7.5 * perp
+    f = O.tunnel_force([x], o, u, 5.0, 50.0, 4.0)
+    assert np.abs(f[0] + 4.0 * 2.5 * perp).max() < 1e-12
+
+
 # ------------------------------------------------ Shrake-Rupley SASA --
 def test_shrake_rupley_dots():
     # fibonacci_unit_sphere (sasa.rs:182-199): unit vectors, y evenly spaced
