@@ -430,7 +430,7 @@ __device__ __forceinline__ float strip_coef(const og_pair_args& a, const StripAt
     return fmaf(qq, gb - inv_r2 * inv_r, lj);
 }
 
-// A strip: two i tiles (64 i atoms each, two per lane) against up to four j
+// A strip: two i tiles (64 i atoms each, two per lane) against up to OG_STRIP_J j
 // tiles of the same system, j data in the wave's LDS as in force_tile_lds.
 // Each step evaluates the lane's two i atoms against one j atom, adds both
 // reactions into the rotating j-side sums and rotates them once, so the
