@@ -440,18 +440,21 @@ __device__ __forceinline__ float strip_coef(const og_pair_args& a, const StripAt
 // j tile of their system, so both are full); distinct atoms at one position
 // give a non-finite coefficient, which the fixed-point conversion reports.
 template <bool MODEA>
-__device__ __forceinline__ void force_strip(const og_pair_args& a, const og_strip& st, int lane,
+__device__ __forceinline__ void force_strip(const og_pair_args& a, const og_strip* sp, int lane,
                                             float4* P, float4* Q) {
-    const int ta = st.ti, tb = st.ti + 1;
+    // fields read through the pointer (the j tile list is indexed at run
+    // time, which a local copy would keep in scratch memory)
+    const int ta = __builtin_amdgcn_readfirstlane(sp->ti), tb = ta + 1;
+    const int nj = __builtin_amdgcn_readfirstlane(sp->nj);
     const int a0 = a.tile_atom0[ta], b0 = a.tile_atom0[tb];
     const int ca = a.tile_cnt[ta], cb = a.tile_cnt[tb];
     const double ox = a.pos[3 * (size_t)a0], oy = a.pos[3 * (size_t)a0 + 1], oz = a.pos[3 * (size_t)a0 + 2];
     StripAtom A = strip_atom(load_lane(a, ta, a0, ca, lane, ox, oy, oz));
     StripAtom B = strip_atom(load_lane(a, tb, b0, cb, lane, ox, oy, oz));
-    int* fl = a.sflag ? a.sflag + st.sys : nullptr;
-    int tj = st.tj[0];
+    int* fl = a.sflag ? a.sflag + sp->sys : nullptr;
+    int tj = __builtin_amdgcn_readfirstlane(sp->tj[0]);
     LaneData dj = load_lane(a, tj, a.tile_atom0[tj], a.tile_cnt[tj], lane, ox, oy, oz);
-    for (int q = 0; q < st.nj; ++q) {
+    for (int q = 0; q < nj; ++q) {
         const int a0j = a.tile_atom0[tj], cj = a.tile_cnt[tj];
         const float4 pjl = make_float4(dj.x, dj.y, dj.z, dj.q);
         const float4 qjl = make_float4(dj.par.x, dj.par.y, dj.gb.z, dj.gb.w);
@@ -461,7 +464,7 @@ __device__ __forceinline__ void force_strip(const og_pair_args& a, const og_stri
         if (lane == 0) { P[OG_TILE] = pjl; Q[OG_TILE] = qjl; }
         wave_lds_order();
         // the next j tile's data load while this one is evaluated
-        const int tn = st.tj[q + 1 < st.nj ? q + 1 : q];
+        const int tn = __builtin_amdgcn_readfirstlane(sp->tj[q + 1 < nj ? q + 1 : q]);
         dj = load_lane(a, tn, a.tile_atom0[tn], a.tile_cnt[tn], lane, ox, oy, oz);
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
         auto step = [&](float4 pj, float4 qj) {
@@ -575,8 +578,7 @@ __device__ __forceinline__ void force_body(const og_pair_args& a) {
     const int total = a.n_strip + a.n_fwork;
     for (int w = wave; w < total; w += n_waves) {
         if (w < a.n_strip) {
-            const og_strip st = a.strips[w];
-            force_strip<MODEA>(a, st, lane, P, Q);
+            force_strip<MODEA>(a, a.strips + w, lane, P, Q);
             continue;
         }
         const og_work it = a.fwork[w - a.n_strip];
@@ -798,8 +800,7 @@ __global__ void __launch_bounds__(OG_PERSIST_THREADS) og_k_replica_steps(og_pair
         }
         __syncthreads();
         for (int w = pp.strip_off[s] + wave; w < pp.strip_off[s + 1]; w += NW) {
-            const og_strip st = pa.strips[w];
-            force_strip<MODEA>(pa, st, lane, P, Q);
+            force_strip<MODEA>(pa, pa.strips + w, lane, P, Q);
         }
         for (int w = pp.fwork_off[s] + wave; w < pp.fwork_off[s + 1]; w += NW) {
             const og_work it = pa.fwork[w];

@@ -1116,20 +1116,25 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     // several-fold spread in cost per atom.  Every wave leaves the loop once
     // the counter has passed the last atom.  (The second pass, with few
     // atoms, is split statically.)
+    // One call site for both parts keeps a single copy of the arrangement
+    // code (about half the kernel's size otherwise).
     const int n_static = second ? n_items : (n_items / 4) * 3;
-    for (int it = slot; it < n_static; it += n_slots) {
-        const int w = second ? d.deferred[it] : it;
+    int it = slot;
+    for (;;) {
+        int w;
+        if (it < n_static) {
+            w = second ? d.deferred[it] : it;
+            it += n_slots;
+        } else {
+            if (second) break;
+            int q = 0;
+            if (lane == 0) q = atomicAdd(d.counters + 4, 1);
+            w = n_static + __shfl(q, 0, 64);
+            if (w >= n_items) break;
+        }
+        w = __builtin_amdgcn_readfirstlane(w);
         const int4 rec = arrange_atom<AREA, NC>(L, d, pos, w, lane, lt, wd, wi);
         if (lane == 0) d.rec[w] = rec;
-        wave_sync();
-    }
-    while (!second) {
-        int it = 0;
-        if (lane == 0) it = atomicAdd(d.counters + 4, 1);
-        it = n_static + __shfl(it, 0, 64);
-        if (it >= n_items) break;
-        const int4 rec = arrange_atom<AREA, NC>(L, d, pos, it, lane, lt, wd, wi);
-        if (lane == 0) d.rec[it] = rec;
         wave_sync();
     }
 }
