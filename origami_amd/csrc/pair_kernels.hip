@@ -618,7 +618,9 @@ void og_launch_descreen_bonded(const og_pair_args& a, const og_bonded_args& b, b
 void og_launch_pair_force(const og_pair_args& a, bool mode_a, hipStream_t s) {
     const int n = a.n_strip + a.n_fwork;
     if (n <= 0) return;
-    const int g = a.grid > 0 ? a.grid : pair_grid(n);
+    // the force pass has fewer, larger work items than descreening (strips):
+    // a width set for both is capped at one wave per item
+    const int g = a.grid > 0 ? std::min(a.grid, (n + 3) / 4) : pair_grid(n);
     if (mode_a) hipLaunchKernelGGL(og_k_force_a, dim3(g), dim3(256), 0, s, a);
     else hipLaunchKernelGGL(og_k_force_b, dim3(g), dim3(256), 0, s, a);
 }
@@ -692,6 +694,34 @@ __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
         wave_lds_order();
         const bool diag = it.ti == it.tj;
         double elj = 0.0, ec = 0.0, eg = 0.0;
+        if (!diag && it.mask < 0) {
+            // off-diagonal tile pair without excluded or 1-4 pairs: the same
+            // terms with the mask and 1-4 selects left out
+            const double xo = vi ? xi : xi + 1.0e6;      // padding lanes: far away, added as 0
+            for (int jj = 0; jj < cj; ++jj) {
+                const EJ d = J[jj];
+                const double dx = xo - d.x, dy = yi - d.y, dz = zi - d.z;
+                const double r2 = fma(dx, dx, fma(dy, dy, dz * dz));
+                const double qq = qi * d.q;
+                const double inv_r = rsqrt64(r2);
+                const double t = (rmi + d.rm) * inv_r;
+                const double t2 = t * t;
+                const double t6 = t2 * t2 * t2;
+                const double lj = (sei * d.se) * fma(t6, t6, -2.0 * t6);
+                double c = qq * inv_r;
+                bool nb = vi;
+                if (MODEA) {
+                    c = qq * (inv_r + e.kappa * r2 - e.crf);
+                    nb = nb && (r2 <= e.rc2_nb);
+                }
+                elj += nb ? lj : 0.0;
+                ec += nb ? c : 0.0;
+                const double x = 0.25 * r2 * (iRi * d.iR);
+                double inv_f = inv_r;
+                if (!(x >= 60.0)) inv_f = rsqrt64(fma(Ri * d.R, exp(-x), r2));
+                eg += vi ? qq * inv_f : 0.0;
+            }
+        } else
         for (int jj = 0; jj < cj; ++jj) {
             const EJ d = J[jj];
             const double dx = xi - d.x, dy = yi - d.y, dz = zi - d.z;
@@ -712,9 +742,15 @@ __device__ void energy_body(const og_pair_args& a, const og_energy64_args& e) {
             if (MODEA) c = qq * (inv_r + e.kappa * r2 - e.crf);
             elj += nb ? lj : 0.0;
             ec += nb ? c : 0.0;
-            const double rp = Ri * d.R;
-            const double f2 = fma(rp, exp(-0.25 * r2 * (iRi * d.iR)), r2);
-            eg += ok ? qq * rsqrt64(f2) : 0.0;
+            // f_GB^2 = r^2 + R_i R_j exp(-x), x = r^2 / (4 R_i R_j).  From
+            // x >= 60 on, R_i R_j exp(-x) < 1e-12 x 2^-52 r^2 (r^2 = 4 x R_i R_j),
+            // far below half an ulp of r^2, so the fma rounds to r^2 exactly
+            // and 1/f_GB is the 1/r above: the exponential and the second
+            // square root are left out there, bit for bit.
+            const double x = 0.25 * r2 * (iRi * d.iR);
+            double inv_f = inv_r;
+            if (!(x >= 60.0)) inv_f = rsqrt64(fma(Ri * d.R, exp(-x), r2));
+            eg += ok ? qq * inv_f : 0.0;
         }
         elj = og_wave_sum_d(elj * OG_KCAL_TO_KJ);
         ec = og_wave_sum_d(ec * kc);
