@@ -168,21 +168,39 @@ __device__ __forceinline__ unsigned dpp_umax(unsigned acc, unsigned src) {
     return acc > m ? acc : m;
 }
 
-// Wave-wide maximum of non-negative floats (all lanes active): their bit
-// patterns order as unsigned integers, so the reduction is an integer one over
-// DPP (rows: shr 1, 2, 3 of the input, shr 4 and 8 within the row, then the
-// row broadcasts 15 and 31) and lane 63 ends with the maximum.  Lanes that a
-// DPP step does not write read 0, the identity.
-__device__ __forceinline__ float wave_max_nonneg(float v) {
-    const unsigned x = __float_as_uint(v);
+// Row-wise (16-lane) reductions over DPP: after the five row-local stages
+// (shr 1, 2, 3 of the input, shr 4 and 8 within the row) lane 15 of each row
+// holds the row's maximum (minimum); lanes a stage does not write read the
+// identity.  All lanes must be active.
+__device__ __forceinline__ unsigned row_umax(unsigned x) {
     unsigned y = dpp_umax<0x111, 0xF, 0xF>(x, x);
     y = dpp_umax<0x112, 0xF, 0xF>(y, x);
     y = dpp_umax<0x113, 0xF, 0xF>(y, x);
     y = dpp_umax<0x114, 0xF, 0xE>(y, y);
-    y = dpp_umax<0x118, 0xF, 0xC>(y, y);
-    y = dpp_umax<0x142, 0xA, 0xF>(y, y);
-    y = dpp_umax<0x143, 0xC, 0xF>(y, y);
-    return __uint_as_float((unsigned)__builtin_amdgcn_readlane((int)y, 63));
+    return dpp_umax<0x118, 0xF, 0xC>(y, y);
+}
+
+template <int CTRL, int ROWS, int BANKS>
+__device__ __forceinline__ unsigned dpp_umin(unsigned acc, unsigned src) {
+    const unsigned m = (unsigned)__builtin_amdgcn_update_dpp(-1, (int)src, CTRL, ROWS, BANKS, false);
+    return acc < m ? acc : m;
+}
+
+__device__ __forceinline__ unsigned row_umin(unsigned x) {
+    unsigned y = dpp_umin<0x111, 0xF, 0xF>(x, x);
+    y = dpp_umin<0x112, 0xF, 0xF>(y, x);
+    y = dpp_umin<0x113, 0xF, 0xF>(y, x);
+    y = dpp_umin<0x114, 0xF, 0xE>(y, y);
+    return dpp_umin<0x118, 0xF, 0xC>(y, y);
+}
+
+// The value lane 15 of the caller's row holds.
+__device__ __forceinline__ unsigned row_lane15(unsigned y, int row) {
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)y, 15);
+    const unsigned r1 = (unsigned)__builtin_amdgcn_readlane((int)y, 31);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)y, 47);
+    const unsigned r3 = (unsigned)__builtin_amdgcn_readlane((int)y, 63);
+    return row == 0 ? r0 : row == 1 ? r1 : row == 2 ? r2 : r3;
 }
 
 // Union cover walk: how far past angle p the arc (start e.x, width e.y) reaches
@@ -593,30 +611,59 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                 iv[p] = make_float2(a, wd2);
             }
             wave_sync();
-            for (int ci = 0; ci < nb; ++ci) {
-                const int k = L.hl[c0 + ci];
-                const float2 e0 = lane < nper ? iv[ci * nper + lane] : make_float2(0.0f, -1.0f);
-                const float2 e1 = lane + 64 < nper ? iv[ci * nper + lane + 64] : make_float2(0.0f, -1.0f);
-                bool covered = __ballot(e0.y >= SA_COV_FULL || e1.y >= SA_COV_FULL) != 0;
-                if (!covered) {
-                    const float wmax = wave_max_nonneg(fmaxf(fmaxf(e0.y, e1.y), 0.0f));
-                    if (wmax > 0.0f) {
-                        const unsigned long long b0 = __ballot(e0.y == wmax), b1 = __ballot(e1.y == wmax);
-                        const int l1 = b0 ? __builtin_ctzll(b0) : __builtin_ctzll(b1);
-                        const float a1 = __shfl(b0 ? e0.x : e1.x, l1, 64);
-                        const float start = a1 + wmax, len = 6.28318530717958647692f - wmax;
-                        float t = 0.0f;
-                        for (int step = 0; step <= nper + 1; ++step) {
-                            const float p = start + t;
-                            const float ext = nper > 64 ? fmaxf(cover_ext(p, e0), cover_ext(p, e1)) : cover_ext(p, e0);
-                            const float best = wave_max_nonneg(ext);
-                            if (!(best > 0.0f)) break;
+            // The walk, four circles at a time, one per 16-lane row (lane l
+            // of row r holds arcs l, l + 16, ... of circle g0 + r): each
+            // circle takes exactly its own steps, from the end of its widest
+            // arc (the first of equal widths), each step to the furthest
+            // reach of the arcs that contain the current angle with
+            // SA_COV_EPS to spare; rows that are done stay as they are.
+            const int nslot = (nper + 15) >> 4;                    // <= 8 (nper <= MAXC - 1)
+            const int row = lane >> 4, lr = lane & 15;
+            for (int g0 = 0; g0 < nb; g0 += 4) {
+                const int ci = g0 + row;
+                const bool rv = ci < nb;
+                float2 e[8];
+                bool full = false;
+                float wl = 0.0f;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int mi = lr + 16 * q;
+                    e[q] = (q < nslot && rv && mi < nper) ? iv[ci * nper + mi] : make_float2(0.0f, -1.0f);
+                    full = full || e[q].y >= SA_COV_FULL;
+                    wl = fmaxf(wl, e[q].y);
+                }
+                const bool rfull = ((__ballot(full) >> (16 * row)) & 0xFFFFull) != 0;
+                const float wm = __uint_as_float(row_lane15(row_umax(__float_as_uint(wl)), row));
+                unsigned first = 0xFFFFu;
+#pragma unroll
+                for (int q = 7; q >= 0; --q)
+                    if (e[q].y == wm) first = (unsigned)(lr + 16 * q);
+                first = row_lane15(row_umin(first), row);
+                bool act = rv && !rfull && wm > 0.0f;
+                bool cov = rv && rfull;
+                float start = 0.0f, len = 0.0f, t = 0.0f;
+                if (act) {
+                    start = iv[ci * nper + (int)first].x + wm;
+                    len = 6.28318530717958647692f - wm;
+                }
+                for (int step = 0; step <= nper + 1; ++step) {
+                    if (!__ballot(act)) break;
+                    const float p = start + t;
+                    float ext = 0.0f;
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q < nslot) ext = fmaxf(ext, cover_ext(p, e[q]));
+                    const float best = __uint_as_float(row_lane15(row_umax(__float_as_uint(act ? ext : 0.0f)), row));
+                    if (act) {
+                        if (!(best > 0.0f)) {
+                            act = false;
+                        } else {
                             t += best;
-                            if (t >= len + SA_COV_EPS) { covered = true; break; }
+                            if (t >= len + SA_COV_EPS) { cov = true; act = false; }
                         }
                     }
                 }
-                if (covered && lane == 0) L.skip[k] = 2;
+                if (lr == 0 && cov) L.skip[L.hl[c0 + ci]] = 2;
             }
             wave_sync();
         }
