@@ -43,6 +43,17 @@ OG_HD inline v3 normalize(v3 a) { return a / norm(a); }
 
 struct Cap { v3 axis; double ca; };
 
+// Division of a pair index by the number of other caps without an integer
+// divide: q = floor(p * ceil(2^24 / n) / 2^24).  The product overshoots
+// p / n by less than p / 2^24 < 1/256, while the fractional part of p / n is
+// at most 1 - 1/n < 1 - 1/256, so the floor is exact for p < 65536 and
+// 1 <= n < 256 (checked exhaustively on the host).
+struct PairDiv { int n; unsigned inv; };
+OG_HD inline PairDiv pair_div(int n) { return {n, (unsigned)((((1u << 24) + (unsigned)n - 1u)) / (unsigned)n)}; }
+OG_HD inline int pair_q(const PairDiv& d, int p) {
+    return (int)(((unsigned long long)(unsigned)p * d.inv) >> 24);
+}
+
 OG_HD inline Cap small_circle_new(v3 axis, double ca) { return {normalize(axis), ca}; }
 OG_HD inline bool contains_strict(const Cap& c, v3 p) { return dot(p, c.axis) > c.ca + 1e-12; }
 

@@ -527,10 +527,11 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     for (int i = lane; i < n_h; i += 64) has[i] = 0;
     wave_sync();
     const int nper = nc > 1 ? nc - 1 : 1;
+    const PairDiv pdiv = pair_div(nper);                  // pair indices stay below 65536 (NC <= 128)
     for (int base = 0; base < n_h * nper; base += 64) {
         const int p = base + lane;
         if (p < n_h * nper && nc > 1) {
-            const int i = p / nper, mi = p % nper;
+            const int i = pair_q(pdiv, p), mi = p - i * nper;
             const int k = L.hl[i];
             const Cap ck = cap_at(L, k), chint = cap_at(L, L.hint[k]);
             const double ct = dot(ck.axis, chint.axis);
@@ -602,7 +603,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             }
             wave_sync();
             for (int p = lane; p < nb * nper; p += 64) {
-                const int ci = p / nper, mi = p % nper;
+                const int ci = pair_q(pdiv, p), mi = p - ci * nper;
                 const int k = L.hl[c0 + ci];
                 const int m = mi < k ? mi : mi + 1;
                 const double* o = bas + 6 * ci;
@@ -685,7 +686,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         for (int i = lane; i < n_act; i += 64) c32[i] = 0;
         wave_sync();
         for (int p = lane; p < n_act * nper && nc > 1; p += 64) {
-            const int i = p / nper, mi = p % nper;
+            const int i = pair_q(pdiv, p), mi = p - i * nper;
             const int k = L.act[i];
             const int n = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1));
             if (n) atomicAdd(c32 + i, (unsigned)n);
@@ -732,7 +733,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             bool emit = false;
             int code = 0;
             if (p < npairs) {
-                const int kl = p / nper, mi = p % nper;
+                const int kl = pair_q(pdiv, p), mi = p - kl * nper;
                 const int k = L.act[a0 + kl];
                 if (L.cnt[k] == 0) emit = mi == 0;
                 else if (nc > 1) emit = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1)) != 0;

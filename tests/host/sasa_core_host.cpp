@@ -7,6 +7,16 @@ using namespace og_sasa;
 
 extern "C" {
 
+// Mismatches of pair_q against integer division over p < 65536, n in [1, 255].
+long long host_pair_div_mismatches(void) {
+    long long bad = 0;
+    for (int n = 1; n < 256; ++n) {
+        const PairDiv d = pair_div(n);
+        for (int p = 0; p < 65536; ++p) bad += pair_q(d, p) != p / n;
+    }
+    return bad;
+}
+
 // counts[i] = intersect_count, fast[i] = intersect_count_fast for cap pairs
 // (axis1, ca1, axis2, ca2) packed as 8 doubles each.
 void host_intersect_counts(int n, const double* caps, int* counts, int* fast) {

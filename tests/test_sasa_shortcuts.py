@@ -335,3 +335,10 @@ def test_jvp_translation_invariance(host):
         # the negated sum, as the kernel forms it, against the direct tangent
         direct, formed = t[:, 0], -t[:, 1:].sum(axis=1)
         assert np.allclose(formed, direct, rtol=1e-9, atol=1e-13)
+
+
+def test_pair_index_division_exact(host):
+    # the arrangement kernel decodes (circle, other cap) pair indices with a
+    # reciprocal multiply; it must equal integer division everywhere it is used
+    host.host_pair_div_mismatches.restype = C.c_longlong
+    assert host.host_pair_div_mismatches() == 0
