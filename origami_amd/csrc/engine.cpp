@@ -124,6 +124,7 @@ struct og_engine {
     long long frame_head = 0, frame_tail = 0;   // frames written / read
     std::vector<uint32_t> frame_step;      // step number held by each ring slot
     int pair_grid = 0;                     // autotuned pair-kernel width (0: default)
+    float ds_far = 0.0f;                   // descreening far-field distance (og_pair_args::ds_far)
     std::vector<double> mass_h;            // host copies kept for the surrogate set-up
     std::vector<int> sys_tile_off_h;
     // dynamics
@@ -702,6 +703,13 @@ static int engine_build(og_engine* e, const og_system_desc* sys, int n_sys) {
     HIPCHK(upload(&e->atom_slot, a_slot));
     HIPCHK(upload(&e->par, par));
     HIPCHK(upload(&e->gb, gbv));
+    {
+        // the series of the HCT term needs S / r <= 0.1 and the separated
+        // branch (r - S >= rho~) for every pair of a far tile pair
+        float s_max = 0.0f, rt_max = 0.0f;
+        for (const float4& g : gbv) { s_max = std::max(s_max, g.y); rt_max = std::max(rt_max, g.x); }
+        e->ds_far = std::max(10.0f * s_max, 1.25f * (rt_max + s_max));
+    }
     HIPCHK(upload(&e->qs, qs));
     HIPCHK(upload(&e->tile_atom0, tile_atom0));
     HIPCHK(upload(&e->tile_cnt, tile_cnt));
@@ -805,6 +813,7 @@ static og_pair_args pair_args(og_engine* e, double cutoff_a) {
     a.tile_atom0 = e->tile_atom0; a.tile_cnt = e->tile_cnt;
     a.work = e->work; a.n_work = e->n_work; a.masks = e->masks;
     a.strips = e->strips; a.n_strip = e->n_strip; a.fwork = e->fwork; a.n_fwork = e->n_fwork;
+    a.ds_far = e->ds_far;
     a.facc = e->facc; a.iacc = e->iacc; a.eacc = e->eacc;
     a.grid = e->pair_grid;
     a.sflag = e->sflag;

@@ -32,6 +32,10 @@ struct og_pair_args {
     int n_strip;
     const og_work* fwork;
     int n_fwork;
+    // descreening: tile pairs whose atoms are all at least ds_far apart take
+    // the far-field series of the HCT term (ds_far >= 10 S and >= 1.25
+    // (rho~ + S) for every atom; 0 switches the series off)
+    float ds_far;
 };
 
 // energy slots per system

@@ -168,6 +168,59 @@ __device__ __forceinline__ void descreen_tile_lds(const og_pair_args& a, const L
     acc_j += gj;
 }
 
+// Far-field form of descreen_tile_lds for tile pairs whose atoms are all at
+// least og_pair_args::ds_far apart: there S / r <= 0.1 and L = r - S, and
+// the HCT term (1/r)[x/(1-x^2) - x^3/(1-x^2)^2 - atanh x], x = S/r, is the
+// series (1/r) sum_n c_n x^(2n+1), c_n = 1 - n - 1/(2n+1):
+//   h = -(S^3 / (3 r^4)) (1 + 3.6 y + (45/7) y^2 + (28/3) y^3),  y = x^2,
+// whose next term, (135/11) y^4 relative, is below 1.3e-7 at x = 0.1 (FP32
+// rounding is 6e-8); the closed form evaluated in FP32 there loses about
+// 6e-8 / x^2 to cancellation.  Every term is negative, as h is.  Q[e].z holds
+// -S^3/3 of j atom e.
+template <bool MODEA, int PART>
+__device__ __forceinline__ void descreen_tile_far(const og_pair_args& a, const LaneData& di,
+                                                  const float4* P, const float4* Q, int lane,
+                                                  float& acc_i, float& acc_j) {
+    float gj = 0.0f;
+    const float Si2 = di.gb.y * di.gb.y;
+    const float ci3 = (-1.0f / 3.0f) * Si2 * di.gb.y;
+    auto series = [](float y) { return fmaf(y, fmaf(y, fmaf(y, 28.0f / 3.0f, 45.0f / 7.0f), 3.6f), 1.0f); };
+    auto step = [&](float4 pj, float4 qj) {
+        const float dx = pj.x - di.x, dy = pj.y - di.y, dz = pj.z - di.z;
+        const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+        const float inv_r = __builtin_amdgcn_rsqf(r2);
+        const float inv_r2 = inv_r * inv_r;
+        const float inv_r4 = inv_r2 * inv_r2;
+        const float hij = (qj.z * inv_r4) * series(qj.y * inv_r2);
+        const float hji = (ci3 * inv_r4) * series(Si2 * inv_r2);
+        if (MODEA) {
+            const bool ok = r2 <= a.rc2_ds;
+            acc_i += ok ? hij : 0.0f;
+            gj += ok ? hji : 0.0f;
+        } else {
+            acc_i += hij;
+            gj += hji;
+        }
+        gj = rot(gj);                  // after step 63 this is the way home
+    };
+    constexpr int kb = PART == 1 ? 32 : 0, ke = PART == 0 ? 32 : 64;
+    for (int k = kb; k < ke; k += 2) {
+        const int idx = (lane + k) & 63;
+        const float4 p0 = P[idx], p1 = P[idx + 1];
+        const float4 q0 = Q[idx], q1 = Q[idx + 1];
+        step(p0, q0);
+        step(p1, q1);
+    }
+    if (PART == 0) gj = swap32(gj);
+    acc_j += gj;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
 // One descreening work item (a tile pair, or half of one) of the wave; P, Q:
 // the wave's LDS staging.
 template <bool MODEA>
@@ -185,13 +238,28 @@ __device__ __forceinline__ void descreen_item(const og_pair_args& a, const og_wo
         } else {
             const LaneData dj = load_lane(a, it.tj, a0j, cj, lane, ox, oy, oz);
             const float4 pj = make_float4(dj.x, dj.y, dj.z, dj.gb.x);
-            const float4 qj = make_float4(dj.gb.y, dj.gb.y * dj.gb.y, 0.0f, 0.0f);
+            const float4 qj = make_float4(dj.gb.y, dj.gb.y * dj.gb.y, (-1.0f / 3.0f) * (dj.gb.y * dj.gb.y) * dj.gb.y,
+                                          0.0f);
             wave_lds_order();                       // the previous tile's reads are done
             P[lane] = pj;
             Q[lane] = qj;
             if (lane == 0) { P[OG_TILE] = pj; Q[OG_TILE] = qj; }
             wave_lds_order();
-            if (it.part < 0) descreen_tile_lds<MODEA, -1>(a, di, P, Q, lane, acc_i, acc_j);
+            // far tile pair: the distance of the two first atoms less the
+            // radii of both tiles about them (positions are relative to atom
+            // a0i, so lane 0's i position is the origin) bounds every pair's
+            // distance from below; 1e-3 A covers the FP32 rounding
+            const float jx0 = rl(dj.x, 0), jy0 = rl(dj.y, 0), jz0 = rl(dj.z, 0);
+            const float ri = lane < ci ? sqrtf(fmaf(di.x, di.x, fmaf(di.y, di.y, di.z * di.z))) : 0.0f;
+            const float ex = dj.x - jx0, ey = dj.y - jy0, ez = dj.z - jz0;
+            const float rj = lane < cj ? sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez))) : 0.0f;
+            const float d0 = sqrtf(fmaf(jx0, jx0, fmaf(jy0, jy0, jz0 * jz0)));
+            const bool far = a.ds_far > 0.0f && d0 - wave_max_f(ri) - wave_max_f(rj) - 1e-3f >= a.ds_far;
+            if (far) {
+                if (it.part < 0) descreen_tile_far<MODEA, -1>(a, di, P, Q, lane, acc_i, acc_j);
+                else if (it.part == 0) descreen_tile_far<MODEA, 0>(a, di, P, Q, lane, acc_i, acc_j);
+                else descreen_tile_far<MODEA, 1>(a, di, P, Q, lane, acc_i, acc_j);
+            } else if (it.part < 0) descreen_tile_lds<MODEA, -1>(a, di, P, Q, lane, acc_i, acc_j);
             else if (it.part == 0) descreen_tile_lds<MODEA, 0>(a, di, P, Q, lane, acc_i, acc_j);
             else descreen_tile_lds<MODEA, 1>(a, di, P, Q, lane, acc_i, acc_j);
         }
