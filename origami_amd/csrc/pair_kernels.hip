@@ -474,7 +474,14 @@ __device__ __forceinline__ StripAtom strip_atom(const LaneData& d) {
 // LJ + Coulomb + GB-pair force for an unmasked pair; j data as in the LDS
 // layout of force_tile_lds (P = x, y, z, q; Q = se, rm, R, 1/R scaled).
 // Mode B groups the two charge terms: q_i q_j (k_GB (1 - e/4) / f^3 - 1/r^3).
-template <bool MODEA>
+// FAR: every pair of the j tile has r^2 gw_i gw_j >= 32 (gw = sqrt(log2 e / 4)
+// / R), so e = exp2(-r^2 gw_i gw_j) <= 2^-32.  Then R_i R_j e is below 1e-11
+// r^2 (r^2 = 4 R_i R_j x, x = r^2 gw_i gw_j / log2 e >= 22), far under half an
+// FP32 ulp of r^2, so f^2 = fma(R_i R_j, e, r^2) rounds to r^2 and 1/f is the
+// 1/r above; likewise k (1 - e/4) rounds to k.  The exponential and the
+// second square root are left out, with the same products in the same order:
+// the coefficient is bit-identical.
+template <bool MODEA, bool FAR = false>
 __device__ __forceinline__ float strip_coef(const og_pair_args& a, const StripAtom& I, float4 pj, float4 qj,
                                             float dx, float dy, float dz) {
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
@@ -484,11 +491,16 @@ __device__ __forceinline__ float strip_coef(const og_pair_args& a, const StripAt
     const float t2 = rmin * rmin * inv_r2;
     const float t6 = t2 * t2 * t2;
     const float lj = (I.se * qj.x) * inv_r2 * fmaf(-t6, t6, t6);
-    const float e = __builtin_amdgcn_exp2f(-r2 * (I.gw * qj.w));
-    const float f2 = fmaf(I.R * qj.z, e, r2);
-    const float inv_f = __builtin_amdgcn_rsqf(f2);
     const float qq = I.q * pj.w;
-    const float gb = fmaf(-0.25f * kGbK, e, kGbK) * (inv_f * inv_f * inv_f);
+    float gb;
+    if (FAR) {
+        gb = kGbK * (inv_r * inv_r * inv_r);
+    } else {
+        const float e = __builtin_amdgcn_exp2f(-r2 * (I.gw * qj.w));
+        const float f2 = fmaf(I.R * qj.z, e, r2);
+        const float inv_f = __builtin_amdgcn_rsqf(f2);
+        gb = fmaf(-0.25f * kGbK, e, kGbK) * (inv_f * inv_f * inv_f);
+    }
     if (MODEA) {
         float cnb = fmaf(-qq, inv_r2 * inv_r - a.inv_rc3, lj);
         cnb = (r2 <= a.rc2_nb) ? cnb : 0.0f;
@@ -520,6 +532,11 @@ __device__ __forceinline__ void force_strip(const og_pair_args& a, const og_stri
     StripAtom A = strip_atom(load_lane(a, ta, a0, ca, lane, ox, oy, oz));
     StripAtom B = strip_atom(load_lane(a, tb, b0, cb, lane, ox, oy, oz));
     int* fl = a.sflag ? a.sflag + sp->sys : nullptr;
+    // bounds of the two i tiles about the origin (atom a0; both tiles are
+    // full) and their smallest gw, for the far test of each j tile
+    const float rad_i = wave_max_f(fmaxf(sqrtf(fmaf(A.x, A.x, fmaf(A.y, A.y, A.z * A.z))),
+                                         sqrtf(fmaf(B.x, B.x, fmaf(B.y, B.y, B.z * B.z)))));
+    const float gw_i = -wave_max_f(-fminf(A.gw, B.gw));
     int tj = __builtin_amdgcn_readfirstlane(sp->tj[0]);
     LaneData dj = load_lane(a, tj, a.tile_atom0[tj], a.tile_cnt[tj], lane, ox, oy, oz);
     for (int q = 0; q < nj; ++q) {
@@ -531,15 +548,28 @@ __device__ __forceinline__ void force_strip(const og_pair_args& a, const og_stri
         Q[lane] = qjl;
         if (lane == 0) { P[OG_TILE] = pjl; Q[OG_TILE] = qjl; }
         wave_lds_order();
+        // far j tile: the smallest pair distance (distance of the first atoms
+        // less both tiles' radii, 1e-3 A for FP32 rounding) gives
+        // r^2 gw_i gw_j >= 33 for every pair (padding slots excluded)
+        bool far;
+        {
+            const float jx0 = rl(dj.x, 0), jy0 = rl(dj.y, 0), jz0 = rl(dj.z, 0);
+            const float ex = dj.x - jx0, ey = dj.y - jy0, ez = dj.z - jz0;
+            const float rad_j = wave_max_f(lane < cj ? sqrtf(fmaf(ex, ex, fmaf(ey, ey, ez * ez))) : 0.0f);
+            const float gw_j = -wave_max_f(lane < cj ? -dj.gb.w : -1.0e30f);
+            const float rmn = sqrtf(fmaf(jx0, jx0, fmaf(jy0, jy0, jz0 * jz0))) - rad_i - rad_j - 1e-3f;
+            far = rmn > 0.0f && rmn * rmn * (gw_i * gw_j) >= 33.0f;
+        }
         // the next j tile's data load while this one is evaluated
         const int tn = __builtin_amdgcn_readfirstlane(sp->tj[q + 1 < nj ? q + 1 : q]);
         dj = load_lane(a, tn, a.tile_atom0[tn], a.tile_cnt[tn], lane, ox, oy, oz);
         float gx = 0.0f, gy = 0.0f, gz = 0.0f;
-        auto step = [&](float4 pj, float4 qj) {
+        auto step = [&](float4 pj, float4 qj, auto far_tag) {
+            constexpr bool F = decltype(far_tag)::value;
             const float dxa = pj.x - A.x, dya = pj.y - A.y, dza = pj.z - A.z;
             const float dxb = pj.x - B.x, dyb = pj.y - B.y, dzb = pj.z - B.z;
-            const float c_a = strip_coef<MODEA>(a, A, pj, qj, dxa, dya, dza);
-            const float c_b = strip_coef<MODEA>(a, B, pj, qj, dxb, dyb, dzb);
+            const float c_a = strip_coef<MODEA, F>(a, A, pj, qj, dxa, dya, dza);
+            const float c_b = strip_coef<MODEA, F>(a, B, pj, qj, dxb, dyb, dzb);
             A.fx = fmaf(c_a, dxa, A.fx);
             A.fy = fmaf(c_a, dya, A.fy);
             A.fz = fmaf(c_a, dza, A.fz);
@@ -550,12 +580,22 @@ __device__ __forceinline__ void force_strip(const og_pair_args& a, const og_stri
             gy = rot(fmaf(-c_b, dyb, fmaf(-c_a, dya, gy)));
             gz = rot(fmaf(-c_b, dzb, fmaf(-c_a, dza, gz)));
         };
-        for (int k = 0; k < OG_TILE; k += 2) {
-            const int idx = (lane + k) & 63;
-            const float4 p0 = P[idx], q0 = Q[idx];
-            const float4 p1 = P[idx + 1], q1 = Q[idx + 1];
-            step(p0, q0);
-            step(p1, q1);
+        if (far) {
+            for (int k = 0; k < OG_TILE; k += 2) {
+                const int idx = (lane + k) & 63;
+                const float4 p0 = P[idx], q0 = Q[idx];
+                const float4 p1 = P[idx + 1], q1 = Q[idx + 1];
+                step(p0, q0, std::true_type{});
+                step(p1, q1, std::true_type{});
+            }
+        } else {
+            for (int k = 0; k < OG_TILE; k += 2) {
+                const int idx = (lane + k) & 63;
+                const float4 p0 = P[idx], q0 = Q[idx];
+                const float4 p1 = P[idx + 1], q1 = Q[idx + 1];
+                step(p0, q0, std::false_type{});
+                step(p1, q1, std::false_type{});
+            }
         }
         if (lane < cj) {
             long long* f = a.facc + 3 * (size_t)(a0j + lane);
