@@ -388,7 +388,10 @@ def main():
             "roof": "FP32 vector ALU, 157.3 TFLOP/s (MI355X FP32 peak)", "achieved": ach,
             "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": ach / FP32_PEAK_TFLOPS,
             "traffic": None, "flops_per_pair": FLOPS_FORCE, "pairs_per_launch": pairs,
-            "avg_ms": k_ms["force"]}
+            "avg_ms": k_ms["force"],
+            "flops_note": ("algorithmic flops of the pair term for every pair; j tiles whose every pair "
+                           "has exp(-r^2/(4 R_i R_j)) <= 2^-32 leave out the GB exponential and the second "
+                           "reciprocal square root, which provably cannot change the FP32 result there")}
     # HBM bytes per launch of the force kernel, from the rocprofv3 --pmc runs
     # summarised in profiles/ (same kernel, same 157,072-atom batch)
     tf = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",

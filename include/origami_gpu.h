@@ -255,18 +255,19 @@ int og_frames_read(og_engine* e, double* xyz, uint32_t* steps, int32_t max_frame
  * Every mode gives bit-identical trajectories. */
 int og_use_graphs(og_engine* e, int32_t on);
 
-/* Autotuner (no counterpart in origami): times the candidate launch widths
- * of the all-pairs kernels (2..32 blocks of 256 threads per CU) and of the
- * SASA arrangement kernel (2..8 blocks of 2 waves per CU) at the current
- * positions and keeps the fastest.  With OG_SASA in flags the widths are
- * timed on the whole force evaluation (SASA on its own stream beside the
- * pair kernels), with OG_SURROGATE (surrogate enabled) on the evaluation
- * with learned radii.  choice[2] (optional) receives the pair and SASA block
- * counts, best_ms[2] the pair kernel's time alone and the best evaluation's.
- * It also times the force pass's strip length (og_strip_tiles: 2, 4, 8 j
- * tiles) with the pair widths and keeps the fastest pair.  Other than the
- * strip length (FP32 rounding of the per-strip partial sums), results do not
- * change: accumulation is order-independent fixed point. */
+/* Autotuner (no counterpart in origami): at the current positions, times
+ * the force pass's strip length (og_strip_tiles: 2, 4, 8 j tiles; the pass
+ * runs one wave per work item), the descreening pass's launch width (2..32
+ * blocks of 256 threads per CU) and the SASA arrangement kernel's (2..8
+ * blocks of 2 waves per CU), each candidate twice, and keeps the fastest.
+ * With OG_SASA in flags the arrangement width, and then the descreening
+ * width again, are timed on the whole force evaluation (SASA on its own
+ * stream beside the pair kernels), with OG_SURROGATE (surrogate enabled) on
+ * the evaluation with learned radii.  choice[2] (optional) receives the pair
+ * and SASA block counts, best_ms[2] the force and descreening passes' time
+ * alone and the best evaluation's.  Other than the strip length (FP32
+ * rounding of the per-strip partial sums), results do not change:
+ * accumulation is order-independent fixed point. */
 int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double* best_ms);
 
 /* Launch widths without timing (no counterpart in origami), e.g. to restore

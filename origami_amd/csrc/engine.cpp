@@ -1470,22 +1470,33 @@ extern "C" int og_autotune(og_engine* e, uint32_t flags, int32_t* choice, double
         return OG_OK;
     };
     const bool mode_a = flags & OG_MODE_A;
+    // the force pass's tile shape (j tiles per strip; the pass runs one wave
+    // per work item), then the descreening pass's launch width; each
+    // candidate timed twice, the faster kept
     int best_grid = 0, best_strip = e->strip_j;
-    float best_pair = 1e30f;
-    // the force pass's tile shape (j tiles per strip) with its launch width
+    float best_force = 1e30f, best_desc = 1e30f;
     for (int sj : {2, 4, 8}) {
         if (int rc = build_force_lists(e, sj)) return rc;
-        for (int per_cu : {2, 4, 8, 16, 32}) {
-            const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
-            og_pair_args a = pair_args(e, 10.0);
-            a.grid = g;
+        og_pair_args a = pair_args(e, 10.0);
+        for (int rep = 0; rep < 2; ++rep) {
             float ms = 0.0f;
             if (int rc = timed([&] { og_launch_pair_force(a, mode_a, e->stream); }, ms)) return rc;
-            if (ms < best_pair) { best_pair = ms; best_grid = g; best_strip = sj; }
-            if (g == std::max((e->n_work + 3) / 4, 1)) break;
+            if (ms < best_force) { best_force = ms; best_strip = sj; }
         }
     }
     if (int rc = build_force_lists(e, best_strip)) return rc;
+    for (int per_cu : {2, 4, 8, 16, 32}) {
+        const int g = std::min(per_cu * n_cu, std::max((e->n_work + 3) / 4, 1));
+        og_pair_args a = pair_args(e, 10.0);
+        a.grid = g;
+        for (int rep = 0; rep < 2; ++rep) {
+            float ms = 0.0f;
+            if (int rc = timed([&] { og_launch_pair_descreen(a, mode_a, e->stream); }, ms)) return rc;
+            if (ms < best_desc) { best_desc = ms; best_grid = g; }
+        }
+        if (g == std::max((e->n_work + 3) / 4, 1)) break;
+    }
+    const float best_pair = best_force + best_desc;
     e->pair_grid = best_grid;
     int best_sasa = og_sasa_max_blocks(&e->sasa);
     float best_s = 0.0f;
