@@ -1,4 +1,5 @@
-"""SASA forces and areas on the 157 072-atom batch, pinned bit for bit.
+"""SASA forces and areas, and the pair and bonded forces, on the 157 072-atom
+batch, pinned bit for bit.
 
 The arrangement kernel's shortcuts (skip, cover and burial tests, the code
 layout of its loops) only decide which work is left out, so they must leave
@@ -16,6 +17,12 @@ on the 64 seeded chains (10-300 residues) with seeded 0.02 Å jitter, in
 both arrangement pass forms.  The digests were taken from the arrangement
 kernel before its loop and walk restructuring; any change to the emitted
 arrangement changes them.
+
+The forces without SASA (bonded + LJ + Coulomb + GB, all pairs, strips of
+four j tiles) are pinned the same way: the far-field shortcuts of the pair
+passes (the descreening series, whose radii still clamp to rho~, and the
+force pass's exponential left out where it is below 2^-32) were checked
+bit-identical against the passes without them when this digest was taken.
 """
 import hashlib
 import os
@@ -28,6 +35,7 @@ pytestmark = pytest.mark.gpu
 
 SASA_FORCE_SHA256 = "52d648634e785ef297b08a149f920f00377f73cfdc7e9cfef532d4478e1ea1cf"
 SASA_ENERGY_SHA256 = "855dde84ae82951a81f2aa17bd4ffb70a1257a946efbb6824928eb41166cb6fa"
+PAIR_FORCE_SHA256 = "d0329f8cfeaf8d2c83f34a03d9cd12ab2d17a69fb67aaac9c43338421c759215"
 
 
 @pytest.fixture(scope="module")
@@ -52,3 +60,13 @@ def test_sasa_forces_and_areas_bit_identical(batch, passes, blocks_per_cu):
     print("SASA |F| sum %.6f, energy sum %.6f" % (np.abs(f_sasa).sum(), e_sasa.sum()))
     assert hashlib.sha256(f_sasa.tobytes()).hexdigest() == SASA_FORCE_SHA256
     assert hashlib.sha256(e_sasa.tobytes()).hexdigest() == SASA_ENERGY_SHA256
+
+
+def test_pair_forces_bit_identical(batch):
+    E, systems, x = batch
+    eng = E.Engine(systems)
+    eng.strip_tiles(4)
+    f = eng.total_force(x)
+    digest = hashlib.sha256(f.tobytes()).hexdigest()
+    print("pair + bonded forces sha256", digest)
+    assert digest == PAIR_FORCE_SHA256
