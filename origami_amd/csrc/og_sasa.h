@@ -32,8 +32,8 @@ struct og_sasa_data {
     double* scratch = nullptr;    // per-wave FP64 staging slots
     int* iscratch = nullptr;      // per-wave int staging slots
     int* overflow = nullptr;      // [1] count of atoms whose arrangement exceeded capacity
-    int* counters = nullptr;      // [8] pool fill: arcs, vertices, cap owners; deferred atoms;
-                                  //     [4] work queue of the arrangement's dynamic share
+    int* counters = nullptr;      // pool fill: arcs, vertices, cap owners [0..2]; deferred
+                                  //     atoms [3]; the arrangement's work queues [64 (g + 1)]
     int* deferred = nullptr;      // [n_work] atoms with more caps than the main pass holds
     int4* rec = nullptr;          // [n_work] (arc_off, vtx_off, owner_off, na | nv<<9 | nc<<18)
     // pool

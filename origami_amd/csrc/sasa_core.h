@@ -125,7 +125,10 @@ OG_HD int intersect_count_fast(const Cap& c1, const Cap& c2) {
 // where they are well conditioned (|cos| <= 1 - 1e-3, amplitude >= 1e-3),
 // nearly full arcs are cut to width 2 (pi - 0.05) and tiny ones dropped, so
 // both ends lie within 2e-6 rad of the exact arc or inside it.
-#define SA_COV_DELTA 1e-6
+// SA_COV_DELTA is also the depth the arrangement kernel requires before it
+// leaves a covered circle's transitions out of the other circles' boundaries
+// (it exceeds the 2.6e-4 that chains of deduplicated points can span).
+#define SA_COV_DELTA 1e-3
 #define SA_COV_FULL 100.0f
 // The containment value on the circle is g + SA_COV_DELTA + amp cos(phi -
 // phi_m) with g = B - SA_COV_DELTA; the whole-circle and no-arc cases are

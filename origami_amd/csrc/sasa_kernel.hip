@@ -55,6 +55,9 @@
 #define SA_COV_CAPS (SA_TMAX / 6)     // union cover test: circles per batch (bases in ta)
 #define SA_COV_PAIRS (3 * SA_TMAX)    // union cover test: arcs per batch (in tx..tz)
 #define SA_COV_EPS 1e-5f              // cover margin on angles (rad)
+#define SA_QUEUES 8                   // arrangement work queues (one per XCD)
+#define SA_QSTRIDE 64                 // ints between queue counters (256 bytes)
+#define SA_NCOUNTERS (SA_QSTRIDE * (SA_QUEUES + 1))
 
 // per-wave global staging (arcs and vertices of the atom in progress)
 #define WD_ARC 0                      // MAXA x (sx, sy, sz, ex, ey, ez, theta)
@@ -146,8 +149,9 @@ __device__ __forceinline__ bool bound_inside(v3 p, const Cap& ck, double ct, dou
 // sine and cosine come from sincosf (a few FP32 ulp, plus the FP32 rounding
 // of the angle), which keeps it within 1e-6 of the exact midpoint.  A cap
 // that contains it with margin 1e-4 therefore also passes the exact test
-// dot(mid, axis) > cos(a) + 1e-12: such a slot is buried with certainty.
-// Slots it does not settle take the exact route.
+// dot(mid, axis) > cos(a) + 1e-12 (the slot is buried with certainty), and
+// one it lies outside of by 1e-4 fails it.  Slots that neither settles take
+// the exact route.
 __device__ __forceinline__ v3 approx_midpoint(v3 a, double4 n4, double ccw) {
     const v3 n = mk(n4.x, n4.y, n4.z);
     const double h = dot(a, n);
@@ -156,10 +160,6 @@ __device__ __forceinline__ v3 approx_midpoint(v3 a, double4 n4, double ccw) {
     float sd, cd;
     sincosf((float)(0.5 * ccw), &sd, &cd);
     return n * h + r * (double)cd + t * (double)sd;
-}
-
-__device__ __forceinline__ bool contains_margin(double4 c, v3 m) {
-    return c.x * m.x + c.y * m.y + c.z * m.z > c.w + 1e-4;
 }
 
 template <int CTRL, int ROWS, int BANKS>
@@ -484,14 +484,18 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             }
         }
         // skip[k] low bits: 0 keep, 1 test every point below, 2 proven (see
-        // below); bit 2: a circle meeting circle k is nearly parallel to it
+        // below); bit 2: a circle meeting circle k is nearly parallel to it;
+        // bit 3: proven with depth SA_COV_DELTA (a deep circle, see the
+        // transition counts).  ct cos a_k - st sin a_k - cos a_m is the least
+        // containment value of cap m on circle k.
         int sk8 = near_par ? 4 : 0;
         if (hint != 0xFF) {
             const Cap ch = cap_at(L, hint);
             const double sk = sqrt(sk2);
             const double ct = dot(ck.axis, ch.axis);
             const double st = norm(cross(ck.axis, ch.axis));
-            sk8 |= (!near_par && sk >= 0.05 && ct * ck.ca - st * sk - ch.ca > 1e-6) ? 2 : 1;
+            const double least = ct * ck.ca - st * sk - ch.ca;
+            sk8 |= (!near_par && sk >= 0.05 && least > 1e-6) ? (least > SA_COV_DELTA ? 10 : 2) : 1;
         }
         L.hint[k] = (unsigned char)hint;
         L.skip[k] = (unsigned char)sk8;
@@ -575,7 +579,9 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     // widest arc proves the circle covered, every angle then lying inside
     // some exact arc.  As for the skip test, the points origami forms lie
     // within 1e-8 of circle k (no nearly parallel partner, sin a_k >= 0.05),
-    // which moves f by less than 2e-8 << SA_COV_DELTA = 1e-6.
+    // which moves f by less than 2e-8 << SA_COV_DELTA = 1e-3.  A covered
+    // circle is also deep (bit 3 of skip): every point of it lies inside
+    // another cap with containment value >= SA_COV_DELTA.
     int n_cand = 0;
     for (int base = 0; base < nc; base += 64) {
         const int k = base + lane;
@@ -664,7 +670,7 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                         }
                     }
                 }
-                if (lr == 0 && cov) L.skip[L.hl[c0 + ci]] = 2;
+                if (lr == 0 && cov) L.skip[L.hl[c0 + ci]] = 10;
             }
             wave_sync();
         }
@@ -680,7 +686,21 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         n_act += popc(am);
     }
     wave_sync();
-    // transition counts of the active caps, lanes over (active cap, other cap)
+    // Transition counts of the active caps, lanes over (active cap, other
+    // cap).  The transitions of deep circles are left out.  A deep circle m
+    // lies, at every point, inside another cap with containment value
+    // >= SA_COV_DELTA (1e-3), which changes by at most the distance moved,
+    // so the accessible region keeps that distance from circle m.  Leaving
+    // m's points out of circle k's list merges segments that all lie inside
+    // such a cap, which are buried either way; the exposed segments, their
+    // ends and the caps there stay the same.  Deduplication (a point within
+    // 1e-6 of the last kept one is dropped, so a run of dropped points spans
+    // at most 2 (NC - 1) 1e-6 < 2.6e-4) cannot carry a left-out point's
+    // effect to an exposed end.  A circle whose transitions are all left out
+    // lies inside the accessible region's complement and takes the probe
+    // test like a circle without transitions.  The arcs are origami's unless
+    // the accessible boundary touches circle k within rounding of a tested
+    // midpoint.
     {
         unsigned* c32 = reinterpret_cast<unsigned*>(L.ta);      // [n_act], free until the first chunk
         for (int i = lane; i < n_act; i += 64) c32[i] = 0;
@@ -688,7 +708,8 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
         for (int p = lane; p < n_act * nper && nc > 1; p += 64) {
             const int i = pair_q(pdiv, p), mi = p - i * nper;
             const int k = L.act[i];
-            const int n = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1));
+            const int m = mi < k ? mi : mi + 1;
+            const int n = (L.skip[m] & 8) ? 0 : intersect_count_fast(cap_at(L, k), cap_at(L, m));
             if (n) atomicAdd(c32 + i, (unsigned)n);
         }
         wave_sync();
@@ -735,8 +756,9 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             if (p < npairs) {
                 const int kl = pair_q(pdiv, p), mi = p - kl * nper;
                 const int k = L.act[a0 + kl];
+                const int m = mi < k ? mi : mi + 1;
                 if (L.cnt[k] == 0) emit = mi == 0;
-                else if (nc > 1) emit = intersect_count_fast(cap_at(L, k), cap_at(L, mi < k ? mi : mi + 1)) != 0;
+                else if (nc > 1) emit = !(L.skip[m] & 8) && intersect_count_fast(cap_at(L, k), cap_at(L, m)) != 0;
                 code = (kl << 7) | mi;
             }
             const unsigned long long em = __ballot(emit);
@@ -891,13 +913,13 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
 
         // One slot per kept sorted position: the segment to the next kept
         // position (cyclic), or the probe test of a probe entry.  The burial
-        // test is an any() over the other caps, so it is run in two rounds:
-        // (A) every slot tries the likeliest covers (the hint cap, the two
-        // caps bounding the segment, and the caps of the four transitions
-        // before it, whose entries may still be open) on an approximate
-        // midpoint, with a margin that implies the exact test;
-        // (B) the slots none of them covers form the exact midpoint and scan
-        // all caps, lane-parallel over a compacted list.
+        // test is an any() over the other caps, run in two rounds:
+        // (A) every slot tests an approximate midpoint against every other
+        // cap: contained with margin 1e-4 by one of them, or outside all of
+        // them by 1e-4, implies the exact test's answer (a probe entry takes
+        // the exact test here, on its exact point);
+        // (B) the slots within the margin of some cap form the exact
+        // midpoint and scan all caps, lane-parallel over a compacted list.
         // est[y]: 0 no arc, 1 undecided, 2 arc.
         for (int y = lane; y < T; y += 64) {
             unsigned char st8 = 0;
@@ -905,10 +927,11 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                 const int x = L.perm[y];
                 const int kl = L.tk[x];
                 const int kk = L.act[a0 + kl];
-                const int hint = L.hint[kk];
                 const v3 apt = mk(L.tx[x], L.ty[x], L.tz[x]);
                 if (L.tq[x] == 2) {
-                    st8 = (hint != 0xFF && contains_strict(cap_at(L, hint), apt)) ? 0 : 1;
+                    bool in = false;
+                    for (int c = 0; c < nc; ++c) in = in | (c != kk && contains_strict(cap_at(L, c), apt));
+                    st8 = in ? 0 : 2;
                 } else {
                     const int s0 = L.seg[kl], s1 = L.seg[kl + 1];
                     int yn = y + 1 < s1 ? y + 1 : s0;
@@ -916,23 +939,16 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                     const int bx = L.perm[yn];
                     const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
                     if (!(ccw < 1e-9)) {
-                        // all seven candidates at once (independent LDS chains)
                         const v3 mid = approx_midpoint(apt, L.cap4[kk], ccw);
-                        const int len = s1 - s0;
-                        int cand[7];
-                        cand[0] = L.tm[x];
-                        cand[1] = L.tm[bx];
-                        cand[2] = hint != 0xFF ? hint : cand[0];
-#pragma unroll
-                        for (int t = 1; t <= 4; ++t) {
-                            int yp = y - t;
-                            while (yp < s0) yp += len;
-                            cand[2 + t] = L.tm[L.perm[yp]];
+                        bool in = false, near = false;
+                        for (int c = 0; c < nc; ++c) {
+                            const double4 cc = L.cap4[c];
+                            const double v = fma(cc.z, mid.z, fma(cc.y, mid.y, cc.x * mid.x)) - cc.w;
+                            const bool o = c != kk;
+                            in = in | (o && v > 1e-4);
+                            near = near | (o && !(v <= -1e-4));
                         }
-                        bool bad = false;
-#pragma unroll
-                        for (int c = 0; c < 7; ++c) bad = bad | contains_margin(L.cap4[cand[c]], mid);
-                        st8 = bad ? 0 : 1;
+                        st8 = in ? 0 : (near ? 1 : 2);
                     }
                 }
             }
@@ -1157,33 +1173,49 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     int* wi = d.iscratch + (size_t)slot * WI_TOTAL;
     const unsigned long long lt = lanemask_lt(lane);
     static_assert(!SECOND || NC == MAXC, "the second pass holds MAXC caps");
-    const bool second = SECOND;
-    const int n_items = second ? __builtin_amdgcn_readfirstlane(d.counters[3]) : d.n_work;
-    // The first three quarters of the atoms are split statically; the rest
-    // are handed out one at a time by a queue counter, which evens out the
-    // several-fold spread in cost per atom.  Every wave leaves the loop once
-    // the counter has passed the last atom.  (The second pass, with few
-    // atoms, is split statically.)
-    // One call site for both parts keeps a single copy of the arrangement
-    // code (about half the kernel's size otherwise).
-    const int n_static = second ? n_items : (n_items / 4) * 3;
-    int it = slot;
-    for (;;) {
-        int w;
-        if (it < n_static) {
-            w = second ? d.deferred[it] : it;
-            it += n_slots;
-        } else {
-            if (second) break;
-            int q = 0;
-            if (lane == 0) q = atomicAdd(d.counters + 4, 1);
-            w = n_static + __shfl(q, 0, 64);
-            if (w >= n_items) break;
+    if (SECOND) {
+        // the atoms the main pass deferred (usually few), split statically
+        const int n_def = __builtin_amdgcn_readfirstlane(d.counters[3]);
+        for (int it = slot; it < n_def; it += n_slots) {
+            const int w = __builtin_amdgcn_readfirstlane(d.deferred[it]);
+            const int4 rec = arrange_atom<AREA, NC>(L, d, pos, w, lane, lt, wd, wi);
+            if (lane == 0) d.rec[w] = rec;
+            wave_sync();
         }
-        w = __builtin_amdgcn_readfirstlane(w);
+        return;
+    }
+    // Work queues, one per XCD (blocks go to the XCDs in turn, so block b
+    // runs on XCD b mod 8): queue g hands out the g-th eighth of the atom
+    // list one atom at a time, which evens out the several-fold spread in
+    // cost per atom and keeps each XCD's L2 on a contiguous part of the
+    // batch (neighbouring atoms share their neighbours).  A wave whose
+    // queue is spent takes atoms from the next ones in turn and leaves once
+    // it has found all eight spent.  The counters sit in 256-byte lines of
+    // their own.  One call site keeps a single copy of the arrangement code.
+    const int n_items = d.n_work;
+    const int g0 = blockIdx.x & (SA_QUEUES - 1);
+    int g = g0, tried = 1;
+    auto fetch = [&](int grp) {
+        int q = 0;
+        if (lane == 0) q = atomicAdd(d.counters + SA_QSTRIDE * (grp + 1), 1);
+        return __builtin_amdgcn_readfirstlane(__shfl(q, 0, 64));
+    };
+    int q = fetch(g);
+    for (;;) {
+        const int b = (int)((long long)g * n_items / SA_QUEUES);
+        const int e = (int)((long long)(g + 1) * n_items / SA_QUEUES);
+        if (b + q >= e) {
+            if (tried == SA_QUEUES) break;
+            ++tried;
+            g = (g + 1) & (SA_QUEUES - 1);
+            q = fetch(g);
+            continue;
+        }
+        const int w = b + q;
         const int4 rec = arrange_atom<AREA, NC>(L, d, pos, w, lane, lt, wd, wi);
         if (lane == 0) d.rec[w] = rec;
         wave_sync();
+        q = fetch(g);
     }
 }
 
@@ -1378,13 +1410,15 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
 }
 
 // Per-tile bounding spheres (center = first atom, radius = max distance).
-// counters != null: block 0 also clears the SASA launch's 8 counters (one
-// launch fewer at the head of the SASA chain).
+// counters != null: block 0 also clears the SASA launch's counters (the
+// pool fill and deferred count, and the work queues; one launch fewer at
+// the head of the SASA chain).
 __global__ void og_k_tile_bounds(const double* pos, const int* tile_atom0, const int* tile_cnt,
                                  int n_tiles, double4* out, int* counters) {
     const int t = blockIdx.x;
     const int lane = threadIdx.x;
     if (counters && t == 0 && lane < 8) counters[lane] = 0;
+    if (counters && t == 0 && lane < SA_QUEUES) counters[SA_QSTRIDE * (lane + 1)] = 0;
     if (t >= n_tiles) return;
     const int a0 = tile_atom0[t];
     const double cx = pos[3 * (size_t)a0], cy = pos[3 * (size_t)a0 + 1], cz = pos[3 * (size_t)a0 + 2];
@@ -1484,7 +1518,7 @@ int og_sasa_build(og_sasa_data* d, int n_atoms, const std::vector<int>& atom_sys
         if (!up(&d->probes, pr) || !up(&d->padj_start, st) || !up(&d->padj, adj))
             return fail("device allocation failed");
     }
-    if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, 8) ||
+    if (!zalloc(&d->tile_bound, (size_t)n_tiles) || !zalloc(&d->overflow, 1) || !zalloc(&d->counters, SA_NCOUNTERS) ||
         !zalloc(&d->rec, (size_t)d->n_work) || !zalloc(&d->deferred, (size_t)d->n_work))
         return fail("device allocation failed");
     if (d->n_work > 0) {

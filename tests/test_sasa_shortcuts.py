@@ -153,7 +153,8 @@ def _cover_arcs(host, pairs):
 
 def test_cover_arcs_inside_exact_arcs(host):
     # every angle the cover arc claims (SA_COV_EPS inside its ends) has
-    # containment value >= 1e-6 on the exact circle
+    # containment value >= SA_COV_DELTA = 1e-3 on the exact circle (the
+    # depth that lets the kernel leave a covered circle's transitions out)
     rng = np.random.default_rng(4)
     n = 200000
     n1, n2 = _unit(rng, n), _unit(rng, n)
@@ -177,7 +178,7 @@ def test_cover_arcs_inside_exact_arcs(host):
             X = (n1[idx] * c1[idx, None] + s1[idx, None]
                  * (np.cos(phi)[:, None] * bas[idx, 0:3] + np.sin(phi)[:, None] * bas[idx, 3:6]))
             val = np.sum(X * n2[idx], axis=1) - c2[idx]
-            assert val.min() > 1e-6 - 1e-9, (f, val.min())
+            assert val.min() > 1e-3 - 1e-9, (f, val.min())
 
 
 def _cover_walk(arcs):
@@ -247,7 +248,7 @@ def test_cover_turn_count_by_reciprocal():
 
 def test_cover_walk_on_chain_atoms(host):
     # atoms of a jittered chain: every circle the walk declares covered has
-    # each of 4001 sampled points inside another cap with the 1e-6 margin
+    # each of 4001 sampled points inside another cap with the 1e-3 margin
     import math
     from origami_amd import system as S
     s = S.build_system(["Lys", "Trp", "Ala", "Leu", "Phe", "Glu"])
@@ -281,7 +282,7 @@ def test_cover_walk_on_chain_atoms(host):
             phi = np.linspace(0, 2 * np.pi, 4001)
             X = caps[k, :3] * caps[k, 3] + sk * (np.cos(phi)[:, None] * u + np.sin(phi)[:, None] * v)
             val = (X @ caps[others, :3].T - caps[others, 3]).max(axis=1)
-            assert val.min() > 1e-6 - 1e-9
+            assert val.min() > 1e-3 - 1e-9
             checked += 1
     assert checked > 200
 
