@@ -43,6 +43,7 @@
 // bit-reproducible.
 #include <cstddef>
 #include <string>
+#include <type_traits>
 
 #include "og_device.h"
 #include "og_kernels.h"
@@ -208,8 +209,9 @@ __device__ __forceinline__ unsigned row_lane15(unsigned y, int row) {
 // from a multiply by 1/(2 pi): where it differs from the rounded quotient the
 // reduced angle lies within 4e-6 of 0 or 2 pi, which both forms reject (arc
 // widths stay below 2 pi - 0.08), so the result equals the divided form's.
+// An empty slot (width -1) fails d < e.y - SA_COV_EPS for every reduced
+// angle (d > -4e-6), so no branch is needed for it.
 __device__ __forceinline__ float cover_ext(float p, float2 e) {
-    if (!(e.y > 0.0f)) return 0.0f;
     const float two_pi = 6.28318530717958647692f;
     float d = p - e.x;
     d -= two_pi * floorf(d * (1.0f / two_pi));
@@ -455,32 +457,44 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
 
     // ---------------- burial hints ----------------
     // hint[k]: the first cap m that (up to rounding) contains circle k
-    // entirely.  It is tested first in the burial tests below (that changes
-    // only the order in which an any() is evaluated), and it drives the exact
-    // skip test that follows.  Two caps m per step, so that the two
-    // dependent FP64 chains overlap.  Transition counts are formed later,
-    // for the caps that stay active.
+    // entirely.  It drives the exact skip test that follows (any cap that
+    // contains the circle serves there; the test itself is exact).  The scan
+    // has no branches: one cap per step, FMA arithmetic.  It also lists the
+    // caps whose axis is within about 5.7 degrees of axis k (1 - cos^2 <
+    // 1.0001e-2, a superset of the exact test's < 1e-2); those are then
+    // checked with origami's arithmetic, and a nearly parallel circle that
+    // meets circle k sets near_par exactly as a full exact scan would.
+    // Transition counts are formed later, for the caps that stay active.
     for (int k = lane; k < nc; k += 64) {
         const Cap ck = cap_at(L, k);
         const double sk2 = fmax(1.0 - ck.ca * ck.ca, 0.0);
-        int hint = 0xFF;
+        int hint = 0xFF, np_m = 0, np_n = 0;
+#pragma unroll 2
+        for (int m = 0; m < nc; ++m) {
+            const double4 c = L.cap4[m];
+            const double sm = fma(ck.axis.z, c.z, fma(ck.axis.y, c.y, ck.axis.x * c.x));
+            const double dm = fma(-sm, sm, 1.0);
+            const double lm = fma(sm, ck.ca, -c.w);
+            const bool o = m != k;
+            // circle k inside cap m: cos(axes) cos(a_k) - sin(axes) sin(a_k) > cos(a_m)
+            const bool in = o & (lm > 0.0) & (lm * lm > dm * sk2);
+            hint = ((hint == 0xFF) & in) ? m : hint;
+            const bool np = o & (dm < 1.0001e-2);
+            np_m = (np & (np_n == 0)) ? m : np_m;
+            np_n += np ? 1 : 0;
+        }
+        // a nearly parallel circle matters only where it meets circle k
         bool near_par = false;
-        for (int m0 = 0; m0 < nc; m0 += 2) {
-            const int m1 = m0 + 1 < nc ? m0 + 1 : m0;
-            const Cap c0 = cap_at(L, m0), c1 = cap_at(L, m1);
-            const bool v0 = m0 != k, v1 = m0 + 1 < nc && m1 != k;
-            const double s0 = dot(ck.axis, c0.axis), s1 = dot(ck.axis, c1.axis);
-            const double d0 = 1.0 - s0 * s0, d1 = 1.0 - s1 * s1;
-            // a nearly parallel circle matters only where it meets circle k
-            if ((v0 && d0 < 1e-2) || (v1 && d1 < 1e-2))
-                near_par = near_par || (v0 && d0 < 1e-2 && intersect_count_fast(ck, c0) > 0) ||
-                           (v1 && d1 < 1e-2 && intersect_count_fast(ck, c1) > 0);
-            if (hint == 0xFF) {
-                // circle k inside cap m (up to rounding):
-                // cos(axes) cos(a_k) - sin(axes) sin(a_k) > cos(a_m)
-                const double l0 = s0 * ck.ca - c0.ca, l1 = s1 * ck.ca - c1.ca;
-                if (v0 && l0 > 0.0 && l0 * l0 > d0 * sk2) hint = m0;
-                else if (v1 && l1 > 0.0 && l1 * l1 > d1 * sk2) hint = m1;
+        if (np_n == 1) {
+            const Cap cm = cap_at(L, np_m);
+            const double s0 = dot(ck.axis, cm.axis);
+            near_par = 1.0 - s0 * s0 < 1e-2 && intersect_count_fast(ck, cm) > 0;
+        } else if (np_n > 1) {
+            for (int m = 0; m < nc; ++m) {
+                if (m == k) continue;
+                const Cap cm = cap_at(L, m);
+                const double s0 = dot(ck.axis, cm.axis);
+                if (1.0 - s0 * s0 < 1e-2 && intersect_count_fast(ck, cm) > 0) near_par = true;
             }
         }
         // skip[k] low bits: 0 keep, 1 test every point below, 2 proven (see
@@ -623,55 +637,61 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             // circle takes exactly its own steps, from the end of its widest
             // arc (the first of equal widths), each step to the furthest
             // reach of the arcs that contain the current angle with
-            // SA_COV_EPS to spare; rows that are done stay as they are.
-            const int nslot = (nper + 15) >> 4;                    // <= 8 (nper <= MAXC - 1)
+            // SA_COV_EPS to spare; rows that are done stay as they are.  The
+            // slots per lane (NS) are a compile-time count, the smallest of
+            // 1, 2, 3, 4 and 8 that holds the circle's nper arcs (the extra
+            // slots are empty), so the step loop has no branches per slot.
             const int row = lane >> 4, lr = lane & 15;
-            for (int g0 = 0; g0 < nb; g0 += 4) {
-                const int ci = g0 + row;
-                const bool rv = ci < nb;
-                float2 e[8];
-                bool full = false;
-                float wl = 0.0f;
+            auto walk = [&](auto ns) {
+                constexpr int NS = decltype(ns)::value;
+                for (int g0 = 0; g0 < nb; g0 += 4) {
+                    const int ci = g0 + row;
+                    const bool rv = ci < nb;
+                    float2 e[NS];
+                    bool full = false;
+                    float wl = 0.0f;
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int mi = lr + 16 * q;
-                    e[q] = (q < nslot && rv && mi < nper) ? iv[ci * nper + mi] : make_float2(0.0f, -1.0f);
-                    full = full || e[q].y >= SA_COV_FULL;
-                    wl = fmaxf(wl, e[q].y);
-                }
-                const bool rfull = ((__ballot(full) >> (16 * row)) & 0xFFFFull) != 0;
-                const float wm = __uint_as_float(row_lane15(row_umax(__float_as_uint(wl)), row));
-                unsigned first = 0xFFFFu;
-#pragma unroll
-                for (int q = 7; q >= 0; --q)
-                    if (e[q].y == wm) first = (unsigned)(lr + 16 * q);
-                first = row_lane15(row_umin(first), row);
-                bool act = rv && !rfull && wm > 0.0f;
-                bool cov = rv && rfull;
-                float start = 0.0f, len = 0.0f, t = 0.0f;
-                if (act) {
-                    start = iv[ci * nper + (int)first].x + wm;
-                    len = 6.28318530717958647692f - wm;
-                }
-                for (int step = 0; step <= nper + 1; ++step) {
-                    if (!__ballot(act)) break;
-                    const float p = start + t;
-                    float ext = 0.0f;
-#pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        if (q < nslot) ext = fmaxf(ext, cover_ext(p, e[q]));
-                    const float best = __uint_as_float(row_lane15(row_umax(__float_as_uint(act ? ext : 0.0f)), row));
-                    if (act) {
-                        if (!(best > 0.0f)) {
-                            act = false;
-                        } else {
-                            t += best;
-                            if (t >= len + SA_COV_EPS) { cov = true; act = false; }
-                        }
+                    for (int q = 0; q < NS; ++q) {
+                        const int mi = lr + 16 * q;
+                        e[q] = (rv && mi < nper) ? iv[ci * nper + mi] : make_float2(0.0f, -1.0f);
+                        full = full || e[q].y >= SA_COV_FULL;
+                        wl = fmaxf(wl, e[q].y);
                     }
+                    const bool rfull = ((__ballot(full) >> (16 * row)) & 0xFFFFull) != 0;
+                    const float wm = __uint_as_float(row_lane15(row_umax(__float_as_uint(wl)), row));
+                    unsigned first = 0xFFFFu;
+#pragma unroll
+                    for (int q = NS - 1; q >= 0; --q)
+                        first = e[q].y == wm ? (unsigned)(lr + 16 * q) : first;
+                    first = row_lane15(row_umin(first), row);
+                    bool act = rv && !rfull && wm > 0.0f;
+                    bool cov = rv && rfull;
+                    float start = 0.0f, len = 0.0f, t = 0.0f;
+                    if (act) {
+                        start = iv[ci * nper + (int)first].x + wm;
+                        len = 6.28318530717958647692f - wm;
+                    }
+                    for (int step = 0; step <= nper + 1; ++step) {
+                        if (!__ballot(act)) break;
+                        const float p = start + t;
+                        float ext = 0.0f;
+#pragma unroll
+                        for (int q = 0; q < NS; ++q) ext = fmaxf(ext, cover_ext(p, e[q]));
+                        const float best = __uint_as_float(row_lane15(row_umax(__float_as_uint(act ? ext : 0.0f)), row));
+                        const bool adv = act && best > 0.0f;
+                        t = adv ? t + best : t;
+                        const bool done = adv && t >= len + SA_COV_EPS;
+                        cov = cov || done;
+                        act = adv && !done;
+                    }
+                    if (lr == 0 && cov) L.skip[L.hl[c0 + ci]] = 10;
                 }
-                if (lr == 0 && cov) L.skip[L.hl[c0 + ci]] = 10;
-            }
+            };
+            if (nper <= 16) walk(std::integral_constant<int, 1>{});
+            else if (nper <= 32) walk(std::integral_constant<int, 2>{});
+            else if (nper <= 48) walk(std::integral_constant<int, 3>{});
+            else if (nper <= 64) walk(std::integral_constant<int, 4>{});
+            else walk(std::integral_constant<int, 8>{});
             wave_sync();
         }
     }
