@@ -86,7 +86,7 @@ struct ArrLds {
     unsigned char hint[NC];                          // a cap that likely buries circle k
     unsigned char skip[NC];                          // circle k provably contributes nothing
     unsigned char act[NC];                           // caps still to process, ascending
-    unsigned char hl[NC];                            // caps with a hint, ascending
+    unsigned char hl[NC];                            // ascending cap lists: hinted, cover candidates, not deep
     unsigned char cnt[NC];                           // transitions of cap k (<= 2 (NC - 1))
     unsigned short seg[NC + 1];                      // chunk: segment start per chunk cap
     alignas(16) double ta[SA_TMAX];                  // chunk: transition angles (push order)
@@ -720,16 +720,26 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     // lies inside the accessible region's complement and takes the probe
     // test like a circle without transitions.  The arcs are origami's unless
     // the accessible boundary touches circle k within rounding of a tested
-    // midpoint.
+    // midpoint.  The caps that are not deep are listed once (ascending,
+    // in hl, free after the cover test), and the pair loops here and in the
+    // chunks below run over (active cap, listed cap).
+    int n_nd = 0;
+    for (int base = 0; base < nc; base += 64) {
+        const int k = base + lane;
+        const bool nd = k < nc && !(L.skip[k] & 8);
+        const unsigned long long ndm = __ballot(nd);
+        if (nd) L.hl[n_nd + popc(ndm & lt)] = (unsigned char)k;
+        n_nd += popc(ndm);
+    }
+    const PairDiv ndiv = pair_div(n_nd);                  // n_nd >= n_act >= 1
     {
         unsigned* c32 = reinterpret_cast<unsigned*>(L.ta);      // [n_act], free until the first chunk
         for (int i = lane; i < n_act; i += 64) c32[i] = 0;
         wave_sync();
-        for (int p = lane; p < n_act * nper && nc > 1; p += 64) {
-            const int i = pair_q(pdiv, p), mi = p - i * nper;
-            const int k = L.act[i];
-            const int m = mi < k ? mi : mi + 1;
-            const int n = (L.skip[m] & 8) ? 0 : intersect_count_fast(cap_at(L, k), cap_at(L, m));
+        for (int p = lane; p < n_act * n_nd; p += 64) {
+            const int i = pair_q(ndiv, p), j = p - i * n_nd;
+            const int k = L.act[i], m = L.hl[j];
+            const int n = m == k ? 0 : intersect_count_fast(cap_at(L, k), cap_at(L, m));
             if (n) atomicAdd(c32 + i, (unsigned)n);
         }
         wave_sync();
@@ -767,19 +777,19 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
             L.seg[nck] = (unsigned short)acc;
         }
         unsigned short* plist = reinterpret_cast<unsigned short*>(L.rm);   // [SA_TMAX], over rm and est
-        const int npairs = nck * nper;
+        const int npairs = nck * n_nd;
         int npl = 0;
         for (int base = 0; base < npairs; base += 64) {
             const int p = base + lane;
             bool emit = false;
             int code = 0;
             if (p < npairs) {
-                const int kl = pair_q(pdiv, p), mi = p - kl * nper;
+                const int kl = pair_q(ndiv, p), j = p - kl * n_nd;
                 const int k = L.act[a0 + kl];
-                const int m = mi < k ? mi : mi + 1;
-                if (L.cnt[k] == 0) emit = mi == 0;
-                else if (nc > 1) emit = !(L.skip[m] & 8) && intersect_count_fast(cap_at(L, k), cap_at(L, m)) != 0;
-                code = (kl << 7) | mi;
+                const int m = L.hl[j];
+                if (L.cnt[k] == 0) emit = j == 0;
+                else emit = m != k && intersect_count_fast(cap_at(L, k), cap_at(L, m)) != 0;
+                code = (kl << 7) | m;
             }
             const unsigned long long em = __ballot(emit);
             const int pos = npl + popc(em & lt);
@@ -797,11 +807,10 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                 const int code = plist[q];
                 kl = code >> 7;
                 k = L.act[a0 + kl];
-                const int mi = code & 127;
                 if (L.cnt[k] == 0) {
                     probe = 1; cnt = 1; P = sample_circle_point(cap_at(L, k));
                 } else {
-                    m = mi < k ? mi : mi + 1;
+                    m = code & 127;
                     const int r = intersect_circles(cap_at(L, k), cap_at(L, m), P, Q);
                     cnt = r == 0 ? 2 : (r == 1 ? 1 : 0);
                 }
@@ -959,16 +968,15 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
                     const int bx = L.perm[yn];
                     const double ccw = wrap_2pi_positive(L.ta[bx] - L.ta[x]);
                     if (!(ccw < 1e-9)) {
+                        // the largest containment value over the other caps
                         const v3 mid = approx_midpoint(apt, L.cap4[kk], ccw);
-                        bool in = false, near = false;
+                        double vmax = -1.0e300;
                         for (int c = 0; c < nc; ++c) {
                             const double4 cc = L.cap4[c];
                             const double v = fma(cc.z, mid.z, fma(cc.y, mid.y, cc.x * mid.x)) - cc.w;
-                            const bool o = c != kk;
-                            in = in | (o && v > 1e-4);
-                            near = near | (o && !(v <= -1e-4));
+                            vmax = fmax(vmax, c != kk ? v : -1.0e300);
                         }
-                        st8 = in ? 0 : (near ? 1 : 2);
+                        st8 = vmax > 1e-4 ? 0 : (vmax > -1e-4 ? 1 : 2);
                     }
                 }
             }
