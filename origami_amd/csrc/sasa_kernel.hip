@@ -103,6 +103,9 @@ static_assert(offsetof(ArrLds<SA_NC_MAIN>, est) == offsetof(ArrLds<SA_NC_MAIN>, 
               offsetof(ArrLds<MAXC>, est) == offsetof(ArrLds<MAXC>, rm) + SA_TMAX &&
               offsetof(ArrLds<SA_NC_MAIN>, rm) % 2 == 0 && offsetof(ArrLds<MAXC>, rm) % 2 == 0,
               "rm, est must be contiguous and 2-aligned");
+// the caps phase buffers atom indices in perm (64 ints)
+static_assert(offsetof(ArrLds<SA_NC_MAIN>, perm) % 4 == 0 && offsetof(ArrLds<MAXC>, perm) % 4 == 0,
+              "perm must be 4-aligned");
 // the union cover test stores its arcs across tx, ty and tz
 static_assert(offsetof(ArrLds<SA_NC_MAIN>, tz) == offsetof(ArrLds<SA_NC_MAIN>, tx) + 2 * SA_TMAX * sizeof(double),
               "tx, ty, tz must be contiguous");
@@ -372,75 +375,117 @@ __device__ int4 arrange_atom(ArrLds<NC>& L, const og_sasa_data& d, const double*
     // ---------------- caps (build_atom_cache + build_caps) ----------------
     // Candidates in ascending atom order; an enclosing neighbour ends the
     // scan (no accessible surface) unless the cap count overflowed first.
-    // Tiles are culled 64 at a time (lane per tile), then the near ones are
-    // scanned in ascending order.
-    int nc = 0;
+    // A neighbour can only be a cap or enclose
+    // atom i when |p_j - p_i| <= r_i + r_j (rounded); the scan keeps the
+    // atoms whose squared distance is within (r_i + r_j)^2 (1 + 1e-12),
+    // which covers the rounding of the square root, and buffers them in LDS
+    // (ascending); origami's exact tests then run on full waves of buffered
+    // atoms, so the square root and the divisions are formed only for them.
+    int nc = 0, nb = 0;
     bool enclosed = false, overflow = false;
-    const int t_end = d.tile_end[w];
-    for (int tb0 = d.tile_begin[w]; tb0 < t_end && !enclosed && !overflow; tb0 += 64) {
-        bool near = false;
-        if (tb0 + lane < t_end) {
-            const double4 tb = d.tile_bound[tb0 + lane];
-            const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
-            const double reach = ri + d.max_radius + tb.w;
-            near = !(dxc * dxc + dyc * dyc + dzc * dzc > reach * reach);
-        }
-        // near tiles in ascending order, the next tile's atoms loaded while
-        // the current one is processed
-        unsigned long long near_m = __ballot(near);
-        int t = -1, j = 0;
-        bool valid = false;
-        v3 pj = mk(0, 0, 0);
-        double rj = 0.0;
-        if (near_m) {
-            t = tb0 + __builtin_ctzll(near_m);
-            near_m &= near_m - 1;
-            j = d.tile_atom0[t] + lane;
-            valid = lane < d.tile_cnt[t] && j != ia;
-            if (valid) { pj = ldp(pos, j); rj = d.radius[j]; }
-        }
-        while (t >= 0) {
-            int tn = -1, jn = 0;
-            bool validn = false;
-            v3 pjn = mk(0, 0, 0);
-            double rjn = 0.0;
-            if (near_m) {
-                tn = tb0 + __builtin_ctzll(near_m);
-                near_m &= near_m - 1;
-                jn = d.tile_atom0[tn] + lane;
-                validn = lane < d.tile_cnt[tn] && jn != ia;
-                if (validn) { pjn = ldp(pos, jn); rjn = d.radius[jn]; }
-            }
-            bool is_cap = false, enc = false;
-            v3 axis = mk(0, 0, 0);
-            double ca = 0.0;
-            if (valid) {
-                const double dd = norm(pj - pi);
-                if (dd <= ri + rj) {
-                    if (dd + ri <= rj) {
-                        enc = true;
-                    } else if (!(dd + rj <= ri || dd >= ri + rj)) {
-                        ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
-                        if (ca >= -1.0 && ca <= 1.0) {
-                            is_cap = true;
-                            axis = (pj - pi) / dd;
-                        }
+    int* cj = reinterpret_cast<int*>(L.perm);              // [64] buffered atoms (chunk arrays are free here)
+    double* crj = L.ta;                                    // [64] their radii, positions in tx, ty, tz
+    auto flush = [&]() {
+        bool is_cap = false, enc = false;
+        v3 axis = mk(0, 0, 0);
+        double ca = 0.0;
+        int j = 0;
+        if (lane < nb) {
+            j = cj[lane];
+            const v3 pj = mk(L.tx[lane], L.ty[lane], L.tz[lane]);
+            const double rj = crj[lane];
+            const double dd = norm(pj - pi);
+            if (dd <= ri + rj) {
+                if (dd + ri <= rj) {
+                    enc = true;
+                } else if (!(dd + rj <= ri || dd >= ri + rj)) {
+                    ca = (dd * dd + ri * ri - rj * rj) / (2.0 * dd * ri);
+                    if (ca >= -1.0 && ca <= 1.0) {
+                        is_cap = true;
+                        axis = (pj - pi) / dd;
                     }
                 }
             }
-            unsigned long long m = __ballot(is_cap);
-            const unsigned long long em = __ballot(enc);
-            if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
-            if (nc + popc(m) > NC) { overflow = true; break; }
-            if ((m >> lane) & 1ull) {
-                const int p = nc + popc(m & lt);
-                L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
-                L.own[p] = j;
-            }
-            nc += popc(m);
-            if (em) { enclosed = true; break; }
-            t = tn; j = jn; valid = validn; pj = pjn; rj = rjn;
         }
+        unsigned long long m = __ballot(is_cap);
+        const unsigned long long em = __ballot(enc);
+        if (em) m &= (em & (~em + 1)) - 1;     // caps before the first enclosing atom
+        nb = 0;
+        if (nc + popc(m) > NC) { overflow = true; return; }
+        if ((m >> lane) & 1ull) {
+            const int p = nc + popc(m & lt);
+            L.cap4[p] = make_double4(axis.x, axis.y, axis.z, ca);
+            L.own[p] = j;
+        }
+        nc += popc(m);
+        if (em) enclosed = true;
+    };
+    // near tiles in ascending order (culled 64 at a time); the next tile's
+    // atoms load while the current one is scanned; one flush site (after
+    // the last tile, t < 0, it takes what is left)
+    const int t_end = d.tile_end[w];
+    int t_base = 0, t_next = d.tile_begin[w];
+    unsigned long long near_m = 0;
+    auto next_tile = [&]() -> int {
+        while (near_m == 0 && t_next < t_end) {
+            bool near = false;
+            if (t_next + lane < t_end) {
+                const double4 tb = d.tile_bound[t_next + lane];
+                const double dxc = tb.x - pi.x, dyc = tb.y - pi.y, dzc = tb.z - pi.z;
+                const double reach = ri + d.max_radius + tb.w;
+                near = !(dxc * dxc + dyc * dyc + dzc * dzc > reach * reach);
+            }
+            near_m = __ballot(near);
+            t_base = t_next;
+            t_next += 64;
+        }
+        if (near_m == 0) return -1;
+        const int t = t_base + __builtin_ctzll(near_m);
+        near_m &= near_m - 1;
+        return t;
+    };
+    int t = next_tile(), j = 0;
+    bool valid = false;
+    v3 pj = mk(0, 0, 0);
+    double rj = 0.0;
+    if (t >= 0) {
+        j = d.tile_atom0[t] + lane;
+        valid = lane < d.tile_cnt[t] && j != ia;
+        if (valid) { pj = ldp(pos, j); rj = d.radius[j]; }
+    }
+    for (;;) {
+        const int tn = t >= 0 ? next_tile() : -1;
+        int jn = 0;
+        bool validn = false;
+        v3 pjn = mk(0, 0, 0);
+        double rjn = 0.0;
+        if (tn >= 0) {
+            jn = d.tile_atom0[tn] + lane;
+            validn = lane < d.tile_cnt[tn] && jn != ia;
+            if (validn) { pjn = ldp(pos, jn); rjn = d.radius[jn]; }
+        }
+        bool keep = false;
+        if (t >= 0 && valid) {
+            const v3 dv = pj - pi;
+            const double rs = ri + rj;
+            keep = dot(dv, dv) <= rs * rs * (1.0 + 1e-12);
+        }
+        const unsigned long long km = __ballot(keep);
+        if (t < 0 ? nb > 0 : nb + popc(km) > 64) {
+            wave_sync();
+            flush();
+            wave_sync();
+            if (enclosed || overflow) break;
+        }
+        if (t < 0) break;
+        if (keep) {
+            const int q = nb + popc(km & lt);
+            cj[q] = j;
+            crj[q] = rj;
+            L.tx[q] = pj.x; L.ty[q] = pj.y; L.tz[q] = pj.z;
+        }
+        nb += popc(km);
+        t = tn; j = jn; valid = validn; pj = pjn; rj = rjn;
     }
     wave_sync();
     if (overflow) {
