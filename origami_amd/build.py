@@ -15,8 +15,13 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_lib")
 LIB = os.path.join(OUT_DIR, "liborigami_gpu.so")
 
-SOURCES = ["engine.cpp", "pair_kernels.hip", "md_kernels.hip", "sasa_kernel.hip", "surrogate.hip",
-           "lbfgs.hip"]
+SOURCES = ["engine.cpp", "pair_kernels.hip", "md_kernels.hip", "sasa_kernel.hip", "sasa_terms.hip",
+           "surrogate.hip", "lbfgs.hip"]
+
+# Per-file additions.  The SASA JVP terms take one-hot tangents whose zero
+# products the compiler may drop only under finite-math, no-signed-zero
+# assumptions; for finite values the terms are unchanged (see the file).
+EXTRA = {"sasa_terms.hip": ["-ffinite-math-only", "-fno-signed-zeros"]}
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -36,7 +41,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         obj = os.path.join(OUT_DIR, os.path.splitext(src)[0] + ".o")
-        cmd = [HIPCC, "-x", "hip", "-c", path, "-o", obj] + flags()
+        cmd = [HIPCC, "-x", "hip", "-c", path, "-o", obj] + flags() + EXTRA.get(src, [])
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

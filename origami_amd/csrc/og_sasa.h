@@ -79,6 +79,8 @@ int og_sasa_max_blocks(const og_sasa_data* d);
 void og_sasa_set_blocks(og_sasa_data* d, int blocks);
 // 0: one pass when the grid fits the LDS of MAXC caps, else two; 1 or 2: forced
 void og_sasa_set_passes(og_sasa_data* d, int passes);
+// The JVP-terms kernel of a force evaluation (sasa_terms.hip).
+void og_launch_sasa_terms(const og_sasa_data& d, const double* pos, hipStream_t s);
 // Reads and clears the overflow count.
 int og_sasa_overflow(og_sasa_data* d, int* count);
 const char* og_sasa_error();

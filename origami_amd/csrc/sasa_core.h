@@ -8,10 +8,11 @@
 #include <math.h>
 #include <stddef.h>
 
+// internal linkage: the header is included by more than one kernel file
 #ifdef __HIPCC__
-#define OG_HD __host__ __device__
+#define OG_HD static __host__ __device__
 #else
-#define OG_HD
+#define OG_HD static
 #endif
 
 // No FMA contraction here: the boundary topology is decided by tolerance

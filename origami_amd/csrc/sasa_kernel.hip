@@ -1292,99 +1292,6 @@ og_k_sasa_arrange(og_sasa_data d, const double* pos) {
     }
 }
 
-// The JVP terms of one arc (12: role i, k, l, m x axis) or one vertex (9:
-// role i, in, out x axis) per thread: the primal geometry once, then one
-// tangent per direction of the roles other than i (bit-identical to one full
-// JVP per term); role i's terms are minus their sum.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) og_k_sasa_terms(og_sasa_data d, const double* pos) {
-    const int n_a = min(d.counters[0], d.arc_cap);
-    const int n_v = min(d.counters[1], d.vtx_cap);
-    const int total = n_a + n_v;
-    const v3 zero = mk(0, 0, 0);
-    for (int it = blockIdx.x * blockDim.x + threadIdx.x; it < total; it += gridDim.x * blockDim.x) {
-        if (it < n_a) {
-            const int A = it;
-            const int4 at = d.arc_atoms[A];
-            const double4 pr = d.arc_par[A];
-            const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y);
-            const double ri = d.radius[at.x], rk = d.radius[at.y];
-            const CosAlphaP cp = cos_alpha_primal(pi, pk, ri, rk);
-            const bool has_v = at.z >= 0;
-            v3 pl = zero, pm = zero;
-            double rl = 0.0, rm = 0.0, cak = 0.0;
-            ArcThetaP tp{};
-            tp.ok = false;
-            if (has_v) {
-                pl = ldp(pos, at.z);
-                pm = ldp(pos, at.w);
-                rl = d.radius[at.z];
-                rm = d.radius[at.w];
-                const double dik = norm(pk - pi);
-                cak = (dik * dik + ri * ri - rk * rk) / (2.0 * dik * ri);
-                tp = arc_theta_primal(pi, pk, pl, pm, ri, rk, rl, rm, pr.y, pr.z);
-            }
-            // Roles k, l, m by their tangents; role i as minus their sum: the
-            // arc depends on the positions only through differences from
-            // atom i's (translation invariance).
-            double out[12];
-            double ti[3] = {0.0, 0.0, 0.0};
-            for (int role = 1; role < 4; ++role) {
-                const int xa = role == 1 ? at.y : role == 2 ? at.z : (at.w == at.z ? -1 : at.w);
-                for (int axis = 0; axis < 3; ++axis) {
-                    double term = 0.0;
-                    if (xa >= 0) {
-                        const v3 b = unit(axis);
-                        const v3 drk = xa == at.y ? b : zero;
-                        const double dcos = cos_alpha_tangent(cp, zero, drk);
-                        if (!has_v) {
-                            term = pr.x * dcos;
-                        } else {
-                            const v3 drl = xa == at.z ? b : zero;
-                            const v3 drm = xa == at.w ? b : zero;
-                            const double dth = tp.ok ? arc_theta_tangent(tp, zero, drk, drl, drm) : 0.0;
-                            term = pr.x * dcos + cak * dth;
-                        }
-                    }
-                    out[3 * role + axis] = term;
-                    ti[axis] -= term;
-                }
-            }
-            out[0] = ti[0];
-            out[1] = ti[1];
-            out[2] = ti[2];
-            double* dst = d.arc_terms + 12ll * A;      // one pass, in order
-#pragma unroll
-            for (int q = 0; q < 12; ++q) dst[q] = out[q];
-        } else {
-            const int V = it - n_a;
-            const int4 at = d.vtx_atoms[V];
-            const v3 pi = ldp(pos, at.x), pk = ldp(pos, at.y), pl = ldp(pos, at.z);
-            const VertexEpsP ep = vertex_eps_primal(pi, pk, pl, d.radius[at.x], d.radius[at.y], d.radius[at.z],
-                                                    d.vtx_sign[V]);
-            // roles k, l by their tangents, role i as minus their sum
-            double out[9];
-            double ti[3] = {0.0, 0.0, 0.0};
-            for (int role = 1; role < 3; ++role) {
-                const int xa = role == 1 ? at.y : at.z;
-                for (int axis = 0; axis < 3; ++axis) {
-                    const v3 b = unit(axis);
-                    const v3 drk = xa == at.y ? b : zero;
-                    const v3 drl = xa == at.z ? b : zero;
-                    const double term = ep.ok ? vertex_eps_tangent(ep, zero, drk, drl) : 0.0;
-                    out[3 * role + axis] = term;
-                    ti[axis] -= term;
-                }
-            }
-            out[0] = ti[0];
-            out[1] = ti[1];
-            out[2] = ti[2];
-            double* dst = d.vtx_terms + 9ll * V;
-#pragma unroll
-            for (int q = 0; q < 9; ++q) dst[q] = out[q];
-        }
-    }
-}
-
 __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data d, long long* facc,
                                                                    double gamma_scale) {
     const int lane = threadIdx.x & 63;
@@ -1751,7 +1658,7 @@ int og_sasa_launch(og_sasa_data* d, const double* pos, long long* facc, double g
         hipLaunchKernelGGL((og_k_sasa_arrange<false, MAXC, true>), dim3(blocks2), blk, 0, s, *d, pos);
     }
     if (gamma_scale != 0.0) {                 // forces (not only areas)
-        hipLaunchKernelGGL(og_k_sasa_terms, dim3(d->n_cu * 8), dim3(256), 0, s, *d, pos);
+        og_launch_sasa_terms(*d, pos, s);
         hipLaunchKernelGGL(og_k_sasa_scatter, dim3((d->n_work + SA_WAVES - 1) / SA_WAVES), dim3(64 * SA_WAVES),
                            0, s, *d, facc, gamma_scale);
     }
