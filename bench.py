@@ -232,9 +232,11 @@ def pmc_probe(E):
     torch.cuda.synchronize()
     systems, _ = build_config4()
     eng = E.Engine(systems)
-    # the autotuner's usual choice for this batch on 256 CUs: 4 arrangement
-    # blocks per CU, one pass holding 128 caps per atom
-    eng.set_launch(sasa_blocks=4 * 256, sasa_passes=0)
+    # the autotuner's usual choice for this batch on 256 CUs: 3 arrangement
+    # blocks per CU, one pass holding 128 caps per atom, force strips of 8
+    # j tiles
+    eng.set_launch(sasa_blocks=3 * 256, sasa_passes=0)
+    eng.strip_tiles(8)
     for _ in range(3):
         eng.total_force(include_sasa=True, sasa_gamma_scale=0.25)
     print(json.dumps({"pmc_probe": "copy 536870912 bytes read + written; 3 force evaluations",

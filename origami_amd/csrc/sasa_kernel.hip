@@ -1314,69 +1314,36 @@ __global__ void __launch_bounds__(64 * SA_WAVES) og_k_sasa_scatter(og_sasa_data 
     auto vtx_code = [&](int V) -> unsigned {
         return V < 64 ? (unsigned)__builtin_amdgcn_readlane((int)vcode, V) : d.vtx_code[rec.y + V];
     };
+    // Every lane reads the current arc's (vertex's) block of terms, at its
+    // own role or at role 0 when atom x has none there (one 96- or 72-byte
+    // block per step, no divergent loads), and adds the value or +0.0.  The
+    // sums run in arc (vertex) order; a +0.0 can only turn a -0.0 sum into
+    // +0.0, which the zero test below treats alike.
     for (int x = lane; x <= nc; x += 64) {
         double sa0 = 0.0, sa1 = 0.0, sa2 = 0.0, sv0 = 0.0, sv1 = 0.0, sv2 = 0.0;
         bool any = false;
-        // four arcs (vertices) per step: their loads are issued together and
-        // the sums still run in arc (vertex) order
-        for (int A0 = 0; A0 < na; A0 += 4) {
-            double t[4][3];
-            bool hit[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                hit[q] = false;
-                t[q][0] = t[q][1] = t[q][2] = 0.0;
-                if (A0 + q < na) {
-                    const int A = rec.x + A0 + q;
-                    const unsigned xr = arc_code(A0 + q);
-                    int role = -1;
-                    if (x == 0) role = 0;
-                    else if (x == (int)(xr & 0xFF)) role = 1;
-                    else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
-                    else if (x == (int)((xr >> 16) & 0xFF)) role = 3;
-                    if (role >= 0) {
-                        const double* T = d.arc_terms + 12ll * A + 3 * role;
-                        t[q][0] = T[0]; t[q][1] = T[1]; t[q][2] = T[2];
-                        hit[q] = true;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (hit[q]) {
-                    sa0 += t[q][0]; sa1 += t[q][1]; sa2 += t[q][2];
-                    any = true;
-                }
-            }
+        for (int A = 0; A < na; ++A) {
+            const unsigned xr = arc_code(A);
+            const int role = x == 0 ? 0 : x == (int)(xr & 0xFF) ? 1 : x == (int)((xr >> 8) & 0xFF) ? 2
+                           : x == (int)((xr >> 16) & 0xFF) ? 3 : -1;
+            const bool hit = role >= 0;
+            const double* T = d.arc_terms + 12ll * (rec.x + A) + 3 * (hit ? role : 0);
+            const double t0 = T[0], t1 = T[1], t2 = T[2];
+            sa0 += hit ? t0 : 0.0;
+            sa1 += hit ? t1 : 0.0;
+            sa2 += hit ? t2 : 0.0;
+            any = any || hit;
         }
-        for (int V0 = 0; V0 < nv; V0 += 4) {
-            double t[4][3];
-            bool hit[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                hit[q] = false;
-                t[q][0] = t[q][1] = t[q][2] = 0.0;
-                if (V0 + q < nv) {
-                    const int V = rec.y + V0 + q;
-                    const unsigned xr = vtx_code(V0 + q);
-                    int role = -1;
-                    if (x == 0) role = 0;
-                    else if (x == (int)(xr & 0xFF)) role = 1;
-                    else if (x == (int)((xr >> 8) & 0xFF)) role = 2;
-                    if (role >= 0) {
-                        const double* T = d.vtx_terms + 9ll * V + 3 * role;
-                        t[q][0] = T[0]; t[q][1] = T[1]; t[q][2] = T[2];
-                        hit[q] = true;
-                    }
-                }
-            }
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (hit[q]) {
-                    sv0 += t[q][0]; sv1 += t[q][1]; sv2 += t[q][2];
-                    any = true;
-                }
-            }
+        for (int V = 0; V < nv; ++V) {
+            const unsigned xr = vtx_code(V);
+            const int role = x == 0 ? 0 : x == (int)(xr & 0xFF) ? 1 : x == (int)((xr >> 8) & 0xFF) ? 2 : -1;
+            const bool hit = role >= 0;
+            const double* T = d.vtx_terms + 9ll * (rec.y + V) + 3 * (hit ? role : 0);
+            const double t0 = T[0], t1 = T[1], t2 = T[2];
+            sv0 += hit ? t0 : 0.0;
+            sv1 += hit ? t1 : 0.0;
+            sv2 += hit ? t2 : 0.0;
+            any = any || hit;
         }
         if (!any) continue;
         const int atom = x == 0 ? ia : d.owners[rec.z + x - 1];
