@@ -895,8 +895,8 @@ void og_launch_pair_energy64(const og_pair_args& a, const og_energy64_args& e, b
 // one the launches give, bit for bit.  A system that stops (non-finite
 // velocities, a range flag) leaves the loop at that step.
 // ---------------------------------------------------------------------------
-// threads per block of the persistent replica-step kernel (16 waves: every
-// tile pair of a small system's force or descreening pass at once)
+// threads per block of the persistent replica-step kernel: 8 waves, which
+// take the work items of each phase (tile pairs, strips) in turn
 #define OG_PERSIST_THREADS 512
 template <bool MODEA, bool PRE>
 __global__ void __launch_bounds__(OG_PERSIST_THREADS) og_k_replica_steps(og_pair_args pa, og_bonded_args ba, og_radii_args ra,
