@@ -1353,7 +1353,7 @@ extern "C" int og_run(og_engine* e, int32_t n_steps, double* ke_per_step) {
     const size_t n_ke = (size_t)n_steps * e->n_sys;
     if (ke_per_step && n_steps > 0) {
         HIPCHK(hipMallocAsync((void**)&ke_dev, n_ke * sizeof(long long), e->stream));
-        HIPCHK(hipMemsetAsync(ke_dev, 0, n_ke * sizeof(long long), e->stream));
+        og_launch_zero_i64(ke_dev, (long long)n_ke, e->stream);
     }
     if (int rc = run_steps(e, n_steps, ke_dev, true)) {
         if (ke_dev) (void)hipFreeAsync(ke_dev, e->stream);
@@ -1411,7 +1411,7 @@ extern "C" int og_run_noalloc(og_engine* e, int32_t n_steps, double* ke_per_step
             return fail(OG_ERR_ARG, "n_steps x n_systems = %zu exceeds the %d kinetic-energy entries of og_run_noalloc",
                         n_ke, OG_NOALLOC_KE_MAX);
         ke_dev = e->noalloc_ke;
-        HIPCHK(hipMemsetAsync(ke_dev, 0, n_ke * sizeof(long long), e->stream));
+        og_launch_zero_i64(ke_dev, (long long)n_ke, e->stream);
     }
     if (int rc = run_steps(e, n_steps, ke_dev, false)) return rc;
     if (ke_dev) {

@@ -235,6 +235,16 @@ __global__ void __launch_bounds__(256) og_k_back(og_back_args a) {
     back_atom(a, i);
 }
 
+__global__ void __launch_bounds__(256) og_k_zero_i64(long long* p, long long n) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+void og_launch_zero_i64(long long* p, long long n, hipStream_t s) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(og_k_zero_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+}
+
 __global__ void og_k_set_u32(uint32_t* p, uint32_t v) {
     if (threadIdx.x == 0) *p = v;
 }

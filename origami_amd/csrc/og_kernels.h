@@ -178,6 +178,8 @@ void og_launch_cluster_step(const og_cluster_args& a, hipStream_t s);
 void og_launch_zero_acc(long long* facc, long long* iacc, int n_atoms, hipStream_t s);
 void og_launch_back(const og_back_args& a, hipStream_t s);
 void og_launch_set_u32(uint32_t* p, uint32_t v, hipStream_t s);
+// p[0..n) = 0 with a kernel (no memset node when a caller captures the stream)
+void og_launch_zero_i64(long long* p, long long n, hipStream_t s);
 void og_launch_swap(double* pos, double* vel, double* force, int off_a, int off_b, int n,
                     double scale_a, double scale_b, hipStream_t s);
 
