@@ -850,8 +850,13 @@ static void invalidate_graphs(og_engine* e) {
 // formed one thread per atom by the bonded gather kernel, so that the cluster
 // step only reads them.  They stay on the engine stream: a second branch in a
 // captured step costs more than the work.
+// step_mode (direct-launch dynamics with SASA): the cluster kernel already
+// ran on the side stream, which keeps the SASA chain and, after this call,
+// the final kick; the pair stream waits for the positions and the side
+// stream for the pair forces, so no event wait sits on the SASA chain.
 static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double sasa_scale, bool energy,
-                         int sur_check = -1, bool forces = true, const og_noise_args* next_noise = nullptr) {
+                         int sur_check = -1, bool forces = true, const og_noise_args* next_noise = nullptr,
+                         bool step_mode = false) {
     const bool mode_a = flags & OG_MODE_A;
     if (!mode_a) cutoff_a = OG_GB_CUTOFF_A;         // not used by the all-pairs kernels
     if (!forces) sasa_scale = 0.0;
@@ -860,13 +865,19 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
     // atomics only, so the sums do not depend on the interleaving.  The
     // per-kernel profile keeps one stream so its event pairs time each class.
     const bool fork = (flags & OG_SASA) && !e->prof;
+    step_mode = step_mode && fork;
     if (fork) {
-        HIPCHK(hipEventRecord(e->fork_ev, e->stream));
-        HIPCHK(hipStreamWaitEvent(e->side, e->fork_ev, 0));
+        if (step_mode) {
+            HIPCHK(hipEventRecord(e->fork_ev, e->side));      // positions of this step
+            HIPCHK(hipStreamWaitEvent(e->stream, e->fork_ev, 0));
+        } else {
+            HIPCHK(hipEventRecord(e->fork_ev, e->stream));
+            HIPCHK(hipStreamWaitEvent(e->side, e->fork_ev, 0));
+        }
         const int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->side,
                                       energy ? e->eacc : nullptr, e->atom_sys, e->sflag);
         if (rc != OG_OK) return fail(rc, "SASA launch failed: %s", og_sasa_error());
-        HIPCHK(hipEventRecord(e->join_ev, e->side));
+        if (!step_mode) HIPCHK(hipEventRecord(e->join_ev, e->side));
     }
     og_sur* sur = (flags & OG_SURROGATE) ? e->sur : nullptr;
     // Small batches (the LDS gather's range): the bonded terms share one
@@ -951,7 +962,12 @@ static int launch_forces(og_engine* e, uint32_t flags, double cutoff_a, double s
             }
         }
     }
-    if (fork) HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
+    if (step_mode) {
+        HIPCHK(hipEventRecord(e->join_ev, e->stream));         // pair forces of this step
+        HIPCHK(hipStreamWaitEvent(e->side, e->join_ev, 0));
+    } else if (fork) {
+        HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
+    }
     if (!fork && (flags & OG_SASA)) {
         ProfScope p(e, OG_K_SASA);
         int rc = og_sasa_launch(&e->sasa, e->pos, e->facc, sasa_scale, e->stream,
@@ -1103,20 +1119,37 @@ extern "C" int og_set_temperature(og_engine* e, int32_t s, double t) {
     return OG_OK;
 }
 
+// side_steps: the cluster kernel and the final kick on the side stream with
+// the SASA chain between them (direct launches with SASA; the caller brackets
+// the steps with a fork to and a join from the side stream).
 static int record_step(og_engine* e, const og_cluster_args& c, const og_back_args& b, int sur_check,
-                       const og_noise_args* next_noise) {
+                       const og_noise_args* next_noise, bool side_steps = false) {
     const og_dynamics_params& p = e->dyn;
+    hipStream_t is = side_steps ? e->side : e->stream;
     {
         ProfScope ps(e, OG_K_CLUSTER);
-        og_launch_cluster_step(c, e->stream);
+        og_launch_cluster_step(c, is);
     }
     int rc = launch_forces(e, p.flags & ~OG_ENERGY, p.cutoff_a, p.sasa_gamma_scale, false, sur_check, true,
-                           next_noise);
+                           next_noise, side_steps);
     if (rc) return rc;
     {
         ProfScope ps(e, OG_K_BACK);
-        og_launch_back(b, e->stream);
+        og_launch_back(b, is);
     }
+    return OG_OK;
+}
+
+// side_fork: the side stream waits for the work enqueued on the engine stream
+// so far; side_join: the engine stream waits for the side stream's.
+static int side_fork(og_engine* e) {
+    HIPCHK(hipEventRecord(e->fork_ev, e->stream));
+    HIPCHK(hipStreamWaitEvent(e->side, e->fork_ev, 0));
+    return OG_OK;
+}
+static int side_join(og_engine* e) {
+    HIPCHK(hipEventRecord(e->join_ev, e->side));
+    HIPCHK(hipStreamWaitEvent(e->stream, e->join_ev, 0));
     return OG_OK;
 }
 
@@ -1292,17 +1325,31 @@ static int run_steps(og_engine* e, int32_t n_steps, long long* ke_dev, bool allo
                 if (int rc = save_frame(e, st)) return rc;
         }
     } else {
+        // with SASA the integrator runs on the SASA (critical) stream
+        const bool side_steps = (p.flags & OG_SASA) && !e->prof && n_steps > 0;
+        if (side_steps)
+            if (int rc = side_fork(e)) return rc;
         for (int k = 0; k < n_steps; ++k) {
             b.ke_acc = ke_dev ? ke_dev + (size_t)k * e->n_sys : nullptr;
-            if (int rc = record_step(e, c, b, -1, nzp)) return rc;
+            if (int rc = record_step(e, c, b, -1, nzp, side_steps)) {
+                if (side_steps) (void)side_join(e);
+                return rc;
+            }
             const uint32_t st = e->step + (uint32_t)k + 1u;
-            if (e->frame_every > 0 && st % (uint32_t)e->frame_every == 0)
+            if (e->frame_every > 0 && st % (uint32_t)e->frame_every == 0) {
+                if (side_steps)
+                    if (int rc = side_join(e)) return rc;
                 if (int rc = save_frame(e, st)) return rc;
+                if (side_steps)
+                    if (int rc = side_fork(e)) return rc;
+            }
             if (e->prof && (k & 63) == 63) {
                 HIPCHK(hipStreamSynchronize(e->stream));
                 prof_collect(e);
             }
         }
+        if (side_steps)
+            if (int rc = side_join(e)) return rc;
     }
     e->step += (uint32_t)n_steps;
     HIPCHK(hipGetLastError());
